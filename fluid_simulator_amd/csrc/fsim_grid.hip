@@ -1,0 +1,594 @@
+// fsim_grid.hip — grid-side stages of the step on gfx950.
+//
+// All grid kernels run over the dense cell array in the reference's flat
+// order (z fastest).  P2G is a deterministic gather over the particles
+// binned by cell (no float atomics): each cell's three faces sum the
+// contributions of the particles of its 3x3x3 cell neighbourhood in a fixed
+// order (neighbour cells in x,y,z order, particles by id inside a cell).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fsim_internal.hpp"
+
+namespace fsim {
+
+// ---- fills (graph-capture safe replacements for hipMemsetAsync) -------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill(T* __restrict__ p, T v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+template <typename T>
+static int fill(fsim_ctx* ctx, T* p, T v, int64_t n) {
+    if (n <= 0) return FSIM_OK;
+    unsigned nb = (unsigned)std::min<int64_t>(nblocks(n), 65536);
+    hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kBlock), 0, ctx->stream, p, v, n);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+int fill_u8(fsim_ctx* ctx, uint8_t* p, uint8_t v, int64_t n) { return fill(ctx, p, v, n); }
+int fill_f32(fsim_ctx* ctx, float* p, float v, int64_t n) { return fill(ctx, p, v, n); }
+int fill_f64(fsim_ctx* ctx, double* p, double v, int64_t n) { return fill(ctx, p, v, n); }
+int fill_i32(fsim_ctx* ctx, int32_t* p, int32_t v, int64_t n) { return fill(ctx, p, v, n); }
+
+struct GridArgs {
+    int nx, ny, nz;
+    int64_t nc;
+    double cd, cdinv;
+    float cdf, invf;
+};
+static GridArgs grid_args(const fsim_ctx* ctx) {
+    GridArgs G;
+    G.nx = ctx->g.nx;
+    G.ny = ctx->g.ny;
+    G.nz = ctx->g.nz;
+    G.nc = ctx->g.nc;
+    G.cd = ctx->g.cd;
+    G.cdinv = ctx->g.cdinv;
+    G.cdf = (float)ctx->g.cd;
+    G.invf = (float)ctx->g.cdinv;
+    return G;
+}
+
+__device__ inline int64_t cidx(const GridArgs& G, int x, int y, int z) { return ((int64_t)x * G.ny + y) * G.nz + z; }
+
+// Base cell of the 8 faces of axis `a` around a particle, getFacesAround
+// (macGrid.cpp:142-148): ivec3(pos*cellDInv - offset_a), then coord[a] -= 1.
+// Evaluated in FP64 so stencil membership is decided exactly as in FP64.
+__device__ inline void face_base(const GridArgs& G, float px, float py, float pz, int a, int b[3]) {
+    const double p[3] = {(double)px * G.cdinv, (double)py * G.cdinv, (double)pz * G.cdinv};
+#pragma unroll
+    for (int d = 0; d < 3; d++) b[d] = (int)(p[d] - (d == a ? 0.0 : 0.5));
+    b[a] -= 1;
+}
+
+// Face centre of axis a of cell (x,y,z) (macGrid.cpp:25-27), rounded to FP32.
+__device__ inline void face_pos(const GridArgs& G, int a, int x, int y, int z, float f[3]) {
+    f[0] = (float)((x + (a == 0 ? 1.0 : 0.5)) * G.cd);
+    f[1] = (float)((y + (a == 1 ? 1.0 : 0.5)) * G.cd);
+    f[2] = (float)((z + (a == 2 ? 1.0 : 0.5)) * G.cd);
+}
+
+// trilinearInterpoll, interpolation.h:7-20
+__device__ inline float tent3(const float f[3], float px, float py, float pz, float inv) {
+    float x = 1.0f - fabsf((px - f[0]) * inv);
+    float y = 1.0f - fabsf((py - f[1]) * inv);
+    float z = 1.0f - fabsf((pz - f[2]) * inv);
+    return x * y * z;
+}
+
+// ---- P2G: resetGridValues + p2gTransfer (simulator.cpp:314-352) ------------------
+struct P2GArgs {
+    GridArgs G;
+    const float* pos[3];
+    const float* vel[3];
+    const float* c[9];
+    const int32_t* start;
+    const int32_t* count;
+    float* uvw[3];
+    float* uvw2[3];
+    uint8_t* label;
+    float* density;
+    int apic;
+    int fuse_mark;   // 1: also label WATER where the bin count is > 0
+};
+
+__global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
+    const GridArgs& G = A.G;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.nc) return;
+    const int z = (int)(i % G.nz);
+    const int y = (int)((i / G.nz) % G.ny);
+    const int x = (int)(i / ((int64_t)G.nz * G.ny));
+    float fpos[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) face_pos(G, a, x, y, z, fpos[a]);
+    float swv[3] = {0.f, 0.f, 0.f}, sw[3] = {0.f, 0.f, 0.f};
+    for (int dx = -1; dx <= 1; dx++) {
+        int cx = x + dx;
+        if (cx < 0 || cx >= G.nx) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int cy = y + dy;
+            if (cy < 0 || cy >= G.ny) continue;
+            for (int dz = -1; dz <= 1; dz++) {
+                int cz = z + dz;
+                if (cz < 0 || cz >= G.nz) continue;
+                int64_t n = cidx(G, cx, cy, cz);
+                int b = A.start[n], e = A.start[n + 1];
+                for (int j = b; j < e; j++) {
+                    const float px = A.pos[0][j], py = A.pos[1][j], pz = A.pos[2][j];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        int bs[3];
+                        face_base(G, px, py, pz, a, bs);
+                        unsigned ox = (unsigned)(x - bs[0]), oy = (unsigned)(y - bs[1]), oz = (unsigned)(z - bs[2]);
+                        if (ox > 1u || oy > 1u || oz > 1u) continue;
+                        float w = tent3(fpos[a], px, py, pz, G.invf);
+                        float va = A.vel[a][j];
+                        if (A.apic) {
+                            // (v_a + c[a] . (face - p)) * w, simulator.cpp:326-328
+                            float d0 = fpos[a][0] - px, d1 = fpos[a][1] - py, d2 = fpos[a][2] - pz;
+                            va = va + (A.c[3 * a][j] * d0 + A.c[3 * a + 1][j] * d1 + A.c[3 * a + 2][j] * d2);
+                        }
+                        swv[a] += va * w;
+                        sw[a] += w;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        A.uvw[a][i] = sw[a] > 1e-6f ? swv[a] / sw[a] : 0.0f;
+        A.uvw2[a][i] = 0.0f;
+    }
+    A.label[i] = (A.fuse_mark && A.count[i] > 0) ? (uint8_t)FSIM_WATER : (uint8_t)FSIM_AIR;
+    A.density[i] = 0.0f;
+}
+
+static int p2g(fsim_ctx* ctx, int fuse_mark) {
+    if (ctx->prm.transfer == FSIM_APIC && !ctx->c_valid) {
+        // APIC after PIC/FLIP steps: the matrices were not carried, start from 0
+        FSIM_TRY(fill_f32(ctx, ctx->P.c[0], 0.0f, (int64_t)ctx->P.cap * 9));
+        ctx->c_valid = true;
+    }
+    P2GArgs A{};
+    A.G = grid_args(ctx);
+    for (int a = 0; a < 3; a++) {
+        A.pos[a] = ctx->P.pos[a];
+        A.vel[a] = ctx->P.vel[a];
+    }
+    for (int k = 0; k < 9; k++) A.c[k] = ctx->P.c[k];
+    A.start = ctx->cell_start;
+    A.count = ctx->cell_count;
+    A.uvw[0] = ctx->u; A.uvw[1] = ctx->v; A.uvw[2] = ctx->w;
+    A.uvw2[0] = ctx->u2; A.uvw2[1] = ctx->v2; A.uvw2[2] = ctx->w2;
+    A.label = ctx->label;
+    A.density = ctx->density;
+    A.apic = ctx->prm.transfer == FSIM_APIC;
+    A.fuse_mark = fuse_mark;
+    hipLaunchKernelGGL(k_p2g_gather, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+int stage_p2g(fsim_ctx* ctx) { return p2g(ctx, 0); }
+int stage_p2g_fused(fsim_ctx* ctx) { return p2g(ctx, 1); }
+
+// ---- markFluidCellsAndCalculateParticleDensities (simulator.cpp:354-368) --------
+// WATER where the cell holds a particle (the bin key is the same
+// ivec3(pos*cellDInv)), density = sum of cell-centre tent weights gathered from
+// the 3x3x3 neighbourhood with the getCellsAround stencil (macGrid.cpp:188-202).
+__global__ void __launch_bounds__(kBlock) k_mark(P2GArgs A, int with_density) {
+    const GridArgs& G = A.G;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.nc) return;
+    if (A.count[i] > 0) A.label[i] = FSIM_WATER;
+    if (!with_density) return;
+    const int z = (int)(i % G.nz);
+    const int y = (int)((i / G.nz) % G.ny);
+    const int x = (int)(i / ((int64_t)G.nz * G.ny));
+    const float cp[3] = {(float)((x + 0.5) * G.cd), (float)((y + 0.5) * G.cd), (float)((z + 0.5) * G.cd)};
+    float sum = 0.f;
+    for (int dx = -1; dx <= 1; dx++) {
+        int cx = x + dx;
+        if (cx < 0 || cx >= G.nx) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int cy = y + dy;
+            if (cy < 0 || cy >= G.ny) continue;
+            for (int dz = -1; dz <= 1; dz++) {
+                int cz = z + dz;
+                if (cz < 0 || cz >= G.nz) continue;
+                int64_t n = cidx(G, cx, cy, cz);
+                int b = A.start[n], e = A.start[n + 1];
+                for (int j = b; j < e; j++) {
+                    const float px = A.pos[0][j], py = A.pos[1][j], pz = A.pos[2][j];
+                    unsigned ox = (unsigned)(x - (int)((double)px * G.cdinv - 0.5));
+                    unsigned oy = (unsigned)(y - (int)((double)py * G.cdinv - 0.5));
+                    unsigned oz = (unsigned)(z - (int)((double)pz * G.cdinv - 0.5));
+                    if (ox > 1u || oy > 1u || oz > 1u) continue;
+                    sum += tent3(cp, px, py, pz, G.invf);
+                }
+            }
+        }
+    }
+    A.density[i] += sum;
+}
+
+int stage_mark(fsim_ctx* ctx) {
+    P2GArgs A{};
+    A.G = grid_args(ctx);
+    for (int a = 0; a < 3; a++) A.pos[a] = ctx->P.pos[a];
+    A.start = ctx->cell_start;
+    A.count = ctx->cell_count;
+    A.label = ctx->label;
+    A.density = ctx->density;
+    hipLaunchKernelGGL(k_mark, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A,
+                       ctx->prm.pressure_enabled ? 1 : 0);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// ---- addObstaclesToGrid / MacGrid::addObstacle (macGrid.cpp:231-292) --------------
+// The reference visits the obstacle's bounding box serially in x,y,z order and
+// sets the face velocity towards each WATER neighbour, reading the labels as
+// they are at that moment: a +neighbour is visited later, so it still has its
+// label from before this obstacle; a -neighbour inside this obstacle's box
+// and shape has already been made SOLID.  The face pass below reads the
+// labels from before the obstacle and applies exactly that rule; the label
+// pass runs after it.
+struct ObsArgs {
+    GridArgs G;
+    int mn[3], ext[3];       // first cell and extent of the visited box
+    int sphere;
+    double ctr[3], r2;
+    float speed[3];
+    uint8_t* label;
+    float* uvw[3];
+};
+
+__device__ inline bool obs_inside(const ObsArgs& A, int x, int y, int z) {
+#pragma clang fp contract(off)
+    if (x < A.mn[0] || y < A.mn[1] || z < A.mn[2] || x >= A.mn[0] + A.ext[0] || y >= A.mn[1] + A.ext[1] ||
+        z >= A.mn[2] + A.ext[2])
+        return false;
+    if (!A.sphere) return true;
+    double c0 = x + 0.5 - A.ctr[0], c1 = y + 0.5 - A.ctr[1], c2 = z + 0.5 - A.ctr[2];
+    return c0 * c0 + c1 * c1 + c2 * c2 < A.r2;
+}
+
+__global__ void __launch_bounds__(kBlock) k_obstacle_faces(ObsArgs A) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t vol = (int64_t)A.ext[0] * A.ext[1] * A.ext[2];
+    if (t >= vol) return;
+    int z = A.mn[2] + (int)(t % A.ext[2]);
+    int y = A.mn[1] + (int)((t / A.ext[2]) % A.ext[1]);
+    int x = A.mn[0] + (int)(t / ((int64_t)A.ext[2] * A.ext[1]));
+    if (!obs_inside(A, x, y, z)) return;
+    const GridArgs& G = A.G;
+    int64_t i = cidx(G, x, y, z);
+    const int64_t st[3] = {(int64_t)G.ny * G.nz, G.nz, 1};
+    const int p[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (A.label[i + st[a]] == FSIM_WATER) A.uvw[a][i] = A.speed[a];
+        int m[3] = {p[0], p[1], p[2]};
+        m[a] -= 1;
+        if (A.label[i - st[a]] == FSIM_WATER && !obs_inside(A, m[0], m[1], m[2])) A.uvw[a][i - st[a]] = A.speed[a];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_obstacle_labels(ObsArgs A) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t vol = (int64_t)A.ext[0] * A.ext[1] * A.ext[2];
+    if (t >= vol) return;
+    int z = A.mn[2] + (int)(t % A.ext[2]);
+    int y = A.mn[1] + (int)((t / A.ext[2]) % A.ext[1]);
+    int x = A.mn[0] + (int)(t / ((int64_t)A.ext[2] * A.ext[1]));
+    if (obs_inside(A, x, y, z)) A.label[cidx(A.G, x, y, z)] = FSIM_SOLID;
+}
+
+int stage_obstacles(fsim_ctx* ctx) {
+    const int n[3] = {ctx->g.nx, ctx->g.ny, ctx->g.nz};
+    for (const fsim_obstacle& o : ctx->obs) {
+        ObsArgs A{};
+        A.G = grid_args(ctx);
+        A.label = ctx->label;
+        A.uvw[0] = ctx->u; A.uvw[1] = ctx->v; A.uvw[2] = ctx->w;
+        for (int a = 0; a < 3; a++) {
+            A.ctr[a] = o.pos[a] * ctx->g.cdinv;
+            A.speed[a] = (float)o.speed[a];
+        }
+        int mx[3];
+        if (o.kind == FSIM_OBS_SPHERE || o.kind == FSIM_OBS_SOURCE) {
+            double r = ctx->g.cdinv * o.r;
+            A.r2 = r * r;
+            A.sphere = 1;
+            for (int a = 0; a < 3; a++) {
+                A.mn[a] = (int)std::fmax(A.ctr[a] - r, 1.0);
+                mx[a] = (int)std::fmin(A.ctr[a] + r, n[a] - 2.0);   // inclusive
+            }
+        } else if (o.kind == FSIM_OBS_BOX) {
+            A.sphere = 0;
+            for (int a = 0; a < 3; a++) {
+                double d = o.size[a] * ctx->g.cdinv * 0.5;
+                A.mn[a] = (int)std::fmax(std::round(A.ctr[a] - d), 1.0);
+                mx[a] = (int)std::fmin(std::round(A.ctr[a] + d), n[a] - 1.0) - 1;   // exclusive -> inclusive
+            }
+        } else {
+            continue;   // sinks are not voxelised
+        }
+        int64_t vol = 1;
+        for (int a = 0; a < 3; a++) {
+            A.ext[a] = mx[a] - A.mn[a] + 1;
+            if (A.ext[a] <= 0) vol = 0;
+        }
+        if (vol == 0) continue;
+        vol = (int64_t)A.ext[0] * A.ext[1] * A.ext[2];
+        hipLaunchKernelGGL(k_obstacle_faces, dim3(nblocks(vol)), dim3(kBlock), 0, ctx->stream, A);
+        hipLaunchKernelGGL(k_obstacle_labels, dim3(nblocks(vol)), dim3(kBlock), 0, ctx->stream, A);
+        FSIM_HIP(ctx, hipGetLastError());
+    }
+    return FSIM_OK;
+}
+
+// ---- restoreBorderingSolidCellsAndSpeeds (macGrid.cpp:75-140) ---------------------
+// Face pass with the labels from before the pass, reproducing which border
+// cells the reference has already made SOLID when it tests a neighbour:
+// x-walls first; then, per x, the y-walls (and top) before the z-walls.
+struct BorderArgs {
+    GridArgs G;
+    const uint8_t* label;
+    float* uvw[3];
+    int top;
+};
+
+__global__ void __launch_bounds__(kBlock) k_border_faces(BorderArgs A) {
+    const GridArgs& G = A.G;
+    const int X = G.nx, Y = G.ny, Z = G.nz;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int wall = blockIdx.y;
+    if (wall == 0) {   // x-walls, loop over (y, z): labels as before the pass
+        if (t >= (int64_t)Y * Z) return;
+        int y = (int)(t / Z), z = (int)(t % Z);
+        if (A.label[cidx(G, 1, y, z)] == FSIM_WATER) A.uvw[0][cidx(G, 0, y, z)] = 0.f;
+        if (A.label[cidx(G, X - 2, y, z)] == FSIM_WATER) A.uvw[0][cidx(G, X - 2, y, z)] = 0.f;
+    } else if (wall == 1) {   // bottom (and top) y-walls, loop over (x, z)
+        if (t >= (int64_t)X * Z) return;
+        int x = (int)(t / Z), z = (int)(t % Z);
+        const bool xb = x == 0 || x == X - 1;
+        uint8_t l1 = xb ? (uint8_t)FSIM_SOLID : A.label[cidx(G, x, 1, z)];
+        if (l1 == FSIM_WATER) A.uvw[1][cidx(G, x, 0, z)] = 0.f;
+        if (A.top) {
+            uint8_t l2 = xb ? (uint8_t)FSIM_SOLID : A.label[cidx(G, x, Y - 2, z)];
+            if (l2 == FSIM_WATER) A.uvw[1][cidx(G, x, Y - 2, z)] = 0.f;
+        }
+    } else {   // z-walls, loop over (x, y) after this x's y-walls
+        if (t >= (int64_t)X * Y) return;
+        int x = (int)(t / Y), y = (int)(t % Y);
+        const bool done = x == 0 || x == X - 1 || y == 0 || (A.top && y == Y - 1);
+        uint8_t l1 = done ? (uint8_t)FSIM_SOLID : A.label[cidx(G, x, y, 1)];
+        uint8_t l2 = done ? (uint8_t)FSIM_SOLID : A.label[cidx(G, x, y, Z - 2)];
+        if (l1 == FSIM_WATER) A.uvw[2][cidx(G, x, y, 0)] = 0.f;
+        if (l2 == FSIM_WATER) A.uvw[2][cidx(G, x, y, Z - 2)] = 0.f;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_border_labels(GridArgs G, uint8_t* label, int top) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int X = G.nx, Y = G.ny, Z = G.nz;
+    const int wall = blockIdx.y;
+    if (wall == 0) {
+        if (t >= (int64_t)Y * Z) return;
+        int y = (int)(t / Z), z = (int)(t % Z);
+        label[cidx(G, 0, y, z)] = FSIM_SOLID;
+        label[cidx(G, X - 1, y, z)] = FSIM_SOLID;
+    } else if (wall == 1) {
+        if (t >= (int64_t)X * Z) return;
+        int x = (int)(t / Z), z = (int)(t % Z);
+        label[cidx(G, x, 0, z)] = FSIM_SOLID;
+        if (top) label[cidx(G, x, Y - 1, z)] = FSIM_SOLID;
+    } else {
+        if (t >= (int64_t)X * Y) return;
+        int x = (int)(t / Y), y = (int)(t % Y);
+        label[cidx(G, x, y, 0)] = FSIM_SOLID;
+        label[cidx(G, x, y, Z - 1)] = FSIM_SOLID;
+    }
+}
+
+int stage_borders(fsim_ctx* ctx) {
+    BorderArgs A{};
+    A.G = grid_args(ctx);
+    A.label = ctx->label;
+    A.uvw[0] = ctx->u; A.uvw[1] = ctx->v; A.uvw[2] = ctx->w;
+    A.top = ctx->prm.top_solid ? 1 : 0;
+    int64_t m = std::max({(int64_t)ctx->g.ny * ctx->g.nz, (int64_t)ctx->g.nx * ctx->g.nz, (int64_t)ctx->g.nx * ctx->g.ny});
+    hipLaunchKernelGGL(k_border_faces, dim3(nblocks(m), 3), dim3(kBlock), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_border_labels, dim3(nblocks(m), 3), dim3(kBlock), 0, ctx->stream, A.G, ctx->label, A.top);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// ---- postP2GUpdate (macGrid.cpp:204-219) -----------------------------------------
+// v2 = v, plus gravity on the y-face when the cell and cell<1,1> are not
+// SOLID.  cell<1,1> is the flat index +Nz, which for the top row is the
+// SOLID border cell (x+1, 0, z): top-row y-faces get no gravity.
+__global__ void __launch_bounds__(kBlock) k_post_p2g(GridArgs G, const float* __restrict__ u, const float* __restrict__ v,
+                                                     const float* __restrict__ w, float* __restrict__ u2,
+                                                     float* __restrict__ v2, float* __restrict__ w2,
+                                                     const uint8_t* __restrict__ label, float g) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.nc) return;
+    int64_t nb = i + G.nz;
+    bool grav = label[i] != FSIM_SOLID && nb < G.nc && label[nb] != FSIM_SOLID;
+    u2[i] = u[i];
+    v2[i] = grav ? v[i] + g : v[i];
+    w2[i] = w[i];
+}
+
+int stage_post_p2g(fsim_ctx* ctx, double grav_inc) {
+    hipLaunchKernelGGL(k_post_p2g, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, grid_args(ctx), ctx->u,
+                       ctx->v, ctx->w, ctx->u2, ctx->v2, ctx->w2, ctx->label, (float)grav_inc);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// ---- extrapolateVelocities (macGrid.cpp:294-336) -----------------------------------
+// Two layers.  A cell that is not yet valid takes, for each of its three
+// faces, the mean of the same-component face of its valid neighbours (a face
+// is kept when its + neighbour is WATER or outside the grid).  Within one
+// layer the cells written are not valid and the cells read are valid, so the
+// pass is race-free in place.
+__global__ void __launch_bounds__(kBlock) k_valid_init(GridArgs G, const uint8_t* __restrict__ label, uint8_t* __restrict__ valid) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.nc) return;
+    const int z = (int)(i % G.nz);
+    const int y = (int)((i / G.nz) % G.ny);
+    const int x = (int)(i / ((int64_t)G.nz * G.ny));
+    bool interior = x > 0 && y > 0 && z > 0 && x < G.nx - 1 && y < G.ny - 1 && z < G.nz - 1;
+    valid[i] = (interior && label[i] == FSIM_WATER) ? 0 : 100;
+}
+
+__global__ void __launch_bounds__(kBlock) k_extrapolate(GridArgs G, float* __restrict__ u2, float* __restrict__ v2,
+                                                        float* __restrict__ w2, const uint8_t* __restrict__ label,
+                                                        uint8_t* __restrict__ valid, int it) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.nc) return;
+    if (valid[i] <= it) return;
+    const int z = (int)(i % G.nz);
+    const int y = (int)((i / G.nz) % G.ny);
+    const int x = (int)(i / ((int64_t)G.nz * G.ny));
+    const int64_t st[3] = {(int64_t)G.ny * G.nz, G.nz, 1};
+    const int p[3] = {x, y, z};
+    const int n[3] = {G.nx, G.ny, G.nz};
+    int cnt = 0;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int off = 0; off < 2; off++) {
+            int q = p[a] + off * 2 - 1;
+            if (q < 0 || q >= n[a]) continue;
+            int64_t j = i + (off ? st[a] : -st[a]);
+            if (valid[j] > it) continue;
+            s0 += u2[j];
+            s1 += v2[j];
+            s2 += w2[j];
+            cnt++;
+        }
+    if (cnt > 0) {
+        float fc = (float)cnt;
+        if (x + 1 < G.nx && label[i + st[0]] != FSIM_WATER) u2[i] = s0 / fc;
+        if (y + 1 < G.ny && label[i + st[1]] != FSIM_WATER) v2[i] = s1 / fc;
+        if (z + 1 < G.nz && label[i + st[2]] != FSIM_WATER) w2[i] = s2 / fc;
+        valid[i] = (uint8_t)(it + 1);
+    }
+}
+
+int stage_extrapolate(fsim_ctx* ctx) {
+    GridArgs G = grid_args(ctx);
+    hipLaunchKernelGGL(k_valid_init, dim3(nblocks(G.nc)), dim3(kBlock), 0, ctx->stream, G, ctx->label, ctx->valid);
+    for (int it = 0; it < 2; it++)
+        hipLaunchKernelGGL(k_extrapolate, dim3(nblocks(G.nc)), dim3(kBlock), 0, ctx->stream, G, ctx->u2, ctx->v2, ctx->w2,
+                           ctx->label, ctx->valid, it);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// ---- g2pTransfer (simulator.cpp:375-418) --------------------------------------------
+struct G2PArgs {
+    GridArgs G;
+    float* pos[3];
+    float* vel[3];
+    float* c[9];
+    const float* uvw[3];
+    const float* uvw2[3];
+    int64_t n;
+    int transfer;
+    float flip, one_minus_flip;
+};
+
+__global__ void __launch_bounds__(kBlock) k_g2p(G2PArgs A) {
+    const GridArgs& G = A.G;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const float px = A.pos[0][i], py = A.pos[1][i], pz = A.pos[2][i];
+    const float inv = G.invf;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int b[3];
+        face_base(G, px, py, pz, a, b);
+        // stay inside the grid for positions outside the clamp box (the step
+        // never produces them; the reference would read out of bounds)
+        b[0] = min(max(b[0], 0), G.nx - 2);
+        b[1] = min(max(b[1], 0), G.ny - 2);
+        b[2] = min(max(b[2], 0), G.nz - 2);
+        // tent factors of the two face rows per dimension
+        float tx[2], ty[2], tz[2], dxs[2], dys[2], dzs[2];
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+            float fx = (float)((b[0] + o + (a == 0 ? 1.0 : 0.5)) * G.cd);
+            float fy = (float)((b[1] + o + (a == 1 ? 1.0 : 0.5)) * G.cd);
+            float fz = (float)((b[2] + o + (a == 2 ? 1.0 : 0.5)) * G.cd);
+            dxs[o] = (px - fx) * inv;
+            dys[o] = (py - fy) * inv;
+            dzs[o] = (pz - fz) * inv;
+            tx[o] = 1.0f - fabsf(dxs[o]);
+            ty[o] = 1.0f - fabsf(dys[o]);
+            tz[o] = 1.0f - fabsf(dzs[o]);
+        }
+        float pic = 0.f, flip = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
+#pragma unroll
+        for (int o = 0; o < 8; o++) {
+            const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
+            int64_t f = cidx(G, b[0] + ox, b[1] + oy, b[2] + oz);
+            float w = tx[ox] * ty[oy] * tz[oz];
+            float v2 = A.uvw2[a][f];
+            pic += v2 * w;
+            if (A.transfer == FSIM_FLIP) flip += (v2 - A.uvw[a][f]) * w;
+            if (A.transfer == FSIM_APIC) {
+                // trilinearInterpollGradient, interpolation.h:23-35 (sign(0) = +1)
+                float sx = dxs[ox] > 0.f ? -1.f : 1.f, sy = dys[oy] > 0.f ? -1.f : 1.f, sz = dzs[oz] > 0.f ? -1.f : 1.f;
+                g0 += sx * ty[oy] * tz[oz] * inv * v2;
+                g1 += sy * tx[ox] * tz[oz] * inv * v2;
+                g2 += sz * tx[ox] * ty[oy] * inv * v2;
+            }
+        }
+        if (A.transfer == FSIM_PIC) {
+            A.vel[a][i] = pic;
+        } else if (A.transfer == FSIM_FLIP) {
+            A.vel[a][i] = pic * A.one_minus_flip + (flip + A.vel[a][i]) * A.flip;
+        } else {
+            A.vel[a][i] = pic;
+            A.c[3 * a][i] = g0;
+            A.c[3 * a + 1][i] = g1;
+            A.c[3 * a + 2][i] = g2;
+        }
+    }
+}
+
+int stage_g2p(fsim_ctx* ctx) {
+    if (ctx->P.n == 0) return FSIM_OK;
+    G2PArgs A{};
+    A.G = grid_args(ctx);
+    for (int a = 0; a < 3; a++) {
+        A.pos[a] = ctx->P.pos[a];
+        A.vel[a] = ctx->P.vel[a];
+    }
+    for (int k = 0; k < 9; k++) A.c[k] = ctx->P.c[k];
+    A.uvw[0] = ctx->u; A.uvw[1] = ctx->v; A.uvw[2] = ctx->w;
+    A.uvw2[0] = ctx->u2; A.uvw2[1] = ctx->v2; A.uvw2[2] = ctx->w2;
+    A.n = ctx->P.n;
+    A.transfer = ctx->prm.transfer;
+    A.flip = ctx->prm.flip_ratio;
+    A.one_minus_flip = 1.0f - ctx->prm.flip_ratio;
+    if (A.transfer == FSIM_APIC) ctx->c_valid = true;
+    hipLaunchKernelGGL(k_g2p, dim3(nblocks(A.n)), dim3(kBlock), 0, ctx->stream, A);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+}  // namespace fsim

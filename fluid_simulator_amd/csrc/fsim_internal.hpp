@@ -1,0 +1,163 @@
+// fsim_internal.hpp — device state and launch interfaces of libfsim_hip.so.
+//
+// Device layout (one context, one GPU):
+//   * grid fields are dense arrays over all Nc = Nx*Ny*Nz cells in the
+//     reference's flat order idx = (x*Ny + y)*Nz + z (z fastest, so a
+//     wavefront's lanes run along z and their accesses coalesce);
+//   * face velocities are FP32 SoA: u,v,w (P2G result after boundary
+//     conditions, the reference's Face::v) and u2,v2,w2 (Face::v2);
+//   * labels are uint8 fsim_cell_type; nbinfo (uint16) caches, per WATER cell,
+//     which of the 6 neighbours are WATER and how many are not SOLID;
+//   * pressure-solve vectors are dense FP32 (b, r, z, s, q, y) with FP64
+//     pressure x and FP64 MIC(0) diagonal; entries outside WATER cells are 0;
+//   * particles are FP32 SoA (pos, vel, APIC C) plus the int32 particle id
+//     (its index in the reference's particle vector), kept sorted by cell
+//     after every BIN stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fsim.h"
+
+namespace fsim {
+
+constexpr int kMaxObstacles = 64;
+constexpr int kBlock = 256;
+
+// nbinfo bit layout
+constexpr uint16_t NB_FLUID = 1u << 0;      // the cell itself is an interior WATER cell
+constexpr uint16_t NB_MX = 1u << 1;         // -x neighbour is WATER (fluid)
+constexpr uint16_t NB_MY = 1u << 2;
+constexpr uint16_t NB_MZ = 1u << 3;
+constexpr uint16_t NB_PX = 1u << 4;         // +x neighbour is WATER (fluid)
+constexpr uint16_t NB_PY = 1u << 5;
+constexpr uint16_t NB_PZ = 1u << 6;
+constexpr int NB_NONSOLID_SHIFT = 7;        // bits 7..9: number of non-SOLID neighbours
+
+struct GridDims {
+    int nx, ny, nz;
+    int64_t nc;
+    double cd, cdinv;
+    double dims[3];
+    double pr;
+};
+
+// Device-resident scalars of the pressure solve (one cache line each group).
+struct SolveScalars {
+    double sigma, alpha, beta, sq;   // PCG scalars
+    double rr, rmax, bb;             // recursive residual ||r||^2, max|r|, ||b||^2
+    double tr_ss, tr_max;            // true residual ||b-Ax||^2, max|b-Ax|
+    double zr;                       // z.r
+    int nan_flag;
+    int pad;
+};
+
+struct Particles {
+    int64_t n = 0, cap = 0;
+    float* base = nullptr;   // one allocation, 16 float planes of cap entries
+    float* pos[3] = {};
+    float* vel[3] = {};
+    float* c[9] = {};
+    int32_t* pid = nullptr;
+};
+
+}  // namespace fsim
+
+struct fsim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    fsim::GridDims g{};
+    fsim_params prm{};
+    std::string err;
+
+    // particles (double-buffered for the binning gather)
+    fsim::Particles P, Palt;
+    uint32_t* key = nullptr;     // cell key per particle (Nc = removed)
+    int32_t* rank = nullptr;     // slot within cell from the histogram atomics
+    int32_t* perm = nullptr;     // sorted slot -> source slot
+    int64_t pwork_cap = 0;
+    uint8_t* removed = nullptr;  // sink removal flags (per particle)
+    bool c_valid = true;         // APIC matrices carried (false after a PIC/FLIP bin)
+    int64_t scan_tmp_cap = 0;
+
+    // grid
+    float *u = nullptr, *v = nullptr, *w = nullptr;
+    float *u2 = nullptr, *v2 = nullptr, *w2 = nullptr;
+    uint8_t* label = nullptr;
+    uint8_t* valid = nullptr;
+    float* density = nullptr;
+    int32_t* cell_count = nullptr;   // Nc + 1 (last = removed particles)
+    int32_t* cell_start = nullptr;   // Nc + 2
+    int32_t* scan_tmp = nullptr;
+    uint16_t* nbinfo = nullptr;
+
+    // pressure solve
+    float *b = nullptr, *r = nullptr, *z = nullptr, *s = nullptr, *q = nullptr, *y = nullptr;
+    float* P32 = nullptr;
+    double* P64 = nullptr;
+    double* x = nullptr;
+    double* partials = nullptr;       // block partial sums (FP64)
+    int partials_cap = 0;
+    fsim::SolveScalars* scal = nullptr;   // device
+    fsim::SolveScalars* scal_host = nullptr;  // pinned host mirror
+    double last_dt = 0.0;
+
+    // obstacles
+    std::vector<fsim_obstacle> obs;
+    fsim_obstacle* obs_dev = nullptr;
+    uint64_t spawn_calls = 0;
+
+    // timing
+    hipEvent_t ev[2 * FSIM_NUM_TIMERS + 4] = {};
+    fsim_step_stats stats{};
+};
+
+namespace fsim {
+
+// ---- error plumbing ------------------------------------------------------
+int set_err(fsim_ctx* ctx, int code, const std::string& msg);
+#define FSIM_HIP(ctx, expr)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::fsim::set_err((ctx), FSIM_ERR_HIP,                                  \
+                                   std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+#define FSIM_TRY(expr)              \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != FSIM_OK) return rc_; \
+    } while (0)
+
+inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+// ---- particle stages (fsim_particles.hip) ---------------------------------
+int particles_reserve(fsim_ctx* ctx, int64_t n);
+int stage_spawn(fsim_ctx* ctx, double dt);
+// flags: 1 = advect, 2 = push out, 4 = stop particles
+int stage_move(fsim_ctx* ctx, double dt, int flags);
+int stage_bin(fsim_ctx* ctx);
+int upload_obstacles(fsim_ctx* ctx);
+
+// ---- grid stages (fsim_grid.hip) ------------------------------------------
+int stage_p2g(fsim_ctx* ctx);
+int stage_mark(fsim_ctx* ctx);
+int stage_obstacles(fsim_ctx* ctx);
+int stage_borders(fsim_ctx* ctx);
+int stage_post_p2g(fsim_ctx* ctx, double grav_inc);
+int stage_extrapolate(fsim_ctx* ctx);
+int stage_g2p(fsim_ctx* ctx);
+int fill_u8(fsim_ctx* ctx, uint8_t* p, uint8_t v, int64_t n);
+int fill_f32(fsim_ctx* ctx, float* p, float v, int64_t n);
+int fill_f64(fsim_ctx* ctx, double* p, double v, int64_t n);
+int fill_i32(fsim_ctx* ctx, int32_t* p, int32_t v, int64_t n);
+
+// ---- pressure solve (fsim_solver.hip) -------------------------------------
+int stage_solve(fsim_ctx* ctx, double dt);
+int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out);
+
+}  // namespace fsim

@@ -1,0 +1,743 @@
+// fsim_particles.hip — particle-side stages of the step on gfx950.
+//
+//  * stage_move:  advectParticles (simulator.cpp:134-251), removal of
+//                 particles that hit a sink, pushParticlesOutOfObstacles
+//                 (simulator.cpp:253-312) and stopParticles
+//                 (simulator.cpp:71-74), fused per particle.  The particle
+//                 state is FP32 but the collision logic runs in FP64 with
+//                 contraction off, so every branch (wall hit, back-step
+//                 count, push axis) is decided exactly as an FP64 evaluation
+//                 of the same expression order would decide it.
+//  * stage_spawn: spawnParticles (simulator.cpp:106-125) with a SplitMix64
+//                 counter RNG in place of std::rand (random.h:13-15).
+//  * stage_bin:   counting sort of the particles by grid cell: key +
+//                 histogram (integer atomics), exclusive scan, scatter,
+//                 per-cell sort by particle id, payload gather.  The result
+//                 (counts, offsets, permutation) is the serial stable
+//                 counting sort by (cell, particle id), bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fsim_internal.hpp"
+
+namespace fsim {
+
+// ---- SplitMix64 counter RNG (Steele, Lea, Flood 2014) ----------------------
+__host__ __device__ inline uint64_t splitmix64(uint64_t seed, uint64_t n) {
+    uint64_t z = seed + (n + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline double u01d(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
+__host__ __device__ inline float u01f(uint64_t x) { return (float)(x >> 40) * (1.0f / 16777216.0f); }
+constexpr uint64_t kSpawnSalt = 0x5350415741ULL;
+
+struct MoveArgs {
+    float* pos[3];
+    float* vel[3];
+    uint8_t* removed;
+    const fsim_obstacle* obs;
+    int nobs;
+    int64_t n;
+    double dt, pr, cd, cdinv;
+    double dims[3];
+    int ncell[3];
+    int flags;
+    int despawn;
+};
+
+__host__ __device__ inline bool is_sphere(int k) { return k == FSIM_OBS_SPHERE || k == FSIM_OBS_SOURCE || k == FSIM_OBS_SINK; }
+
+__host__ __device__ inline bool in_rect(const double p[3], double pr, const double rp[3], const double rs[3]) {
+#pragma clang fp contract(off)
+    return fabs(p[0] - rp[0]) < rs[0] * 0.5 + pr && fabs(p[1] - rp[1]) < rs[1] * 0.5 + pr &&
+           fabs(p[2] - rp[2]) < rs[2] * 0.5 + pr;
+}
+
+__host__ __device__ inline double dot3(const double a[3], const double b[3]) {
+#pragma clang fp contract(off)
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// advectParticles per-particle body, simulator.cpp:146-249.  Returns true if
+// the particle entered a despawning sink.
+__host__ __device__ bool advect_one(const MoveArgs& A, double pos[3], double v[3], const double lo[3], const double hi[3]) {
+#pragma clang fp contract(off)
+    const double wall_rest = 0.3, sphere_rest = 1.0, rect_rest = 0.2;
+    const double dt = A.dt, pr = A.pr;
+    bool removed = false;
+    double t = 0;
+    int run = 0;
+    const int max_run = 200;
+    while (run < max_run) {
+        run++;
+        double min_t = 1e6;
+        int min_axis = 0;
+        for (int a = 0; a < 3; a++) {
+            double comp = v[a];
+            double tmp = 1e6;
+            if (comp > 1e-6) tmp = (hi[a] - pos[a]) / comp;
+            else if (comp < -1e-6) tmp = (pos[a] - lo[a]) / -comp;
+            if (tmp < min_t) { min_t = tmp; min_axis = a; }
+        }
+        if (min_t <= (dt - t)) {
+            for (int a = 0; a < 3; a++) pos[a] += v[a] * min_t * 0.999;
+            v[min_axis] *= -wall_rest;
+            t += 0.999 * min_t;
+            continue;
+        }
+        bool collision = false;
+        for (int a = 0; a < 3; a++) pos[a] += v[a] * (dt - t);
+        for (int oi = 0; oi < A.nobs; oi++) {
+            const fsim_obstacle& o = A.obs[oi];
+            if (is_sphere(o.kind)) {
+                double r = o.r + pr;
+                double d3[3] = {o.pos[0] - pos[0], o.pos[1] - pos[1], o.pos[2] - pos[2]};
+                double d = sqrt(dot3(d3, d3)) - r;
+                if (d < 0) {
+                    if (o.kind == FSIM_OBS_SINK && A.despawn) {
+                        removed = true;
+                        collision = false;
+                        break;
+                    }
+                    double bt = 0;
+                    for (;;) {
+                        double q[3];
+                        for (int a = 0; a < 3; a++) q[a] = (pos[a] - v[a] * bt) - (o.pos[a] - o.speed[a] * bt);
+                        if (!(sqrt(dot3(q, q)) < r && bt < t + (double)0.001f)) break;
+                        bt += 0.0002;
+                    }
+                    if (bt > dt - t) continue;
+                    bt += 0.0002;
+                    double nrm[3], relv[3];
+                    for (int a = 0; a < 3; a++) pos[a] -= bt * v[a];
+                    for (int a = 0; a < 3; a++) nrm[a] = pos[a] - (o.pos[a] - o.speed[a] * bt);
+                    double inv = 1.0 / sqrt(dot3(nrm, nrm));
+                    for (int a = 0; a < 3; a++) nrm[a] *= inv;
+                    for (int a = 0; a < 3; a++) relv[a] = v[a] - o.speed[a];
+                    double ssn = -dot3(nrm, relv);
+                    if (ssn <= 0.0f) continue;
+                    for (int a = 0; a < 3; a++) {
+                        double mirror = -relv[a] / ssn;
+                        v[a] = (nrm[a] - mirror) * 2.0 + mirror;
+                        v[a] *= ssn * sphere_rest;
+                        v[a] += o.speed[a];
+                    }
+                    t += (dt - t) - bt;
+                    collision = true;
+                    break;
+                }
+            }
+            if (o.kind == FSIM_OBS_BOX) {
+                if (in_rect(pos, pr, o.pos, o.size)) {
+                    double bt = 0;
+                    for (;;) {
+                        double pp[3], op[3];
+                        for (int a = 0; a < 3; a++) {
+                            pp[a] = pos[a] - v[a] * bt;
+                            op[a] = o.pos[a] - o.speed[a] * bt;
+                        }
+                        if (!(in_rect(pp, pr, op, o.size) && bt < dt - t)) break;
+                        bt += 0.0002;
+                    }
+                    if (bt > dt - t) continue;
+                    bt += 0.0002;
+                    double ptmp[3];
+                    for (int a = 0; a < 3; a++) pos[a] -= bt * v[a];
+                    for (int a = 0; a < 3; a++) ptmp[a] = o.pos[a] - o.speed[a] * bt;
+                    for (int a = 0; a < 3; a++) {
+                        if (pos[a] >= ptmp[a] + o.size[a] * 0.5 + pr) {
+                            v[a] = -v[a] * rect_rest + o.speed[a];
+                            break;
+                        } else if (pos[a] <= ptmp[a] - o.size[a] * 0.5 - pr) {
+                            v[a] = -v[a] * rect_rest - o.speed[a];
+                            break;
+                        }
+                    }
+                    t += (dt - t) - bt;
+                    collision = true;
+                    break;
+                }
+            }
+        }
+        if (!collision) run = max_run;
+    }
+    for (int a = 0; a < 3; a++) pos[a] = pos[a] < lo[a] ? lo[a] : (pos[a] > hi[a] ? hi[a] : pos[a]);
+    return removed;
+}
+
+// MacGrid::getMinMaxRect, macGrid.h:139-145, expanded by the particle radius
+// (simulator.cpp:281-285).
+__host__ __device__ inline void rect_bounds(const MoveArgs& A, const fsim_obstacle& o, double lo[3], double hi[3]) {
+#pragma clang fp contract(off)
+    for (int a = 0; a < 3; a++) {
+        double c = o.pos[a] * A.cdinv;
+        double d = o.size[a] * A.cdinv * 0.5;
+        int mn = (int)fmax(round(c - d), 1.0);
+        int mx = (int)fmin(round(c + d), A.ncell[a] - 1.0);
+        lo[a] = (double)mn * A.cd - A.pr;
+        hi[a] = (double)mx * A.cd + A.pr;
+    }
+}
+
+// pushParticlesOutOfObstacles per particle, simulator.cpp:263-310.  The
+// reference runs one particle pass per obstacle; passes are independent per
+// particle, so looping the obstacles inside one pass is the same map.
+__host__ __device__ void push_out_one(const MoveArgs& A, double pos[3], const double plo[3], const double phi[3]) {
+#pragma clang fp contract(off)
+    for (int oi = 0; oi < A.nobs; oi++) {
+        const fsim_obstacle& o = A.obs[oi];
+        if (is_sphere(o.kind)) {
+            double r = o.r + A.pr, r2 = r * r;
+            double o2p[3] = {pos[0] - o.pos[0], pos[1] - o.pos[1], pos[2] - o.pos[2]};
+            double d2 = dot3(o2p, o2p);
+            if (d2 < r2) {
+                double d = sqrt(d2);
+                for (int a = 0; a < 3; a++) {
+                    double val = o.pos[a] + o2p[a] / d * r;
+                    pos[a] = val < plo[a] ? plo[a] : (val > phi[a] ? phi[a] : val);
+                }
+            }
+        }
+        if (o.kind == FSIM_OBS_BOX) {
+            double st[3], en[3];
+            rect_bounds(A, o, st, en);
+            if (pos[0] > en[0] || pos[0] < st[0] || pos[1] > en[1] || pos[1] < st[1] || pos[2] > en[2] ||
+                pos[2] < st[2])
+                continue;
+            int axis = 0;
+            double amount = 1e6;
+            for (int a = 0; a < 3; a++) {
+                if (pos[a] > o.pos[a]) {
+                    double tmp = en[a] - pos[a];
+                    if (tmp < fabs(amount)) { amount = tmp; axis = a; }
+                } else {
+                    double tmp = st[a] - pos[a];
+                    if (-tmp < fabs(amount)) { amount = tmp; axis = a; }
+                }
+            }
+            double val = pos[axis] + amount;
+            pos[axis] = val < plo[axis] ? plo[axis] : (val > phi[axis] ? phi[axis] : val);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_move(MoveArgs A) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    double pos[3], v[3];
+    for (int a = 0; a < 3; a++) {
+        pos[a] = A.pos[a][i];
+        v[a] = A.vel[a][i];
+    }
+    bool removed = false;
+    if (A.flags & 1) {
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = A.cd + A.pr * 1.01;
+            hi[a] = A.dims[a] - lo[a];
+        }
+        removed = advect_one(A, pos, v, lo, hi);
+        if (removed) A.removed[i] = 1;
+    }
+    if ((A.flags & 2) && !removed) {
+        double plo[3], phi[3];
+        for (int a = 0; a < 3; a++) {
+            plo[a] = A.cd + A.pr * 1.01;
+            phi[a] = A.dims[a] - plo[a];
+        }
+        push_out_one(A, pos, plo, phi);
+    }
+    if (A.flags & 4) v[0] = v[1] = v[2] = 0.0;
+    for (int a = 0; a < 3; a++) {
+        A.pos[a][i] = (float)pos[a];
+        A.vel[a][i] = (float)v[a];
+    }
+}
+
+int upload_obstacles(fsim_ctx* ctx) {
+    if (ctx->obs.empty()) return FSIM_OK;
+    FSIM_HIP(ctx, hipMemcpyAsync(ctx->obs_dev, ctx->obs.data(), sizeof(fsim_obstacle) * ctx->obs.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    return FSIM_OK;
+}
+
+int stage_move(fsim_ctx* ctx, double dt, int flags) {
+    if (ctx->P.n == 0) return FSIM_OK;
+    if (!ctx->prm.stop_particles) flags &= ~4;
+    MoveArgs A{};
+    for (int a = 0; a < 3; a++) {
+        A.pos[a] = ctx->P.pos[a];
+        A.vel[a] = ctx->P.vel[a];
+        A.dims[a] = ctx->g.dims[a];
+    }
+    A.removed = ctx->removed;
+    A.obs = ctx->obs_dev;
+    A.nobs = (int)ctx->obs.size();
+    A.n = ctx->P.n;
+    A.dt = dt;
+    A.pr = ctx->g.pr;
+    A.cd = ctx->g.cd;
+    A.cdinv = ctx->g.cdinv;
+    A.ncell[0] = ctx->g.nx;
+    A.ncell[1] = ctx->g.ny;
+    A.ncell[2] = ctx->g.nz;
+    A.flags = flags;
+    A.despawn = ctx->prm.despawning_enabled;
+    hipLaunchKernelGGL(k_move, dim3(nblocks(A.n)), dim3(kBlock), 0, ctx->stream, A);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// ---- spawning -----------------------------------------------------------------
+struct SpawnArgs {
+    float* pos[3];
+    float* vel[3];
+    float* c[9];
+    int32_t* pid;
+    uint8_t* removed;
+    int64_t base;       // first new slot
+    int64_t pid_base;   // first new particle id
+    int count;
+    double opos[3];
+    double r;
+    double speed;
+    uint64_t seed;
+    uint64_t ctr_base;
+};
+
+__global__ void __launch_bounds__(kBlock) k_spawn(SpawnArgs A) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.count) return;
+    uint64_t ctr = A.ctr_base + 2ull * (uint64_t)i;
+    double theta = u01d(splitmix64(A.seed, ctr)) * (2.0 * M_PI);
+    double phi = u01d(splitmix64(A.seed, ctr + 1)) * M_PI;
+    double r = A.r;
+    double nrm[3] = {r * sin(phi) * cos(theta), r * sin(phi) * sin(theta), r * cos(phi)};
+    double inv = 1.0 / sqrt(dot3(nrm, nrm));
+    int64_t j = A.base + i;
+    for (int a = 0; a < 3; a++) {
+        A.pos[a][j] = (float)(A.opos[a] + nrm[a]);
+        A.vel[a][j] = (float)(A.speed * (nrm[a] * inv));
+    }
+    for (int k = 0; k < 9; k++) A.c[k][j] = 0.0f;
+    A.pid[j] = (int32_t)(A.pid_base + i);
+    A.removed[j] = 0;
+}
+
+int stage_spawn(fsim_ctx* ctx, double dt) {
+    // counts on the host in FP64, exactly as simulator.cpp:112-114
+    std::vector<int> nsp(ctx->obs.size(), 0);
+    int64_t total = 0;
+    for (size_t o = 0; o < ctx->obs.size(); o++) {
+        fsim_obstacle& ob = ctx->obs[o];
+        if (ob.kind != FSIM_OBS_SOURCE) continue;
+        double num_d = ob.spawn_rate * dt + ob.last_spawn_fraction;
+        int num = (int)num_d;
+        ob.last_spawn_fraction = num_d - num;
+        nsp[o] = num;
+        total += num;
+    }
+    FSIM_TRY(upload_obstacles(ctx));
+    if (total > 0) {
+        FSIM_TRY(particles_reserve(ctx, ctx->P.n + total));
+        int64_t base = ctx->P.n;
+        for (size_t o = 0; o < ctx->obs.size(); o++) {
+            if (!nsp[o]) continue;
+            const fsim_obstacle& ob = ctx->obs[o];
+            SpawnArgs A{};
+            for (int a = 0; a < 3; a++) {
+                A.pos[a] = ctx->P.pos[a];
+                A.vel[a] = ctx->P.vel[a];
+                A.opos[a] = ob.pos[a];
+            }
+            for (int k = 0; k < 9; k++) A.c[k] = ctx->P.c[k];
+            A.pid = ctx->P.pid;
+            A.removed = ctx->removed;
+            A.base = base;
+            A.pid_base = base;
+            A.count = nsp[o];
+            A.r = ob.r + ctx->g.pr;
+            A.speed = ob.spawn_speed;
+            A.seed = ctx->prm.seed ^ kSpawnSalt;
+            A.ctr_base = ((ctx->spawn_calls << 6) + (uint64_t)o) << 32;
+            hipLaunchKernelGGL(k_spawn, dim3(nblocks(A.count)), dim3(kBlock), 0, ctx->stream, A);
+            FSIM_HIP(ctx, hipGetLastError());
+            base += nsp[o];
+        }
+        ctx->P.n = base;
+    }
+    ctx->spawn_calls++;
+    return FSIM_OK;
+}
+
+// ---- particle storage -------------------------------------------------------
+static int alloc_planes(fsim_ctx* ctx, Particles& P, int64_t cap) {
+    float* base = nullptr;
+    if (hipMalloc(&base, sizeof(float) * (size_t)cap * 16) != hipSuccess)
+        return set_err(ctx, FSIM_ERR_OOM, "particle buffer allocation failed");
+    P.base = base;
+    P.cap = cap;
+    for (int a = 0; a < 3; a++) {
+        P.pos[a] = base + (size_t)cap * a;
+        P.vel[a] = base + (size_t)cap * (3 + a);
+    }
+    for (int k = 0; k < 9; k++) P.c[k] = base + (size_t)cap * (6 + k);
+    P.pid = reinterpret_cast<int32_t*>(base + (size_t)cap * 15);
+    return FSIM_OK;
+}
+
+int particles_reserve(fsim_ctx* ctx, int64_t n) {
+    if (n <= ctx->P.cap) return FSIM_OK;
+    if (n > INT32_MAX) return set_err(ctx, FSIM_ERR_INVALID, "particle count exceeds int32 range");
+    int64_t cap = std::max<int64_t>(n + n / 8, 1024);
+    Particles NP, NA;
+    FSIM_TRY(alloc_planes(ctx, NP, cap));
+    FSIM_TRY(alloc_planes(ctx, NA, cap));
+    if (ctx->P.n > 0) {
+        for (int k = 0; k < 16; k++)
+            FSIM_HIP(ctx, hipMemcpyAsync(NP.base + (size_t)cap * k, ctx->P.base + (size_t)ctx->P.cap * k,
+                                         sizeof(float) * ctx->P.n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    uint32_t* key = nullptr;
+    int32_t *rank = nullptr, *perm = nullptr;
+    uint8_t* removed = nullptr;
+    if (hipMalloc(&key, sizeof(uint32_t) * (cap + 1)) != hipSuccess || hipMalloc(&rank, sizeof(int32_t) * cap) != hipSuccess ||
+        hipMalloc(&perm, sizeof(int32_t) * cap) != hipSuccess || hipMalloc(&removed, cap) != hipSuccess)
+        return set_err(ctx, FSIM_ERR_OOM, "particle work buffer allocation failed");
+    FSIM_HIP(ctx, hipMemsetAsync(removed, 0, cap, ctx->stream));
+    if (ctx->P.n > 0)
+        FSIM_HIP(ctx, hipMemcpyAsync(removed, ctx->removed, ctx->P.n, hipMemcpyDeviceToDevice, ctx->stream));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->P.base);
+    (void)hipFree(ctx->Palt.base);
+    (void)hipFree(ctx->key);
+    (void)hipFree(ctx->rank);
+    (void)hipFree(ctx->perm);
+    (void)hipFree(ctx->removed);
+    NP.n = ctx->P.n;
+    ctx->P = NP;
+    ctx->Palt = NA;
+    ctx->key = key;
+    ctx->rank = rank;
+    ctx->perm = perm;
+    ctx->removed = removed;
+    ctx->pwork_cap = cap;
+    // the id-renumbering scan runs over up to cap particles
+    const int64_t need_tmp = (cap + 2) / 4096 + 2;
+    if (ctx->scan_tmp_cap < need_tmp) {
+        int32_t* tmp = nullptr;
+        if (hipMalloc(&tmp, sizeof(int32_t) * need_tmp) != hipSuccess)
+            return set_err(ctx, FSIM_ERR_OOM, "scan buffer allocation failed");
+        (void)hipFree(ctx->scan_tmp);
+        ctx->scan_tmp = tmp;
+        ctx->scan_tmp_cap = need_tmp;
+    }
+    return FSIM_OK;
+}
+
+// ---- binning: counting sort by grid cell --------------------------------------
+__global__ void __launch_bounds__(kBlock) k_bin_keys(const float* __restrict__ px, const float* __restrict__ py,
+                                                     const float* __restrict__ pz, const uint8_t* __restrict__ removed,
+                                                     uint32_t* __restrict__ key, int32_t* __restrict__ rank,
+                                                     int32_t* __restrict__ count, int64_t n, double cdinv, int nx,
+                                                     int ny, int nz, uint32_t nc) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k;
+    if (removed[i]) {
+        k = nc;
+    } else {
+        // ivec3(pos * cellDInv), simulator.cpp:358-359, evaluated in FP64
+        // positions outside the grid (never produced by the step, whose
+        // advection clamps them) are clamped into it so no key leaves [0, Nc)
+        int cx = min(max((int)((double)px[i] * cdinv), 0), nx - 1);
+        int cy = min(max((int)((double)py[i] * cdinv), 0), ny - 1);
+        int cz = min(max((int)((double)pz[i] * cdinv), 0), nz - 1);
+        k = (uint32_t)(((int64_t)cx * ny + cy) * nz + cz);
+    }
+    key[i] = k;
+    rank[i] = atomicAdd(&count[k], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint32_t* __restrict__ key, const int32_t* __restrict__ rank,
+                                                        const int32_t* __restrict__ start, int32_t* __restrict__ perm,
+                                                        int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    perm[start[key[i]] + rank[i]] = (int32_t)i;
+}
+
+// Canonical order inside a cell: ascending particle id (insertion sort; a
+// cell holds ~8 particles in a resting fluid).
+__global__ void __launch_bounds__(kBlock) k_bin_segsort(const int32_t* __restrict__ start, int32_t* __restrict__ perm,
+                                                        const int32_t* __restrict__ pid, int64_t nc) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int b = start[c], e = start[c + 1];
+    for (int i = b + 1; i < e; i++) {
+        int32_t v = perm[i];
+        int32_t kv = pid[v];
+        int j = i - 1;
+        while (j >= b && pid[perm[j]] > kv) {
+            perm[j + 1] = perm[j];
+            j--;
+        }
+        perm[j + 1] = v;
+    }
+}
+
+struct GatherArgs {
+    const float* src[16];
+    float* dst[16];
+    int nplanes;
+    const int32_t* perm;
+    int64_t n;
+};
+
+__global__ void __launch_bounds__(kBlock) k_bin_gather(GatherArgs A) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.n) return;
+    int32_t s = A.perm[j];
+#pragma unroll 4
+    for (int k = 0; k < A.nplanes; k++) A.dst[k][j] = A.src[k][s];
+}
+
+// pid renumbering after sink removals: a removed particle's slot in the
+// reference's vector is erased (hashedParticles.cpp:158-172), so every later
+// id shifts down by the number of removed ids before it.
+__global__ void __launch_bounds__(kBlock) k_mark_removed_ids(const int32_t* __restrict__ pid, const uint8_t* __restrict__ removed,
+                                                             int32_t* __restrict__ flag, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[pid[i]] = removed[i] ? 1 : 0;
+}
+__global__ void __launch_bounds__(kBlock) k_renumber(int32_t* __restrict__ pid, const int32_t* __restrict__ before, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pid[i] -= before[pid[i]];
+}
+
+// ---- exclusive scan of int32 (n values -> n+1 offsets) -------------------------
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ inline int32_t block_exclusive_scan(int32_t v, int32_t* lds_waves, int32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds_waves[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            int32_t t = lds_waves[w];
+            lds_waves[w] = acc;
+            acc += t;
+        }
+        lds_waves[kBlock / 64] = acc;
+    }
+    __syncthreads();
+    int32_t res = x - v + lds_waves[wave];
+    *total = lds_waves[kBlock / 64];
+    __syncthreads();
+    return res;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                       int32_t* __restrict__ tile_sums, int64_t n) {
+    __shared__ int32_t lds[kBlock / 64 + 1];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int32_t vals[kScanItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int64_t i = base + k;
+        vals[k] = i < n ? in[i] : 0;
+        sum += vals[k];
+    }
+    int32_t total;
+    int32_t off = block_exclusive_scan(sum, lds, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int64_t i = base + k;
+        if (i < n) out[i] = off;
+        off += vals[k];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tile_sums(int32_t* __restrict__ tile_sums, int ntiles,
+                                                           int32_t* __restrict__ out_total) {
+    __shared__ int32_t lds[kBlock / 64 + 1];
+    int32_t carry = 0;
+    for (int b = 0; b < ntiles; b += kBlock) {
+        int i = b + threadIdx.x;
+        int32_t v = i < ntiles ? tile_sums[i] : 0;
+        int32_t total;
+        int32_t ex = block_exclusive_scan(v, lds, &total);
+        if (i < ntiles) tile_sums[i] = ex + carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_add(int32_t* __restrict__ out, const int32_t* __restrict__ tile_sums, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] += tile_sums[i / kScanTile];
+}
+
+// out[0..n] = exclusive scan of in[0..n-1]; tmp holds >= ceil(n/4096) ints.
+static int exclusive_scan(fsim_ctx* ctx, const int32_t* in, int32_t* out, int32_t* tmp, int64_t n) {
+    int ntiles = (int)((n + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, ctx->stream, in, out, tmp, n);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, ctx->stream, tmp, ntiles, out + n);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, out, tmp, n);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+int stage_bin(fsim_ctx* ctx) {
+    const int64_t nc = ctx->g.nc;
+    const int64_t n = ctx->P.n;
+    // counts of Nc cells + 1 bucket for removed particles
+    FSIM_TRY(fill_i32(ctx, ctx->cell_count, 0, nc + 1));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_bin_keys, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->P.pos[0], ctx->P.pos[1],
+                           ctx->P.pos[2], ctx->removed, ctx->key, ctx->rank, ctx->cell_count, n, ctx->g.cdinv,
+                           ctx->g.nx, ctx->g.ny, ctx->g.nz, (uint32_t)nc);
+        FSIM_HIP(ctx, hipGetLastError());
+    }
+    FSIM_TRY(exclusive_scan(ctx, ctx->cell_count, ctx->cell_start, ctx->scan_tmp, nc + 1));
+    if (n == 0) return FSIM_OK;
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->key, ctx->rank,
+                       ctx->cell_start, ctx->perm, n);
+    hipLaunchKernelGGL(k_bin_segsort, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, ctx->cell_start, ctx->perm,
+                       ctx->P.pid, nc);
+    FSIM_HIP(ctx, hipGetLastError());
+    int64_t n_new = n;
+    if (ctx->prm.despawning_enabled) {
+        int32_t kept = 0;
+        FSIM_HIP(ctx, hipMemcpyAsync(&kept, ctx->cell_start + nc, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        n_new = kept;
+        if (n_new < n) {
+            // flags over the id space, scanned, then ids shifted down
+            int32_t* flag = ctx->rank;  // rank is free after the scatter
+            hipLaunchKernelGGL(k_mark_removed_ids, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->P.pid,
+                               ctx->removed, flag, n);
+            FSIM_TRY(exclusive_scan(ctx, flag, reinterpret_cast<int32_t*>(ctx->key), ctx->scan_tmp, n));
+            hipLaunchKernelGGL(k_renumber, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->P.pid,
+                               reinterpret_cast<const int32_t*>(ctx->key), n);
+            FSIM_HIP(ctx, hipGetLastError());
+        }
+    }
+    GatherArgs G{};
+    const bool apic = ctx->prm.transfer == FSIM_APIC;
+    int k = 0;
+    for (int a = 0; a < 3; a++) { G.src[k] = ctx->P.pos[a]; G.dst[k++] = ctx->Palt.pos[a]; }
+    for (int a = 0; a < 3; a++) { G.src[k] = ctx->P.vel[a]; G.dst[k++] = ctx->Palt.vel[a]; }
+    if (apic)
+        for (int c = 0; c < 9; c++) { G.src[k] = ctx->P.c[c]; G.dst[k++] = ctx->Palt.c[c]; }
+    G.src[k] = reinterpret_cast<const float*>(ctx->P.pid);
+    G.dst[k++] = reinterpret_cast<float*>(ctx->Palt.pid);
+    G.nplanes = k;
+    G.perm = ctx->perm;
+    G.n = n_new;
+    if (n_new > 0) {
+        hipLaunchKernelGGL(k_bin_gather, dim3(nblocks(n_new)), dim3(kBlock), 0, ctx->stream, G);
+        FSIM_HIP(ctx, hipGetLastError());
+    }
+    // PIC/FLIP never read or write the APIC matrices, so they are not carried
+    // through the gather; they read as zero until APIC mode re-initialises them.
+    if (!apic) ctx->c_valid = false;
+    std::swap(ctx->P, ctx->Palt);
+    ctx->P.n = n_new;
+    ctx->Palt.n = 0;
+    if (n_new < n || ctx->prm.despawning_enabled)
+        FSIM_HIP(ctx, hipMemsetAsync(ctx->removed, 0, ctx->P.cap, ctx->stream));
+    return FSIM_OK;
+}
+
+}  // namespace fsim
+
+// ---- synthetic scene generator (host only) -------------------------------------
+using namespace fsim;
+
+// Voxelisation rule of MacGrid::addObstacle for spheres and boxes
+// (macGrid.cpp:235-291), used to keep scene particles out of solid cells.
+static bool scene_cell_solid(const fsim_grid_desc* g, int x, int y, int z, int nobs, const fsim_obstacle* obs) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const double inv = 1.0 / g->cell_d;
+    const int p[3] = {x, y, z};
+    for (int i = 0; i < nobs; i++) {
+        const fsim_obstacle& o = obs[i];
+        double ctr[3] = {o.pos[0] * inv, o.pos[1] * inv, o.pos[2] * inv};
+        if (o.kind == FSIM_OBS_SPHERE || o.kind == FSIM_OBS_SOURCE) {
+            double r = inv * o.r, r2 = r * r;
+            bool in = true;
+            for (int a = 0; a < 3; a++) {
+                int mn = (int)std::fmax(ctr[a] - r, 1.0), mx = (int)std::fmin(ctr[a] + r, n[a] - 2.0);
+                if (p[a] < mn || p[a] > mx) in = false;
+            }
+            double c2o[3] = {x + 0.5 - ctr[0], y + 0.5 - ctr[1], z + 0.5 - ctr[2]};
+            if (in && c2o[0] * c2o[0] + c2o[1] * c2o[1] + c2o[2] * c2o[2] < r2) return true;
+        } else if (o.kind == FSIM_OBS_BOX) {
+            bool in = true;
+            for (int a = 0; a < 3; a++) {
+                double d = o.size[a] * inv * 0.5;
+                int mn = (int)std::fmax(std::round(ctr[a] - d), 1.0), mx = (int)std::fmin(std::round(ctr[a] + d), n[a] - 1.0);
+                if (p[a] < mn || p[a] >= mx) in = false;
+            }
+            if (in) return true;
+        }
+    }
+    return false;
+}
+
+extern "C" int fsim_scene_dam_break(const fsim_grid_desc* g, int n_blocks, const double* blk, int n_obstacles,
+                                    const fsim_obstacle* obs, uint64_t seed, int64_t capacity, float* pos,
+                                    int64_t* n_out) {
+    if (!g || !n_out || n_blocks < 0 || (n_blocks > 0 && !blk) || g->nx < 4 || g->ny < 4 || g->nz < 4)
+        return FSIM_ERR_INVALID;
+    const double cd = g->cell_d;
+    int64_t np = 0;
+    for (int x = 1; x < g->nx - 1; x++)
+        for (int y = 1; y < g->ny - 1; y++)
+            for (int z = 1; z < g->nz - 1; z++) {
+                const double c[3] = {(x + 0.5) * cd, (y + 0.5) * cd, (z + 0.5) * cd};
+                bool in = false;
+                for (int b = 0; b < n_blocks && !in; b++) {
+                    const double* lo = blk + 6 * b;
+                    const double* hi = lo + 3;
+                    in = c[0] >= lo[0] && c[0] < hi[0] && c[1] >= lo[1] && c[1] < hi[1] && c[2] >= lo[2] && c[2] < hi[2];
+                }
+                if (!in || scene_cell_solid(g, x, y, z, n_obstacles, obs)) continue;
+                if (pos) {
+                    if (np + 8 > capacity) return FSIM_ERR_INVALID;
+                    const int ci[3] = {x, y, z};
+                    for (int s = 0; s < 8; s++) {
+                        const int sub[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1};
+                        int64_t id = np + s;
+                        for (int a = 0; a < 3; a++) {
+                            float u = u01f(splitmix64(seed, 3ull * (uint64_t)id + (uint64_t)a));
+                            pos[3 * id + a] = (float)((ci[a] + ((double)sub[a] + (double)u) * 0.5) * cd);
+                        }
+                    }
+                }
+                np += 8;
+            }
+    *n_out = np;
+    return FSIM_OK;
+}

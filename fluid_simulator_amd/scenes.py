@@ -1,0 +1,64 @@
+"""Synthetic dam-break scenes of the benchmark configurations.
+
+Common choices (the reference leaves them open): domain [0,1]^3 built as
+MacGrid(targetDimensions=(1,1,1), resolution=N), particle radius dx/4,
+push-apart off (the reference's 3D preset, simulationGui.cpp:57), pressure
+(density) term off, open top, rho = 1, gravity -9.81 on y, fixed dt = 1/240,
+PCG cap raised so that the 1e-6 tolerance is reached.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+from . import capi
+
+DT = 1.0 / 240.0
+
+
+def grid(n: int) -> capi.Grid:
+    return capi.Grid(n, n, n, 1.0 / n, 0.25 / n)
+
+
+def params(transfer=capi.FLIP, stop_rule=capi.STOP_REL_L2, tol=1e-6, max_iterations=10000, **kw) -> capi.Params:
+    base = dict(transfer=transfer, flip_ratio=0.95, gravity=-9.81, gravity_enabled=1, push_apart_enabled=0,
+                pressure_enabled=0, top_solid=0, fluid_density=1.0, residual_tolerance=tol,
+                max_iterations=max_iterations, stop_rule=stop_rule)
+    base.update(kw)
+    return capi.make_params(**base)
+
+
+@dataclass
+class Scene:
+    name: str
+    n: int
+    transfer: int
+    blocks: list
+    obstacles: list = field(default_factory=list)
+    seed: int = 0
+    steps: int = 100
+
+    @property
+    def grid(self) -> capi.Grid:
+        return grid(self.n)
+
+    def particles(self):
+        return capi.scene_dam_break(self.grid, self.blocks, self.obstacles, self.seed)
+
+
+def dam_break(n: int, transfer=capi.FLIP, seed=0, sphere=False, steps=100) -> Scene:
+    """Block x < 0.4, y < 0.6 (full z), 8 particles per fluid cell."""
+    obs = [capi.make_obstacle(capi.OBS_SPHERE, (0.5, 0.5, 0.5), r=0.1)] if sphere else []
+    return Scene(f"dam_break_{n}", n, transfer, [[0.0, 0.0, 0.0, 0.4, 0.6, 1.0]], obs, seed, steps)
+
+
+def double_dam_break(n: int, transfer=capi.FLIP, seed=1, steps=100) -> Scene:
+    """Two opposing blocks x < 0.4 and x > 0.6, y < 0.6 (full z)."""
+    return Scene(f"double_dam_break_{n}", n, transfer,
+                 [[0.0, 0.0, 0.0, 0.4, 0.6, 1.0], [0.6, 0.0, 0.0, 1.0, 0.6, 1.0]], [], seed, steps)
+
+
+# The benchmark configurations: 32^3 CPU-path case, 128^3 APIC with a sphere,
+# 256^3 FLIP double dam break.
+CONFIG_32_FLIP = lambda: dam_break(32, capi.FLIP, seed=0, steps=100)  # noqa: E731
+CONFIG_128_APIC_SPHERE = lambda: dam_break(128, capi.APIC, seed=0, sphere=True, steps=200)  # noqa: E731
+CONFIG_256_FLIP_DOUBLE = lambda: double_dam_break(256, capi.FLIP, seed=1, steps=100)  # noqa: E731

@@ -1,0 +1,261 @@
+"""Stage-by-stage and whole-step parity of the HIP path against the FP64 oracle.
+
+Inputs are FP32-representable and identical on both sides.  Bars:
+integer/label/binning results bit-exact; particle positions after advection
+within 1 FP32 ulp of the FP64 result; grid fields within 1e-5 of the field's
+scale (max |gpu - oracle| / max |oracle|); pressures within 1e-5 relative
+when both solve to a tight tolerance.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from fluid_simulator_amd import capi, scenes
+from helpers import dam_state, field_rel_err, pair, random_particles, sync_oracle_from_gpu, ulp_mismatch
+
+pytestmark = pytest.mark.gpu
+DT = scenes.DT
+
+
+def say(msg: str):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_bin_bit_exact(gpu):
+    grid = scenes.grid(16)
+    say("case: bin 16^3, 5000 random particles, seed 3")
+    pos, vel, _ = random_particles(grid, 5000, seed=3)
+    sim, ref = pair(grid, scenes.params(), pos, vel)
+    sim.run_stage(capi.STAGE_BIN)
+    count, start, perm = ref.bin()
+    assert np.array_equal(sim.get_grid(capi.FIELD_CELL_COUNT), count)
+    assert np.array_equal(sim.get_grid(capi.FIELD_CELL_START), start)
+    assert np.array_equal(sim.slot_ids(), perm.astype(np.int32))
+    # binning does not change the particles seen through the id order
+    p2, v2, _ = sim.get_particles()
+    assert np.array_equal(p2, pos) and np.array_equal(v2, vel)
+
+
+def test_bin_empty_and_single(gpu):
+    grid = scenes.grid(8)
+    say("case: bin 8^3 empty and one particle")
+    sim = capi.Simulation(grid, scenes.params())
+    sim.set_particles(np.zeros((0, 3), np.float32))
+    sim.run_stage(capi.STAGE_BIN)
+    assert sim.get_grid(capi.FIELD_CELL_COUNT).sum() == 0
+    sim.set_particles(np.array([[0.3, 0.4, 0.5]], np.float32))
+    sim.run_stage(capi.STAGE_BIN)
+    cnt = sim.get_grid(capi.FIELD_CELL_COUNT)
+    assert cnt.sum() == 1 and cnt[(2 * 8 + 3) * 8 + 4] == 1
+
+
+OBSTACLE_SETS = {
+    "none": [],
+    "sphere": [capi.make_obstacle(capi.OBS_SPHERE, (0.5, 0.45, 0.5), r=0.15)],
+    "moving_sphere_box": [capi.make_obstacle(capi.OBS_SPHERE, (0.3, 0.5, 0.6), speed=(0.4, -0.2, 0.1), r=0.12),
+                          capi.make_obstacle(capi.OBS_BOX, (0.7, 0.3, 0.4), speed=(-0.3, 0.0, 0.2),
+                                             size=(0.2, 0.25, 0.3))],
+}
+
+
+@pytest.mark.parametrize("obs_name", list(OBSTACLE_SETS))
+def test_advect_push_out_parity(gpu, obs_name):
+    grid = scenes.grid(16)
+    say(f"case: advect+push-out 16^3, obstacles={obs_name}, seed 5")
+    pos, vel, _ = random_particles(grid, 20000, seed=5, vmax=12.0)
+    obs = OBSTACLE_SETS[obs_name]
+    sim, ref = pair(grid, scenes.params(), pos, vel, obstacles=obs)
+    dt = 1.0 / 60.0
+    for stage in (capi.STAGE_ADVECT, capi.STAGE_PUSH_OUT):
+        sim.run_stage(stage, dt)
+        ref.run_stage(stage, dt)
+        p_gpu, v_gpu, _ = sim.get_particles()
+        p_ref, v_ref, _ = ref.get_particles()
+        # the FP64 result rounded to FP32 (same branch decisions everywhere)
+        assert ulp_mismatch(p_gpu, p_ref, 1.0) == 0, stage
+        assert ulp_mismatch(v_gpu, v_ref, 1.0) == 0, stage
+        # next stage starts from the same FP32 state on both sides
+        sync_oracle_from_gpu(sim, ref)
+
+
+def test_sink_despawn_parity(gpu):
+    grid = scenes.grid(16)
+    say("case: advect with a despawning sink 16^3, seed 7")
+    pos, vel, _ = random_particles(grid, 8000, seed=7, vmax=6.0)
+    obs = [capi.make_obstacle(capi.OBS_SINK, (0.5, 0.5, 0.5), r=0.2)]
+    prm = scenes.params(despawning_enabled=1)
+    sim, ref = pair(grid, prm, pos, vel, obstacles=obs)
+    sim.run_stage(capi.STAGE_ADVECT, 1.0 / 60.0)
+    sim.run_stage(capi.STAGE_BIN)   # compaction happens in the binning
+    ref.run_stage(capi.STAGE_ADVECT, 1.0 / 60.0)
+    assert sim.num_particles() == ref.num_particles() < 8000
+    p_gpu, _, _ = sim.get_particles()
+    p_ref, _, _ = ref.get_particles()
+    assert ulp_mismatch(p_gpu, p_ref, 1.0) == 0
+
+
+def test_spawn_parity(gpu):
+    grid = scenes.grid(16)
+    say("case: spawn from a source 16^3")
+    src = capi.make_obstacle(capi.OBS_SOURCE, (0.5, 0.7, 0.5), r=0.05, spawn_rate=12345.0, spawn_speed=0.8,
+                             last_spawn_fraction=0.25)
+    pos, vel, _ = random_particles(grid, 100, seed=1)
+    sim, ref = pair(grid, scenes.params(seed=99), pos, vel, obstacles=[src])
+    for _ in range(3):
+        sim.run_stage(capi.STAGE_SPAWN, DT)
+        ref.run_stage(capi.STAGE_SPAWN, DT)
+    assert sim.num_particles() == ref.num_particles()
+    p_gpu, v_gpu, _ = sim.get_particles()
+    p_ref, v_ref, _ = ref.get_particles()
+    assert np.abs(p_gpu - p_ref).max() < 1e-6 and np.abs(v_gpu - v_ref).max() < 1e-6
+    assert abs(sim.get_obstacles()[0].last_spawn_fraction - ref.get_obstacles()[0].last_spawn_fraction) < 1e-12
+
+
+@pytest.mark.parametrize("transfer", [capi.PIC, capi.FLIP, capi.APIC])
+def test_p2g_mark_parity(gpu, transfer):
+    grid = scenes.grid(16)
+    say(f"case: p2g+mark 16^3 transfer={transfer}, seed 11")
+    pos, vel, c = random_particles(grid, 30000, seed=11, apic=transfer == capi.APIC)
+    prm = scenes.params(transfer=transfer, pressure_enabled=1)
+    sim, ref = pair(grid, prm, pos, vel, c)
+    for stage in (capi.STAGE_P2G, capi.STAGE_MARK):
+        sim.run_stage(stage, DT)
+        ref.run_stage(stage, DT)
+    for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W, capi.FIELD_DENSITY):
+        assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-5, f
+    assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL))
+
+
+@pytest.mark.parametrize("obs_name", list(OBSTACLE_SETS))
+@pytest.mark.parametrize("top", [0, 1])
+def test_prep_parity(gpu, obs_name, top):
+    """P2G -> mark -> obstacles -> borders -> v2+gravity: labels bit-exact,
+    faces at the field scale."""
+    grid = scenes.grid(16)
+    say(f"case: prep 16^3 obstacles={obs_name} top={top}")
+    pos, vel, _ = random_particles(grid, 30000, seed=13)
+    prm = scenes.params(top_solid=top)
+    sim, ref = pair(grid, prm, pos, vel, obstacles=OBSTACLE_SETS[obs_name])
+    for stage in (capi.STAGE_P2G, capi.STAGE_MARK, capi.STAGE_OBSTACLES, capi.STAGE_BORDERS, capi.STAGE_POST_P2G):
+        sim.run_stage(stage, DT)
+        ref.run_stage(stage, DT)
+    assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL))
+    for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W, capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-5, f
+
+
+def _prepared_pair(n=16, transfer=capi.FLIP, sphere=True, **kw):
+    sc, prm, pos, vel, c = dam_state(n, transfer, steps=3, sphere=sphere, **kw)
+    sim, ref = pair(sc.grid, prm, pos, vel, c, obstacles=sc.obstacles)
+    for stage in (capi.STAGE_P2G, capi.STAGE_MARK, capi.STAGE_OBSTACLES, capi.STAGE_BORDERS, capi.STAGE_POST_P2G):
+        sim.run_stage(stage, DT)
+        ref.run_stage(stage, DT)
+    return sc, prm, sim, ref
+
+
+def test_solver_operators(gpu):
+    say("case: A, MIC(0) factor, MIC(0) apply on a 16^3 dam-break state")
+    sc, prm, sim, ref = _prepared_pair()
+    lab = ref.get_grid(capi.FIELD_LABEL)
+    assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), lab)
+    rng = np.random.default_rng(0)
+    vec = rng.standard_normal(sc.grid.nc).astype(np.float32).astype(np.float64)
+    vec[lab != capi.WATER] = 0
+    fac_gpu = sim.solver_apply(capi.OP_FACTOR_MIC0, DT)
+    fac_ref = ref.solver_apply(capi.OP_FACTOR_MIC0, DT)
+    assert np.array_equal(fac_gpu, fac_ref)   # FP64 factor, same expression order
+    assert field_rel_err(sim.solver_apply(capi.OP_APPLY_A, DT, vec), ref.solver_apply(capi.OP_APPLY_A, DT, vec)) < 1e-6
+    assert field_rel_err(sim.solver_apply(capi.OP_APPLY_MIC0, DT, vec),
+                         ref.solver_apply(capi.OP_APPLY_MIC0, DT, vec)) < 1e-5
+
+
+@pytest.mark.parametrize("rule", [capi.STOP_ABS_INF, capi.STOP_REL_L2])
+def test_solve_parity(gpu, rule):
+    say(f"case: pressure solve 16^3 dam break with sphere, stop rule {rule}")
+    sc, prm, sim, ref = _prepared_pair(stop_rule=rule, tol=1e-9 if rule == capi.STOP_ABS_INF else 1e-10)
+    # identical FP32-representable v2 on both sides, so b differs only by rounding
+    for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        v = ref.get_grid(f).astype(np.float32)
+        sim.set_grid(f, v)
+        ref.set_grid(f, v.astype(np.float64))
+    sim.run_stage(capi.STAGE_SOLVE, DT)
+    ref.run_stage(capi.STAGE_SOLVE, DT)
+    assert field_rel_err(sim.get_grid(capi.FIELD_RHS), ref.get_grid(capi.FIELD_RHS)) < 1e-6
+    p_gpu, p_ref = sim.get_grid(capi.FIELD_PRESSURE), ref.get_grid(capi.FIELD_PRESSURE)
+    assert field_rel_err(p_gpu, p_ref) < 1e-6
+    for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-6, f
+    st_gpu, st_ref = sim.stats(), ref.stats()
+    # FP32 inner iterations with FP64 restarts need at most a few more iterations
+    assert st_ref.pcg_iterations - 2 <= st_gpu.pcg_iterations <= 1.5 * st_ref.pcg_iterations + 5
+    if rule == capi.STOP_REL_L2:
+        assert st_gpu.residual_rel_l2 <= 1e-10
+    else:
+        assert st_gpu.residual_abs_inf < 1e-9
+
+
+def test_solve_early_exit(gpu):
+    say("case: all-zero RHS returns 0 iterations and applies no pressure")
+    grid = scenes.grid(8)
+    sim = capi.Simulation(grid, scenes.params(gravity_enabled=0))
+    lab = np.full(grid.nc, capi.AIR, np.uint8)
+    lab.reshape(8, 8, 8)[2:5, 2:5, 2:5] = capi.WATER
+    sim.set_grid(capi.FIELD_LABEL, lab)
+    for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        sim.set_grid(f, np.zeros(grid.nc, np.float32))
+    sim.run_stage(capi.STAGE_SOLVE, DT)
+    st = sim.stats()
+    assert st.early_exit == 1 and st.pcg_iterations == 0
+    assert np.all(sim.get_grid(capi.FIELD_V2) == 0)
+
+
+@pytest.mark.parametrize("transfer", [capi.PIC, capi.FLIP, capi.APIC])
+def test_extrapolate_g2p_parity(gpu, transfer):
+    say(f"case: solve+extrapolate+g2p 16^3 transfer={transfer}")
+    sc, prm, sim, ref = _prepared_pair(transfer=transfer, tol=1e-10)
+    for stage in (capi.STAGE_SOLVE, capi.STAGE_EXTRAPOLATE):
+        sim.run_stage(stage, DT)
+        ref.run_stage(stage, DT)
+    for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-5, f
+    # G2P from identical grids: copy the oracle's grid into the GPU context
+    for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W, capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        sim.set_grid(f, ref.get_grid(f).astype(np.float32))
+    sync_oracle_from_gpu(sim, ref)
+    for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W, capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        ref.set_grid(f, sim.get_grid(f).astype(np.float64))
+    sim.run_stage(capi.STAGE_G2P, DT)
+    ref.run_stage(capi.STAGE_G2P, DT)
+    _, v_gpu, c_gpu = sim.get_particles()
+    _, v_ref, c_ref = ref.get_particles()
+    assert field_rel_err(v_gpu, v_ref) < 1e-5
+    if transfer == capi.APIC:
+        assert field_rel_err(c_gpu, c_ref) < 1e-5
+
+
+@pytest.mark.parametrize("transfer,sphere", [(capi.FLIP, False), (capi.APIC, True), (capi.PIC, False)])
+def test_whole_step_parity(gpu, transfer, sphere):
+    """Per step, from the same state: labels bit-exact, grid velocities and
+    pressure within 1e-5 of scale, particles within 1e-5 of the domain."""
+    say(f"case: whole step 24^3 transfer={transfer} sphere={sphere}, 4 steps")
+    sc = scenes.dam_break(24, transfer, sphere=sphere)
+    prm = scenes.params(transfer=transfer, tol=1e-10)
+    pos = sc.particles()
+    sim, ref = pair(sc.grid, prm, pos, obstacles=sc.obstacles)
+    for k in range(4):
+        sync_oracle_from_gpu(sim, ref)
+        st = sim.step(DT)
+        ref.step(DT)
+        assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL)), k
+        for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W):
+            assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-5, (k, f)
+        assert field_rel_err(sim.get_grid(capi.FIELD_PRESSURE), ref.get_grid(capi.FIELD_PRESSURE)) < 1e-5, k
+        p_gpu, v_gpu, _ = sim.get_particles()
+        p_ref, v_ref, _ = ref.get_particles()
+        assert np.abs(p_gpu - p_ref).max() < 1e-5, k
+        assert field_rel_err(v_gpu, v_ref) < 1e-4, k
+        assert st.residual_rel_l2 <= 1e-10
