@@ -1,0 +1,403 @@
+"""Pins of the FP64 oracle (CPU only).
+
+The reference ships no tests or golden data and cannot be built here (glm is
+absent from the image), so each oracle function is pinned by what the
+mathematics fixes: closed forms, invariants, brute force on tiny inputs, and
+dense linear algebra, each chosen so that a plausible mistake (a dropped
+term, a wrong sign or index, a transposed stencil) fails.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from fluid_simulator_amd import capi, scenes
+
+DT = scenes.DT
+
+
+def mk(grid, **kw):
+    return oracle.Oracle(grid, scenes.params(**kw))
+
+
+def test_splitmix64_known_answers():
+    # first outputs of SplitMix64 seeded with 0 (Steele, Lea, Flood 2014)
+    assert oracle.splitmix64(0, 0) == 0xE220A8397B1DCDAF
+    assert oracle.splitmix64(0, 1) == 0x6E789E6AA1B965F4
+    assert oracle.splitmix64(0, 2) == 0x06C45D188009454F
+
+
+def rand_particles(grid, n, seed):
+    rng = np.random.default_rng(seed)
+    lo = grid.cell_d + 1.01 * grid.particle_r
+    return rng.uniform(lo, grid.nx * grid.cell_d - lo, (n, 3))
+
+
+def face_positions(grid):
+    n = grid.nx
+    i = np.arange(n)
+    X, Y, Z = np.meshgrid(i, i, i, indexing="ij")
+    d = grid.cell_d
+    return [np.stack([(X + 1) * d, (Y + .5) * d, (Z + .5) * d], -1).reshape(-1, 3),
+            np.stack([(X + .5) * d, (Y + 1) * d, (Z + .5) * d], -1).reshape(-1, 3),
+            np.stack([(X + .5) * d, (Y + .5) * d, (Z + 1) * d], -1).reshape(-1, 3)]
+
+
+def test_p2g_uniform_field_reproduced():
+    grid = scenes.grid(12)
+    o = mk(grid, transfer=capi.FLIP)
+    pos = rand_particles(grid, 6000, 1)
+    o.set_particles(pos, np.tile([0.3, -1.25, 2.0], (len(pos), 1)))
+    o.run_stage(capi.STAGE_P2G)
+    for a, f in enumerate((capi.FIELD_U, capi.FIELD_V, capi.FIELD_W)):
+        v = o.get_grid(f)
+        nz = v != 0
+        assert nz.sum() > 500
+        assert np.allclose(v[nz], [0.3, -1.25, 2.0][a], rtol=1e-12, atol=0)
+
+
+def test_apic_p2g_reproduces_affine_field():
+    """APIC: v_p + C (x_f - x_p) with v_p = v0 + G x_p, C = G gives
+    v0 + G x_f at every face, whatever the weights."""
+    grid = scenes.grid(12)
+    o = mk(grid, transfer=capi.APIC)
+    pos = rand_particles(grid, 6000, 2)
+    v0 = np.array([0.1, -0.2, 0.3])
+    G = np.array([[0.5, -1.0, 0.25], [2.0, 0.1, -0.3], [-0.7, 0.4, 1.5]])
+    vel = v0 + pos @ G.T
+    c = np.tile(G.reshape(-1), (len(pos), 1))
+    o.set_particles(pos, vel, c)
+    o.run_stage(capi.STAGE_P2G)
+    fp = face_positions(grid)
+    for a, f in enumerate((capi.FIELD_U, capi.FIELD_V, capi.FIELD_W)):
+        v = o.get_grid(f)
+        nz = v != 0
+        expect = v0[a] + fp[a][nz] @ G[a]
+        assert np.allclose(v[nz], expect, rtol=0, atol=1e-12)
+
+
+def _grid_affine(grid, v0, G):
+    fp = face_positions(grid)
+    return [v0[a] + fp[a] @ G[a] for a in range(3)]
+
+
+@pytest.mark.parametrize("transfer", [capi.PIC, capi.APIC])
+def test_g2p_interpolates_affine_field_and_gradient(transfer):
+    grid = scenes.grid(12)
+    o = mk(grid, transfer=transfer)
+    pos = rand_particles(grid, 3000, 3)
+    o.set_particles(pos, np.zeros_like(pos))
+    v0 = np.array([1.0, -0.5, 0.25])
+    G = np.array([[0.5, -1.0, 0.25], [2.0, 0.1, -0.3], [-0.7, 0.4, 1.5]])
+    for f, vals in zip((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2), _grid_affine(grid, v0, G)):
+        o.set_grid(f, vals)
+    o.run_stage(capi.STAGE_G2P)
+    _, vel, c = o.get_particles()
+    assert np.allclose(vel, v0 + pos @ G.T, atol=1e-12)
+    if transfer == capi.APIC:
+        assert np.allclose(c, np.tile(G.reshape(-1), (len(pos), 1)), atol=1e-10)
+
+
+def test_flip_identity_when_grid_unchanged():
+    grid = scenes.grid(10)
+    o = mk(grid, transfer=capi.FLIP, flip_ratio=1.0)
+    pos = rand_particles(grid, 2000, 4)
+    vel = np.random.default_rng(4).standard_normal(pos.shape)
+    o.set_particles(pos, vel)
+    rng = np.random.default_rng(5)
+    for f, f2 in ((capi.FIELD_U, capi.FIELD_U2), (capi.FIELD_V, capi.FIELD_V2), (capi.FIELD_W, capi.FIELD_W2)):
+        g = rng.standard_normal(grid.nc)
+        o.set_grid(f, g)
+        o.set_grid(f2, g)
+    o.run_stage(capi.STAGE_G2P)
+    _, v2, _ = o.get_particles()
+    assert np.array_equal(v2, vel)
+
+
+def test_bin_matches_stable_argsort():
+    grid = scenes.grid(10)
+    o = mk(grid)
+    pos = rand_particles(grid, 4000, 6)
+    o.set_particles(pos)
+    count, start, perm = o.bin()
+    cell = np.floor(pos * 10).astype(int)
+    key = (cell[:, 0] * 10 + cell[:, 1]) * 10 + cell[:, 2]
+    assert np.array_equal(count, np.bincount(key, minlength=grid.nc))
+    assert np.array_equal(start, np.concatenate([[0], np.cumsum(count)]))
+    assert np.array_equal(perm, np.argsort(key, kind="stable"))
+
+
+def test_labels_brute_force_voxelisation():
+    n = 12
+    grid = scenes.grid(n)
+    sphere = capi.make_obstacle(capi.OBS_SPHERE, (0.55, 0.4, 0.5), r=0.2)
+    box = capi.make_obstacle(capi.OBS_BOX, (0.25, 0.7, 0.3), size=(0.2, 0.15, 0.33))
+    o = mk(grid, top_solid=1)
+    pos = rand_particles(grid, 1500, 7)
+    o.set_particles(pos)
+    o.set_obstacles([sphere, box])
+    for st in (capi.STAGE_P2G, capi.STAGE_MARK, capi.STAGE_OBSTACLES, capi.STAGE_BORDERS):
+        o.run_stage(st, DT)
+    lab = o.get_grid(capi.FIELD_LABEL).reshape(n, n, n)
+    expect = np.full((n, n, n), capi.AIR, np.uint8)
+    c = np.floor(pos * n).astype(int)
+    expect[c[:, 0], c[:, 1], c[:, 2]] = capi.WATER
+    idx = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1)
+    ctr = np.array(sphere.pos) * n
+    rr = 0.2 * n
+    lo = np.maximum(ctr - rr, 1).astype(int)
+    hi = np.minimum(ctr + rr, n - 2).astype(int)
+    inb = np.all((idx >= lo) & (idx <= hi), -1)
+    expect[inb & (np.sum((idx + 0.5 - ctr) ** 2, -1) < rr ** 2)] = capi.SOLID
+    bc = np.array(box.pos) * n
+    bd = np.array(box.size) * n * 0.5
+    blo = np.maximum(np.round(bc - bd), 1).astype(int)
+    bhi = np.minimum(np.round(bc + bd), n - 1).astype(int)
+    expect[blo[0]:bhi[0], blo[1]:bhi[1], blo[2]:bhi[2]] = capi.SOLID
+    expect[0], expect[-1] = capi.SOLID, capi.SOLID
+    expect[:, 0], expect[:, -1] = capi.SOLID, capi.SOLID
+    expect[:, :, 0], expect[:, :, -1] = capi.SOLID, capi.SOLID
+    assert np.array_equal(lab, expect)
+
+
+def laplacian(lab, scale):
+    """Dense-index sparse A of the fluid cells from the labels alone: the
+    7-point Laplacian with zero-flux SOLID and zero-pressure AIR neighbours."""
+    n = lab.shape
+    interior = np.zeros(n, bool)
+    interior[1:-1, 1:-1, 1:-1] = True
+    fluid = (lab == capi.WATER) & interior
+    ids = -np.ones(n, int)
+    cells = np.argwhere(fluid)   # lexicographic order = the reference's list
+    ids[tuple(cells.T)] = np.arange(len(cells))
+    rows, cols, vals = [], [], []
+    for k, (x, y, z) in enumerate(cells):
+        diag = 0
+        for d in [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]:
+            q = (x + d[0], y + d[1], z + d[2])
+            if lab[q] != capi.SOLID:
+                diag += 1
+            if fluid[q]:
+                rows.append(k)
+                cols.append(ids[q])
+                vals.append(-scale)
+        rows.append(k)
+        cols.append(k)
+        vals.append(diag * scale)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(len(cells), len(cells))), cells
+
+
+def _fluid_state(n=8, seed=8):
+    grid = scenes.grid(n)
+    rng = np.random.default_rng(seed)
+    lab = np.full((n, n, n), capi.AIR, np.uint8)
+    lab[1:-1, 1:n // 2 + 1, 1:-1] = capi.WATER
+    lab[rng.random((n, n, n)) < 0.15] = capi.AIR
+    lab[3, 2, 3] = capi.SOLID
+    lab[0], lab[-1], lab[:, 0], lab[:, :, 0], lab[:, :, -1] = (capi.SOLID,) * 5
+    o = mk(grid)
+    o.set_grid(capi.FIELD_LABEL, lab.reshape(-1))
+    return grid, o, lab
+
+
+def test_mic0_factor_satisfies_its_defining_recurrence():
+    """e_i = A_ii - sum_j (A_ij P_j)^2 - tau * sum_j A_ij P_j^2 * (fill-in of
+    row j), with the sigma safeguard, written on the dense matrix."""
+    grid, o, lab = _fluid_state()
+    scale = DT / (grid.cell_d ** 2)
+    A, cells = laplacian(lab, scale)
+    A = A.toarray()
+    P = o.solver_apply(capi.OP_FACTOR_MIC0, DT).reshape(lab.shape)[tuple(cells.T)]
+    nf = len(cells)
+    for i in range(nf):
+        e = A[i, i]
+        for j in range(i):
+            if A[i, j] == 0:
+                continue
+            e -= (A[i, j] * P[j]) ** 2
+            fill = sum(A[j, k] for k in range(j + 1, nf) if k != i and A[j, k] != 0)
+            e -= 0.97 * A[i, j] * P[j] ** 2 * fill
+        if e < 0.25 * A[i, i]:
+            e = A[i, i]
+        assert P[i] == pytest.approx(1 / np.sqrt(e), rel=1e-12)
+
+
+def test_mic0_apply_equals_dense_inverse():
+    """apply = M^-1 r with M = (E + L) E^-1 (E + L)^T, E = diag(1/P^2)."""
+    grid, o, lab = _fluid_state()
+    scale = DT / (grid.cell_d ** 2)
+    A, cells = laplacian(lab, scale)
+    A = A.toarray()
+    P = o.solver_apply(capi.OP_FACTOR_MIC0, DT).reshape(lab.shape)[tuple(cells.T)]
+    E = np.diag(1 / P ** 2)
+    L = np.tril(A, -1)
+    M = (E + L) @ np.linalg.inv(E) @ (E + L).T
+    r = np.zeros(lab.shape)
+    r[tuple(cells.T)] = np.random.default_rng(9).standard_normal(len(cells))
+    z = o.solver_apply(capi.OP_APPLY_MIC0, DT, r.reshape(-1)).reshape(lab.shape)[tuple(cells.T)]
+    assert np.allclose(z, np.linalg.solve(M, r[tuple(cells.T)]), rtol=1e-10, atol=1e-12)
+    Ax = o.solver_apply(capi.OP_APPLY_A, DT, r.reshape(-1)).reshape(lab.shape)[tuple(cells.T)]
+    assert np.allclose(Ax, A @ r[tuple(cells.T)], rtol=1e-12, atol=1e-9)
+
+
+def _dam_prepared(n=12, **kw):
+    grid = scenes.grid(n)
+    sc = scenes.dam_break(n, capi.FLIP, sphere=True)
+    o = oracle.Oracle(grid, scenes.params(**kw))
+    o.set_particles(sc.particles().astype(np.float64))
+    o.set_obstacles(sc.obstacles)
+    for _ in range(2):
+        o.step(DT)
+    for st in (capi.STAGE_P2G, capi.STAGE_MARK, capi.STAGE_OBSTACLES, capi.STAGE_BORDERS, capi.STAGE_POST_P2G):
+        o.run_stage(st, DT)
+    return grid, o
+
+
+def divergence(grid, o):
+    n = grid.nx
+    u, v, w = (o.get_grid(f).reshape(n, n, n) for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2))
+    div = np.zeros((n, n, n))
+    div[1:-1, 1:-1, 1:-1] = (u[1:-1, 1:-1, 1:-1] - u[:-2, 1:-1, 1:-1] + v[1:-1, 1:-1, 1:-1] - v[1:-1, :-2, 1:-1]
+                             + w[1:-1, 1:-1, 1:-1] - w[1:-1, 1:-1, :-2]) / grid.cell_d
+    return div
+
+
+def test_pcg_matches_direct_solve_and_projects():
+    grid, o = _dam_prepared(stop_rule=capi.STOP_REL_L2, tol=1e-12)
+    n = grid.nx
+    lab = o.get_grid(capi.FIELD_LABEL).reshape(n, n, n)
+    b_expect = -divergence(grid, o)
+    o.run_stage(capi.STAGE_SOLVE, DT)
+    A, cells = laplacian(lab, DT / grid.cell_d ** 2)
+    b = o.get_grid(capi.FIELD_RHS).reshape(n, n, n)[tuple(cells.T)]
+    assert np.allclose(b, b_expect[tuple(cells.T)], rtol=1e-12, atol=1e-9)
+    p = o.get_grid(capi.FIELD_PRESSURE).reshape(n, n, n)[tuple(cells.T)]
+    p_direct = spla.spsolve(A.tocsc(), b)
+    assert np.abs(p - p_direct).max() <= 1e-9 * np.abs(p_direct).max()
+    # the projected field is discretely divergence-free on fluid cells
+    div_after = divergence(grid, o)[tuple(cells.T)]
+    assert np.abs(div_after).max() <= 1e-9 * np.abs(b_expect).max() * 10
+    st = o.stats()
+    assert st.residual_rel_l2 <= 1e-12 and st.pcg_iterations > 0
+
+
+def test_pcg_stop_rules_and_cap():
+    grid, o = _dam_prepared(max_iterations=1)
+    o.run_stage(capi.STAGE_SOLVE, DT)
+    assert o.stats().pcg_iterations == 1   # the cap is returned when reached
+    grid, o = _dam_prepared(stop_rule=capi.STOP_ABS_INF, tol=1e-6)
+    o.run_stage(capi.STAGE_SOLVE, DT)
+    st = o.stats()
+    assert st.residual_abs_inf < 1e-5 and 0 < st.pcg_iterations < 200
+
+
+def test_divergence_free_field_exits_early():
+    """v2 = discrete curl of a stream function has zero discrete divergence:
+    sum(b^2) < 1e-7, the solve returns 0 and applies no pressure."""
+    n = 10
+    grid, o, lab = _fluid_state(n, seed=10)
+    rng = np.random.default_rng(11)
+    # edge potentials: Ax at (i+.5, J, K), Ay at (I, j+.5, K), Az at (I, J, k+.5)
+    Ax, Ay, Az = rng.standard_normal((3, n + 1, n + 1, n + 1)) * 1e-3
+    # the x-face of cell (i,j,k) lies on node plane i+1; its flux is the
+    # circulation of the surrounding y/z edges (discrete curl), likewise v, w
+    u = Az[1:, 1:, :n] - Az[1:, :n, :n] - Ay[1:, :n, 1:] + Ay[1:, :n, :n]
+    v = Ax[:n, 1:, 1:] - Ax[:n, 1:, :n] - Az[1:, 1:, :n] + Az[:n, 1:, :n]
+    w = Ay[1:, :n, 1:] - Ay[:n, :n, 1:] - Ax[:n, 1:, 1:] + Ax[:n, :n, 1:]
+    for f, val in zip((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2), (u, v, w)):
+        o.set_grid(f, val.reshape(-1))
+    assert np.abs(divergence(grid, o)).max() < 1e-12
+    o.run_stage(capi.STAGE_SOLVE, DT)
+    st = o.stats()
+    assert st.early_exit == 1 and st.pcg_iterations == 0
+    assert np.array_equal(o.get_grid(capi.FIELD_U2), u.reshape(-1))
+
+
+def test_wall_bounce_closed_form():
+    """One particle flying at +x into the wall: it moves 0.999 of the way to
+    the wall, reverses with restitution 0.3, then moves for the rest of dt."""
+    grid = scenes.grid(16)
+    o = mk(grid)
+    hi = 1.0 - (grid.cell_d + 1.01 * grid.particle_r)
+    x0, vx, dt = hi - 0.01, 2.0, 0.02
+    o.set_particles(np.array([[x0, 0.5, 0.5]]), np.array([[vx, 0.0, 0.0]]))
+    o.run_stage(capi.STAGE_ADVECT, dt)
+    p, v, _ = o.get_particles()
+    t_hit = (hi - x0) / vx
+    x1 = x0 + vx * t_hit * 0.999
+    expect = x1 + (-0.3 * vx) * (dt - 0.999 * t_hit)
+    assert p[0, 0] == pytest.approx(expect, abs=1e-15)
+    assert v[0, 0] == pytest.approx(-0.3 * vx, abs=1e-15)
+
+
+def test_advect_clamp_and_push_out_invariants():
+    grid = scenes.grid(16)
+    sphere = capi.make_obstacle(capi.OBS_SPHERE, (0.5, 0.5, 0.5), r=0.2)
+    box = capi.make_obstacle(capi.OBS_BOX, (0.2, 0.2, 0.2), size=(0.2, 0.2, 0.2))
+    o = mk(grid)
+    rng = np.random.default_rng(12)
+    pos = rand_particles(grid, 5000, 12)
+    o.set_particles(pos, rng.uniform(-8, 8, pos.shape))
+    o.set_obstacles([sphere, box])
+    o.run_stage(capi.STAGE_ADVECT, 1 / 60)
+    o.run_stage(capi.STAGE_PUSH_OUT, 1 / 60)
+    p, _, _ = o.get_particles()
+    lo = grid.cell_d + 1.01 * grid.particle_r
+    assert p.min() >= lo and p.max() <= 1 - lo
+    d = np.linalg.norm(p - 0.5, axis=1)
+    assert d.min() >= 0.2 + grid.particle_r - 1e-12
+    # box push-out moves particles to the grid-aligned bounds (0.125 .. 0.3125 here)
+    inside = np.all((p > 0.125 - grid.particle_r + 1e-12) & (p < 0.3125 + grid.particle_r - 1e-12), axis=1)
+    assert not inside.any()
+
+
+def test_spawn_count_radius_speed():
+    grid = scenes.grid(16)
+    src = capi.make_obstacle(capi.OBS_SOURCE, (0.5, 0.6, 0.4), r=0.07, spawn_rate=1000.0, spawn_speed=1.5,
+                             last_spawn_fraction=0.5)
+    o = mk(grid, seed=3)
+    o.set_particles(np.zeros((0, 3)))
+    o.set_obstacles([src])
+    o.run_stage(capi.STAGE_SPAWN, 0.0123)
+    n = o.num_particles()
+    assert n == int(1000 * 0.0123 + 0.5)
+    assert o.get_obstacles()[0].last_spawn_fraction == pytest.approx(1000 * 0.0123 + 0.5 - n)
+    p, v, _ = o.get_particles()
+    rel = p - np.array(src.pos)
+    assert np.allclose(np.linalg.norm(rel, axis=1), 0.07 + grid.particle_r, rtol=1e-13)
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.5, rtol=1e-13)
+    assert np.allclose(np.cross(rel, v), 0, atol=1e-13)
+
+
+def test_extrapolation_invariants():
+    """Faces of fluid cells and faces whose + neighbour is WATER are kept;
+    a uniform field stays uniform (mean of equal values)."""
+    n = 10
+    grid, o, lab = _fluid_state(n, seed=13)
+    for f, c in zip((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2), (0.5, -2.0, 1.25)):
+        o.set_grid(f, np.full(grid.nc, c))
+    o.run_stage(capi.STAGE_EXTRAPOLATE)
+    for f, c in zip((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2), (0.5, -2.0, 1.25)):
+        assert np.allclose(o.get_grid(f), c, rtol=1e-15)
+    rng = np.random.default_rng(14)
+    vals = [rng.standard_normal(grid.nc) for _ in range(3)]
+    for f, val in zip((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2), vals):
+        o.set_grid(f, val)
+    o.run_stage(capi.STAGE_EXTRAPOLATE)
+    interior = np.zeros((n, n, n), bool)
+    interior[1:-1, 1:-1, 1:-1] = True
+    fluid = ((lab == capi.WATER) & interior).reshape(-1)
+    for a, (f, val) in enumerate(zip((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2), vals)):
+        out = o.get_grid(f)
+        assert np.array_equal(out[fluid], val[fluid])
+        nb = np.roll(lab, -1, axis=a).reshape(-1) == capi.WATER
+        edge = np.zeros((n, n, n), bool)
+        sl = [slice(None)] * 3
+        sl[a] = -1
+        edge[tuple(sl)] = True
+        keep = nb & ~edge.reshape(-1)
+        assert np.array_equal(out[keep], val[keep])
+        assert not np.array_equal(out, val)   # something was extrapolated
