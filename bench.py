@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark of the Simulator step on one MI355X.
+
+Workload: the 256^3 FLIP (0.95) double dam break (two blocks x<0.4 and x>0.6,
+y<0.6, 8 particles per fluid cell, seed 1), dt = 1/240, MIC(0)-PCG to
+||r||_2 <= 1e-6 ||b||_2.  A "step" is one Simulator::simulate(dt): advection,
+binning, P2G, labels, pressure solve, extrapolation, G2P, with the particle
+and grid state resident in HBM.
+
+Prints one JSON line: steps/s (value), plus the roofline of the dominant
+kernel measured with HIP events on the library's stream, and the FP64 CPU
+oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from fluid_simulator_amd import capi, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+# Algorithmic bytes per unit for each kernel group (FP32 fields, uint8 labels,
+# each array counted once per launch, no cache credit).  Units: f = WATER
+# cells, c = all cells, p = particles.
+ALGO_BYTES = {
+    "sweep_fwd": ("f", 13.0),      # R r 4, R P 4, R label 1, W q 4
+    "sweep_bwd": ("f", 13.0),      # R q 4, R P 4, R label 1, W z 4
+    "mic_factor": ("f", 5.0),      # R label 1, W P 4
+    "spmv": ("f", 9.0),            # R s 4, R label 1, W q 4
+    "update": ("f", 32.0),         # R s 4, R q 4, R/W x 16 (FP64), R/W r 8
+    "p2g": ("p+c", (24.0, 12.0)),  # R particle pos+vel 24 B/p, W u,v,w 12 B/cell
+    "g2p": ("p+c", (36.0, 24.0)),  # R pos,vel W vel 36 B/p, R u,v,w,u2,v2,w2 24 B/cell
+    "move": ("p", 36.0),           # R pos,vel W pos (vel in place) 36 B/p
+    "extrapolate": ("c", 54.0),
+}
+
+
+def algo_bytes(name: str, nf: float, nc: int, np_: int) -> float | None:
+    if name not in ALGO_BYTES:
+        return None
+    unit, b = ALGO_BYTES[name]
+    if unit == "f":
+        return b * nf
+    if unit == "c":
+        return b * nc
+    if unit == "p":
+        return b * np_
+    return b[0] * np_ + b[1] * nc
+
+
+def cpu_baseline(scene, gpu_iterations: float, threads: int, pcg_cap: int) -> dict:
+    """The FP64 oracle on the same scene (initial state), one step with the
+    PCG capped at `pcg_cap` iterations; its per-iteration solve time is
+    extrapolated to the GPU's iteration count for the step."""
+    import oracle
+
+    pos = scene.particles().astype(np.float64)
+    prm = scenes.params(transfer=scene.transfer, max_iterations=pcg_cap)
+    o = oracle.Oracle(scene.grid, prm, threads=threads)
+    o.set_particles(pos)
+    del pos
+    t0 = time.perf_counter()
+    st = o.step(scenes.DT)
+    wall = time.perf_counter() - t0
+    o.close()
+    solve_s = st.timer_ms[capi.TIMER_KEYS.index("Incompressibility")] / 1e3
+    its = max(st.pcg_iterations, 1)
+    est_step = (wall - solve_s) + solve_s * (gpu_iterations / its)
+    return {"value": 1.0 / est_step, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": (f"FP64 oracle, 1 step of the {scene.n}^3 scene from its initial state with PCG capped at "
+                       f"{its} iterations ({wall:.1f} s wall, {solve_s:.1f} s solve), solve time extrapolated "
+                       f"per iteration to the GPU's {gpu_iterations:.0f} iterations")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=256, help="grid cells per axis (default 256)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-pcg-cap", type=int, default=12)
+    ap.add_argument("--profile-steps", type=int, default=None,
+                    help="steps timed per kernel with HIP events after the timed region (default: --steps)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU decomposition is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    scene = scenes.double_dam_break(args.size, capi.FLIP, seed=1)
+    grid = scene.grid
+    prm = scenes.params(transfer=scene.transfer)
+    t_setup = time.perf_counter()
+    pos = scene.particles()
+    sim = capi.Simulation(grid, prm)
+    sim.set_particles(pos)
+    n_particles = len(pos)
+    del pos
+    print(f"# setup {time.perf_counter() - t_setup:.1f}s: {args.size}^3, {n_particles} particles", file=sys.stderr,
+          flush=True)
+    for k in range(args.warmup):
+        st = sim.step(scenes.DT)
+        print(f"# warmup {k}: {st.step_ms:.1f} ms, pcg {st.pcg_iterations}", file=sys.stderr, flush=True)
+
+    # timed region: K whole steps (each fsim_step synchronises its stream)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(sim.step(scenes.DT))
+    elapsed = time.perf_counter() - t0
+    steps_per_s = args.steps / elapsed
+    its = float(np.mean([s.pcg_iterations for s in stats]))
+    solve_ms = float(np.mean([s.solve_ms for s in stats]))
+    nf = float(np.mean([s.num_fluid_cells for s in stats]))
+    print(f"# timed {args.steps} steps in {elapsed:.2f}s: {1e3 * elapsed / args.steps:.1f} ms/step, "
+          f"pcg {its:.1f} it, solve {solve_ms:.1f} ms", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event timing over the same number of further steps
+    psteps = args.profile_steps or args.steps
+    sim.set_option(capi.OPT_PROFILE, 1)
+    sim.reset_kernel_times()
+    pstats = [sim.step(scenes.DT) for _ in range(psteps)]
+    sim.set_option(capi.OPT_PROFILE, 0)
+    kt = sim.kernel_times()
+    nf_p = float(np.mean([s.num_fluid_cells for s in pstats]))
+    dominant = max(kt, key=lambda k: kt[k][0])
+    total_ms, launches = kt[dominant]
+    avg_ms = total_ms / max(launches, 1)
+    b = algo_bytes(dominant, nf_p, grid.nc, n_particles)
+    roofline = None
+    if b is not None and avg_ms > 0:
+        achieved = b / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_ms": round(avg_ms, 4), "launches": launches,
+                    "bytes_per_launch": b}
+    breakdown = {k: round(v[0] / psteps, 3) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0]) if v[1] > 0}
+    print(f"# per-step kernel ms: {json.dumps(breakdown)}", file=sys.stderr, flush=True)
+    sim.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, its, args.cpu_threads, args.cpu_pcg_cap)
+
+    out = {
+        "metric": "steps/s @256^3 FLIP",
+        "value": round(steps_per_s, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 dots, residual and pressure)",
+        "data": "synthetic",
+        "config": {"workload": f"double dam break {args.size}^3 FLIP 0.95, {n_particles} particles, dt 1/240, "
+                               "MIC(0)-PCG to 1e-6 relative residual",
+                   "grid": [args.size] * 3, "particles": n_particles, "fluid_cells": round(nf),
+                   "pcg_iterations": round(its, 1), "pressure_solve_ms": round(solve_ms, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernel_ms_per_step": breakdown,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -99,7 +99,13 @@ PROTOTYPES = {
     "fsim_solver_apply": (C.c_int, [_P, C.c_int, C.c_double, _P, _P]),
     "fsim_scene_dam_break": (C.c_int, [C.POINTER(GridDesc), C.c_int, _P, C.c_int, C.POINTER(Obstacle), C.c_uint64,
                                        C.c_int64, _P, C.POINTER(C.c_int64)]),
+    "fsim_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
+    "fsim_num_kernels": (C.c_int, []),
+    "fsim_kernel_name": (C.c_char_p, [C.c_int]),
+    "fsim_get_kernel_time": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "fsim_reset_kernel_times": (C.c_int, [_P]),
 }
+OPT_SWEEP_MODE, OPT_PROFILE = 0, 1
 
 _lib = None
 
@@ -290,6 +296,21 @@ class Simulation:
     def set_grid(self, field: int, values: np.ndarray):
         a = np.ascontiguousarray(values, dtype=FIELD_DTYPES[field]).reshape(-1)
         self._check(self.lib.fsim_set_grid(self._ctx, field, _ptr(a), a.nbytes))
+
+    def set_option(self, option: int, value: int):
+        self._check(self.lib.fsim_set_option(self._ctx, option, value))
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (total ms, launches)} accumulated while profiling is on."""
+        out = {}
+        for k in range(self.lib.fsim_num_kernels()):
+            ms, n = C.c_double(), C.c_int64()
+            self._check(self.lib.fsim_get_kernel_time(self._ctx, k, C.byref(ms), C.byref(n)))
+            out[self.lib.fsim_kernel_name(k).decode()] = (ms.value, n.value)
+        return out
+
+    def reset_kernel_times(self):
+        self._check(self.lib.fsim_reset_kernel_times(self._ctx))
 
     def solver_apply(self, op: int, dt: float, vec: np.ndarray | None = None) -> np.ndarray:
         out = np.empty(self.grid.nc, np.float64)

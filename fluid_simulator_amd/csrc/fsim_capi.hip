@@ -27,6 +27,51 @@ int set_err(fsim_ctx* ctx, int code, const std::string& msg) {
 
 int stage_p2g_fused(fsim_ctx* ctx);
 
+static const char* const kKernelNames[KID_COUNT] = {
+    "spawn", "move", "bin_keys", "bin_scan", "bin_scatter", "bin_segsort", "bin_gather", "p2g", "mark",
+    "obstacles", "borders", "post_p2g", "rhs", "mic_factor", "sweep_fwd", "sweep_bwd", "spmv", "update", "dot",
+    "xpay", "true_residual", "finalize", "pressure", "extrapolate", "g2p"};
+
+int set_profiling(fsim_ctx* ctx, bool on) {
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FSIM_TRY(prof_collect(ctx));
+    ctx->prof_on = on;
+    return FSIM_OK;
+}
+
+int prof_event(fsim_ctx* ctx) {
+    if (!ctx->prof_on) return -1;
+    if (ctx->prof_next == ctx->prof_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->prof_pool.push_back(e);
+    }
+    const int idx = (int)ctx->prof_next++;
+    if (hipEventRecord(ctx->prof_pool[idx], ctx->stream) != hipSuccess) return -1;
+    return idx;
+}
+
+void prof_close(fsim_ctx* ctx, int kid, int start) {
+    const int end = prof_event(ctx);
+    if (end < 0) return;
+    ctx->prof_pending.push_back(kid);
+    ctx->prof_pending.push_back(start);
+    ctx->prof_pending.push_back(end);
+}
+
+int prof_collect(fsim_ctx* ctx) {
+    for (size_t k = 0; k + 2 < ctx->prof_pending.size(); k += 3) {
+        float ms = 0.f;
+        FSIM_HIP(ctx, hipEventElapsedTime(&ms, ctx->prof_pool[ctx->prof_pending[k + 1]],
+                                          ctx->prof_pool[ctx->prof_pending[k + 2]]));
+        ctx->prof_ms[ctx->prof_pending[k]] += ms;
+        ctx->prof_n[ctx->prof_pending[k]] += 1;
+    }
+    ctx->prof_pending.clear();
+    ctx->prof_next = 0;
+    return FSIM_OK;
+}
+
 }  // namespace fsim
 
 using namespace fsim;
@@ -59,12 +104,14 @@ static void free_all(fsim_ctx* ctx) {
     void* ptrs[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed, ctx->u, ctx->v, ctx->w,
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
                     ctx->scan_tmp, ctx->nbinfo, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
-                    ctx->x, ctx->partials, ctx->scal, ctx->obs_dev};
+                    ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->W, ctx->ybnd, ctx->zbnd,
+                    ctx->wave_ticket, ctx->wave_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
@@ -148,6 +195,8 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(FSIM_ERR_HIP);
     if (particles_reserve(ctx, 1024) != FSIM_OK) return fail(FSIM_ERR_OOM);
+    if ((rc = wave_init(ctx)) != FSIM_OK) return fail(rc);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(FSIM_ERR_HIP);
     *out = ctx;
     return FSIM_OK;
 }
@@ -308,7 +357,7 @@ int fsim_run_stage(fsim_ctx* ctx, int stage, double dt) {
     if (!ctx) return FSIM_ERR_INVALID;
     FSIM_TRY(run_stage(ctx, stage, dt));
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return FSIM_OK;
+    return prof_collect(ctx);
 }
 
 int fsim_step(fsim_ctx* ctx, double dt, fsim_step_stats* stats) {
@@ -352,7 +401,7 @@ int fsim_step(fsim_ctx* ctx, double dt, fsim_step_stats* stats) {
     st.num_particles = ctx->P.n;
     st.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
-    return FSIM_OK;
+    return prof_collect(ctx);
 }
 
 int fsim_get_grid(fsim_ctx* ctx, int field, void* dst, size_t bytes) {
@@ -402,6 +451,41 @@ int fsim_set_grid(fsim_ctx* ctx, int field, const void* src, size_t bytes) {
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     FSIM_HIP(ctx, hipMemcpy(dst, src, need, hipMemcpyHostToDevice));
     return FSIM_OK;
+}
+
+int fsim_num_kernels(void) { return KID_COUNT; }
+
+const char* fsim_kernel_name(int kid) { return (kid >= 0 && kid < KID_COUNT) ? kKernelNames[kid] : "unknown"; }
+
+int fsim_get_kernel_time(fsim_ctx* ctx, int kid, double* total_ms, int64_t* launches) {
+    if (!ctx || kid < 0 || kid >= KID_COUNT) return FSIM_ERR_INVALID;
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FSIM_TRY(prof_collect(ctx));
+    if (total_ms) *total_ms = ctx->prof_ms[kid];
+    if (launches) *launches = ctx->prof_n[kid];
+    return FSIM_OK;
+}
+
+int fsim_reset_kernel_times(fsim_ctx* ctx) {
+    if (!ctx) return FSIM_ERR_INVALID;
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FSIM_TRY(prof_collect(ctx));
+    std::fill(std::begin(ctx->prof_ms), std::end(ctx->prof_ms), 0.0);
+    std::fill(std::begin(ctx->prof_n), std::end(ctx->prof_n), 0);
+    return FSIM_OK;
+}
+
+int fsim_set_option(fsim_ctx* ctx, int option, int value) {
+    if (!ctx) return FSIM_ERR_INVALID;
+    switch (option) {
+    case FSIM_OPT_SWEEP_MODE:
+        if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "sweep mode is 0 or 1");
+        ctx->sweep_mode = value;
+        return FSIM_OK;
+    case FSIM_OPT_PROFILE:
+        return set_profiling(ctx, value != 0);
+    default: return set_err(ctx, FSIM_ERR_INVALID, "unknown option");
+    }
 }
 
 int fsim_solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out) {

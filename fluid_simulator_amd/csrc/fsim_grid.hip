@@ -155,6 +155,7 @@ static int p2g(fsim_ctx* ctx, int fuse_mark) {
         FSIM_TRY(fill_f32(ctx, ctx->P.c[0], 0.0f, (int64_t)ctx->P.cap * 9));
         ctx->c_valid = true;
     }
+    FSIM_PROF(ctx, KID_P2G);
     P2GArgs A{};
     A.G = grid_args(ctx);
     for (int a = 0; a < 3; a++) {
@@ -219,6 +220,7 @@ __global__ void __launch_bounds__(kBlock) k_mark(P2GArgs A, int with_density) {
 }
 
 int stage_mark(fsim_ctx* ctx) {
+    FSIM_PROF(ctx, KID_MARK);
     P2GArgs A{};
     A.G = grid_args(ctx);
     for (int a = 0; a < 3; a++) A.pos[a] = ctx->P.pos[a];
@@ -292,6 +294,7 @@ __global__ void __launch_bounds__(kBlock) k_obstacle_labels(ObsArgs A) {
 }
 
 int stage_obstacles(fsim_ctx* ctx) {
+    FSIM_PROF(ctx, KID_OBSTACLES);
     const int n[3] = {ctx->g.nx, ctx->g.ny, ctx->g.nz};
     for (const fsim_obstacle& o : ctx->obs) {
         ObsArgs A{};
@@ -400,6 +403,7 @@ __global__ void __launch_bounds__(kBlock) k_border_labels(GridArgs G, uint8_t* l
 }
 
 int stage_borders(fsim_ctx* ctx) {
+    FSIM_PROF(ctx, KID_BORDERS);
     BorderArgs A{};
     A.G = grid_args(ctx);
     A.label = ctx->label;
@@ -430,6 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_post_p2g(GridArgs G, const float* __
 }
 
 int stage_post_p2g(fsim_ctx* ctx, double grav_inc) {
+    FSIM_PROF(ctx, KID_POST_P2G);
     hipLaunchKernelGGL(k_post_p2g, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, grid_args(ctx), ctx->u,
                        ctx->v, ctx->w, ctx->u2, ctx->v2, ctx->w2, ctx->label, (float)grav_inc);
     FSIM_HIP(ctx, hipGetLastError());
@@ -489,6 +494,7 @@ __global__ void __launch_bounds__(kBlock) k_extrapolate(GridArgs G, float* __res
 }
 
 int stage_extrapolate(fsim_ctx* ctx) {
+    FSIM_PROF(ctx, KID_EXTRAPOLATE);
     GridArgs G = grid_args(ctx);
     hipLaunchKernelGGL(k_valid_init, dim3(nblocks(G.nc)), dim3(kBlock), 0, ctx->stream, G, ctx->label, ctx->valid);
     for (int it = 0; it < 2; it++)
@@ -586,6 +592,7 @@ int stage_g2p(fsim_ctx* ctx) {
     A.flip = ctx->prm.flip_ratio;
     A.one_minus_flip = 1.0f - ctx->prm.flip_ratio;
     if (A.transfer == FSIM_APIC) ctx->c_valid = true;
+    FSIM_PROF(ctx, KID_G2P);
     hipLaunchKernelGGL(k_g2p, dim3(nblocks(A.n)), dim3(kBlock), 0, ctx->stream, A);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;

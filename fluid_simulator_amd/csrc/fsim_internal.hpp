@@ -37,6 +37,34 @@ constexpr uint16_t NB_PX = 1u << 4;         // +x neighbour is WATER (fluid)
 constexpr uint16_t NB_PY = 1u << 5;
 constexpr uint16_t NB_PZ = 1u << 6;
 constexpr int NB_NONSOLID_SHIFT = 7;        // bits 7..9: number of non-SOLID neighbours
+// bits 10..15: for each axis a, whether the -a neighbour's +((a+1)%3) and
+// +((a+2)%3) neighbours are WATER (the MIC(0) factor's "other" terms)
+constexpr int NB_UP_SHIFT = 10;
+
+// persistent wavefront kernels: tile of WF_TY x WF_TZ (y, z) columns, one
+// compute wave per y row plus two communication waves
+constexpr int WF_TY = 14;
+constexpr int WF_TZ = 64;
+constexpr int WF_THREADS = (WF_TY + 2) * WF_TZ;
+
+struct WaveArgs {
+    int nx, ny, nz;
+    int TJ, TK;                   // tiles along y and z
+    const uint16_t* nbinfo;
+    const float* in;
+    float* out;
+    float ms;                     // -scale in FP32
+    double scale;                 // dt / (rho dx^2)
+    double* P64;
+    float* P32;
+    uint64_t* W;                  // packed (nbinfo << 32 | P32 bits) per cell
+    uint64_t* ybnd;               // boundary packets across y tile edges
+    uint64_t* zbnd;               // boundary packets across z tile edges
+    unsigned long long* ticket;
+    unsigned long long ticket_base;
+    uint32_t epoch;
+    int* err;
+};
 
 struct GridDims {
     int nx, ny, nz;
@@ -52,8 +80,17 @@ struct SolveScalars {
     double rr, rmax, bb;             // recursive residual ||r||^2, max|r|, ||b||^2
     double tr_ss, tr_max;            // true residual ||b-Ax||^2, max|b-Ax|
     double zr;                       // z.r
+    double nfluid;                   // number of WATER cells of the solve
     int nan_flag;
     int pad;
+};
+
+// Kernel groups timed with HIP events when profiling is on.
+enum KernelId {
+    KID_SPAWN = 0, KID_MOVE, KID_BIN_KEYS, KID_BIN_SCAN, KID_BIN_SCATTER, KID_BIN_SEGSORT, KID_BIN_GATHER,
+    KID_P2G, KID_MARK, KID_OBSTACLES, KID_BORDERS, KID_POST_P2G, KID_RHS, KID_MIC_FACTOR, KID_SWEEP_FWD,
+    KID_SWEEP_BWD, KID_SPMV, KID_UPDATE, KID_DOT, KID_XPAY, KID_TRUE_RES, KID_FINALIZE, KID_PRESSURE,
+    KID_EXTRAPOLATE, KID_G2P, KID_COUNT
 };
 
 struct Particles {
@@ -105,6 +142,16 @@ struct fsim_ctx {
     fsim::SolveScalars* scal = nullptr;   // device
     fsim::SolveScalars* scal_host = nullptr;  // pinned host mirror
     double last_dt = 0.0;
+    // wavefront sweeps
+    fsim::WaveArgs wave{};
+    uint64_t* W = nullptr;
+    uint64_t* ybnd = nullptr;
+    uint64_t* zbnd = nullptr;
+    unsigned long long* wave_ticket = nullptr;   // 3 counters: factor, fwd, bwd
+    unsigned long long wave_ticket_base[3] = {0, 0, 0};
+    uint32_t wave_epoch = 0;
+    int* wave_err = nullptr;
+    int sweep_mode = 0;                          // 0 wavefront, 1 one launch per plane
 
     // obstacles
     std::vector<fsim_obstacle> obs;
@@ -114,6 +161,13 @@ struct fsim_ctx {
     // timing
     hipEvent_t ev[2 * FSIM_NUM_TIMERS + 4] = {};
     fsim_step_stats stats{};
+    // per-kernel profiling
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_pool;
+    size_t prof_next = 0;
+    std::vector<int> prof_pending;   // triples: kernel id, start event, end event
+    double prof_ms[fsim::KID_COUNT] = {};
+    int64_t prof_n[fsim::KID_COUNT] = {};
 };
 
 namespace fsim {
@@ -134,6 +188,25 @@ int set_err(fsim_ctx* ctx, int code, const std::string& msg);
     } while (0)
 
 inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+// ---- profiling (fsim_capi.hip) ----------------------------------------------
+int set_profiling(fsim_ctx* ctx, bool on);
+int prof_event(fsim_ctx* ctx);          // records a pooled event, returns its index (-1 if off)
+void prof_close(fsim_ctx* ctx, int kid, int start);
+int prof_collect(fsim_ctx* ctx);        // after a stream synchronisation
+
+// Times the enclosed launches as kernel group `kid` when profiling is on.
+struct ProfRange {
+    fsim_ctx* c;
+    int kid, start;
+    ProfRange(fsim_ctx* ctx, int k) : c(ctx), kid(k), start(prof_event(ctx)) {}
+    ~ProfRange() {
+        if (start >= 0) prof_close(c, kid, start);
+    }
+};
+#define FSIM_CAT2(a, b) a##b
+#define FSIM_CAT(a, b) FSIM_CAT2(a, b)
+#define FSIM_PROF(ctx, kid) ::fsim::ProfRange FSIM_CAT(prof_range_, __LINE__)((ctx), (kid))
 
 // ---- particle stages (fsim_particles.hip) ---------------------------------
 int particles_reserve(fsim_ctx* ctx, int64_t n);
@@ -158,6 +231,9 @@ int fill_i32(fsim_ctx* ctx, int32_t* p, int32_t v, int64_t n);
 
 // ---- pressure solve (fsim_solver.hip) -------------------------------------
 int stage_solve(fsim_ctx* ctx, double dt);
+// kind 0 = MIC(0) factor, 1 = forward sweep, 2 = backward sweep
+int wave_launch(fsim_ctx* ctx, int kind, const float* in, float* out, double scale);
+int wave_init(fsim_ctx* ctx);
 int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out);
 
 }  // namespace fsim

@@ -269,6 +269,7 @@ int upload_obstacles(fsim_ctx* ctx) {
 
 int stage_move(fsim_ctx* ctx, double dt, int flags) {
     if (ctx->P.n == 0) return FSIM_OK;
+    FSIM_PROF(ctx, KID_MOVE);
     if (!ctx->prm.stop_particles) flags &= ~4;
     MoveArgs A{};
     for (int a = 0; a < 3; a++) {
@@ -345,6 +346,7 @@ int stage_spawn(fsim_ctx* ctx, double dt) {
         total += num;
     }
     FSIM_TRY(upload_obstacles(ctx));
+    FSIM_PROF(ctx, KID_SPAWN);
     if (total > 0) {
         FSIM_TRY(particles_reserve(ctx, ctx->P.n + total));
         int64_t base = ctx->P.n;
@@ -614,17 +616,27 @@ int stage_bin(fsim_ctx* ctx) {
     // counts of Nc cells + 1 bucket for removed particles
     FSIM_TRY(fill_i32(ctx, ctx->cell_count, 0, nc + 1));
     if (n > 0) {
+        FSIM_PROF(ctx, KID_BIN_KEYS);
         hipLaunchKernelGGL(k_bin_keys, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->P.pos[0], ctx->P.pos[1],
                            ctx->P.pos[2], ctx->removed, ctx->key, ctx->rank, ctx->cell_count, n, ctx->g.cdinv,
                            ctx->g.nx, ctx->g.ny, ctx->g.nz, (uint32_t)nc);
         FSIM_HIP(ctx, hipGetLastError());
     }
-    FSIM_TRY(exclusive_scan(ctx, ctx->cell_count, ctx->cell_start, ctx->scan_tmp, nc + 1));
+    {
+        FSIM_PROF(ctx, KID_BIN_SCAN);
+        FSIM_TRY(exclusive_scan(ctx, ctx->cell_count, ctx->cell_start, ctx->scan_tmp, nc + 1));
+    }
     if (n == 0) return FSIM_OK;
+    {
+    FSIM_PROF(ctx, KID_BIN_SCATTER);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->key, ctx->rank,
                        ctx->cell_start, ctx->perm, n);
+    }
+    {
+    FSIM_PROF(ctx, KID_BIN_SEGSORT);
     hipLaunchKernelGGL(k_bin_segsort, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, ctx->cell_start, ctx->perm,
                        ctx->P.pid, nc);
+    }
     FSIM_HIP(ctx, hipGetLastError());
     int64_t n_new = n;
     if (ctx->prm.despawning_enabled) {
@@ -656,6 +668,7 @@ int stage_bin(fsim_ctx* ctx) {
     G.perm = ctx->perm;
     G.n = n_new;
     if (n_new > 0) {
+        FSIM_PROF(ctx, KID_BIN_GATHER);
         hipLaunchKernelGGL(k_bin_gather, dim3(nblocks(n_new)), dim3(kBlock), 0, ctx->stream, G);
         FSIM_HIP(ctx, hipGetLastError());
     }

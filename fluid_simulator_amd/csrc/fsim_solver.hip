@@ -46,13 +46,16 @@ __device__ inline double wave_max(double v) {
     for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_down(v, d, 64));
     return v;
 }
-// Writes the block's sum (and max) to partials[blockIdx.x] (and
-// partials[gridDim.x + blockIdx.x]).
-__device__ inline void block_reduce_store(double sum, double mx, double* partials, bool with_max) {
+// Writes the block's sum to partials[blockIdx.x] and, for aux_mode 1 (max)
+// or 2 (sum), the reduced aux value to partials[gridDim.x + blockIdx.x].
+constexpr int AUX_NONE = 0, AUX_MAX = 1, AUX_SUM = 2;
+__device__ inline void block_reduce_store(double sum, double mx, double* partials, int aux_mode) {
     __shared__ double lds_s[kBlock / 64], lds_m[kBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool with_max = aux_mode != AUX_NONE;
     sum = wave_sum(sum);
-    if (with_max) mx = wave_max(mx);
+    if (aux_mode == AUX_MAX) mx = wave_max(mx);
+    if (aux_mode == AUX_SUM) mx = wave_sum(mx);
     if (lane == 0) {
         lds_s[wave] = sum;
         lds_m[wave] = mx;
@@ -62,7 +65,7 @@ __device__ inline void block_reduce_store(double sum, double mx, double* partial
         double s = 0, m = 0;
         for (int w = 0; w < kBlock / 64; w++) {
             s += lds_s[w];
-            m = fmax(m, lds_m[w]);
+            m = aux_mode == AUX_SUM ? m + lds_m[w] : fmax(m, lds_m[w]);
         }
         partials[blockIdx.x] = s;
         if (with_max) partials[gridDim.x + blockIdx.x] = m;
@@ -76,9 +79,11 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
     __shared__ double lds_s[kBlock], lds_m[kBlock];
     double s = 0, m = 0;
     const bool with_max = op == FIN_RES || op == FIN_TRUE;
+    const bool aux_sum = op == FIN_BB;
     for (int i = threadIdx.x; i < nparts; i += kBlock) {
         s += partials[i];
         if (with_max) m = fmax(m, partials[nparts + i]);
+        if (aux_sum) m += partials[nparts + i];
     }
     lds_s[threadIdx.x] = s;
     lds_m[threadIdx.x] = m;
@@ -88,10 +93,10 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
     m = 0;
     for (int i = 0; i < kBlock; i++) {
         s += lds_s[i];
-        m = fmax(m, lds_m[i]);
+        m = aux_sum ? m + lds_m[i] : fmax(m, lds_m[i]);
     }
     switch (op) {
-    case FIN_BB: sc->bb = s; break;
+    case FIN_BB: sc->bb = s; sc->nfluid = m; break;
     case FIN_SIGMA0: sc->sigma = s; break;
     case FIN_ALPHA: {
         sc->sq = s;
@@ -135,9 +140,9 @@ __device__ inline bool is_fluid(const SolveArgs& S, const uint8_t* label, int x,
 __global__ void __launch_bounds__(kBlock) k_nbinfo_rhs(RhsArgs A, int with_rhs) {
 #pragma clang fp contract(off)
     const SolveArgs& S = A.S;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double bsq = 0.0;
-    if (i < S.nc) {
+    double bsq = 0.0, nfl = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.nc; i += stride) {
         const int z = (int)(i % S.nz);
         const int y = (int)((i / S.nz) % S.ny);
         const int x = (int)(i / ((int64_t)S.nz * S.ny));
@@ -146,6 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_nbinfo_rhs(RhsArgs A, int with_rhs) 
         if (is_fluid(S, A.label, x, y, z)) {
             const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
             info = NB_FLUID;
+            nfl += 1.0;
             int nonsolid = 0;
             const int p[3] = {x, y, z};
             for (int a = 0; a < 3; a++) {
@@ -159,19 +165,31 @@ __global__ void __launch_bounds__(kBlock) k_nbinfo_rhs(RhsArgs A, int with_rhs) 
                 if (is_fluid(S, A.label, qm[0], qm[1], qm[2])) info |= (uint16_t)(NB_MX << a);
             }
             info |= (uint16_t)(nonsolid << NB_NONSOLID_SHIFT);
+            // for each -a neighbour that is WATER: are its +(a+1)%3 and
+            // +(a+2)%3 neighbours WATER (the MIC(0) "other" off-diagonals)
+            for (int a = 0; a < 3; a++) {
+                if (!(info & (NB_MX << a))) continue;
+                for (int k = 0; k < 2; k++) {
+                    const int b = (a + 1 + k) % 3;
+                    int q[3] = {p[0], p[1], p[2]};
+                    q[a] -= 1;
+                    q[b] += 1;
+                    if (is_fluid(S, A.label, q[0], q[1], q[2])) info |= (uint16_t)(1u << (NB_UP_SHIFT + 2 * a + k));
+                }
+            }
             if (with_rhs) {
                 double dv = (double)A.u2[i] + (double)A.v2[i] + (double)A.w2[i] - (double)A.u2[i - st[0]] -
                             (double)A.v2[i - st[1]] - (double)A.w2[i - st[2]];
                 double bd = -A.inv_dx * dv +
                             (A.pressure_enabled ? ((double)A.density[i] - A.avg_p) * A.pk : 0.0);
-                bsq = bd * bd;
+                bsq += bd * bd;
                 bv = (float)bd;
             }
         }
         A.nbinfo[i] = info;
         if (with_rhs) A.b[i] = bv;
     }
-    if (with_rhs) block_reduce_store(bsq, 0.0, A.partials, false);
+    if (with_rhs) block_reduce_store(bsq, nfl, A.partials, AUX_SUM);
 }
 
 // ---- MIC(0) factor, one i+j+k plane per launch (FP64) -----------------------------
@@ -205,11 +223,10 @@ __global__ void __launch_bounds__(kBlock) k_mic_factor_plane(PlaneArgs A, double
     for (int a = 0; a < 3; a++) {
         if (!(info & (NB_MX << a))) continue;
         int64_t j = i - st[a];
-        uint16_t nj = A.nbinfo[j];
         double pj = P64[j];
         double ap = -scale * pj;   // xWater of the -neighbour times its precon
-        double o1 = (nj & (NB_PX << ((a + 1) % 3))) ? -scale : 0.0;
-        double o2 = (nj & (NB_PX << ((a + 2) % 3))) ? -scale : 0.0;
+        double o1 = (info & (1u << (NB_UP_SHIFT + 2 * a))) ? -scale : 0.0;
+        double o2 = (info & (1u << (NB_UP_SHIFT + 2 * a + 1))) ? -scale : 0.0;
         eneg += ap * ap;
         eneg_tau += ap * (o1 + o2) * pj;
     }
@@ -223,9 +240,19 @@ __global__ void __launch_bounds__(kBlock) k_mic_factor_plane(PlaneArgs A, double
     P32[i] = (float)p;
 }
 
+// (P32, nbinfo) packed per cell for the sweeps
+__global__ void __launch_bounds__(kBlock) k_pack_w(int64_t n, const uint16_t* __restrict__ nbinfo,
+                                                   const float* __restrict__ P32, uint64_t* __restrict__ W) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    W[i] = ((uint64_t)nbinfo[i] << 32) | __float_as_uint(P32[i]);
+}
+
 // ---- MIC(0) apply: forward and backward sweeps, one plane per launch (FP32) -------
+// Contraction is off in both sweep implementations so that they round alike.
 __global__ void __launch_bounds__(kBlock) k_fwd_plane(PlaneArgs A, const float* __restrict__ r, float* __restrict__ q,
                                                       const float* __restrict__ P) {
+#pragma clang fp contract(off)
     const SolveArgs& S = A.S;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int x, y, z;
@@ -238,12 +265,13 @@ __global__ void __launch_bounds__(kBlock) k_fwd_plane(PlaneArgs A, const float* 
     float qneg = 0.f;
 #pragma unroll
     for (int a = 0; a < 3; a++)
-        if (info & (NB_MX << a)) qneg += ms * q[i - st[a]] * P[i - st[a]];
+        if (info & (NB_MX << a)) qneg += ms * (q[i - st[a]] * P[i - st[a]]);
     q[i] = (r[i] - qneg) * P[i];
 }
 
 __global__ void __launch_bounds__(kBlock) k_bwd_plane(PlaneArgs A, const float* __restrict__ q, float* __restrict__ z,
                                                       const float* __restrict__ P) {
+#pragma clang fp contract(off)
     const SolveArgs& S = A.S;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int x, y, zc;
@@ -264,9 +292,9 @@ __global__ void __launch_bounds__(kBlock) k_bwd_plane(PlaneArgs A, const float* 
 __global__ void __launch_bounds__(kBlock) k_spmv_dot(SolveArgs S, const uint16_t* __restrict__ nbinfo,
                                                      const float* __restrict__ s, float* __restrict__ q,
                                                      double* __restrict__ partials) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
-    if (i < S.nc) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.nc; i += stride) {
         uint16_t info = nbinfo[i];
         float val = 0.f;
         if (info & NB_FLUID) {
@@ -280,35 +308,38 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(SolveArgs S, const uint16_t
 #pragma unroll
             for (int a = 0; a < 3; a++)
                 if (info & (NB_MX << a)) val += ms * s[i - st[a]];
-            d = (double)s[i] * (double)val;
+            d += (double)s[i] * (double)val;
         }
         q[i] = val;
     }
-    block_reduce_store(d, 0.0, partials, false);
+    block_reduce_store(d, 0.0, partials, AUX_NONE);
 }
 
 // x += alpha s (FP64), r -= alpha q, with ||r||^2 and max|r| partials
 __global__ void __launch_bounds__(kBlock) k_update(int64_t n, const uint16_t* __restrict__ nbinfo, const float* __restrict__ s,
                                                    const float* __restrict__ q, double* __restrict__ x, float* __restrict__ r,
                                                    const SolveScalars* __restrict__ sc, double* __restrict__ partials) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0, mx = 0.0;
-    if (i < n && (nbinfo[i] & NB_FLUID)) {
-        const double alpha = sc->alpha;
+    const double alpha = sc->alpha;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (!(nbinfo[i] & NB_FLUID)) continue;
         x[i] += (double)s[i] * alpha;
         float rn = r[i] + q[i] * (float)(-alpha);
         r[i] = rn;
-        ss = (double)rn * (double)rn;
-        mx = fabs((double)rn);
+        ss += (double)rn * (double)rn;
+        mx = fmax(mx, fabs((double)rn));
     }
-    block_reduce_store(ss, mx, partials, true);
+    block_reduce_store(ss, mx, partials, AUX_MAX);
 }
 
 __global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const uint16_t* __restrict__ nbinfo, const float* __restrict__ a,
                                                 const float* __restrict__ b, double* __restrict__ partials) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double d = (i < n && (nbinfo[i] & NB_FLUID)) ? (double)a[i] * (double)b[i] : 0.0;
-    block_reduce_store(d, 0.0, partials, false);
+    double d = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        if (nbinfo[i] & NB_FLUID) d += (double)a[i] * (double)b[i];
+    block_reduce_store(d, 0.0, partials, AUX_NONE);
 }
 
 // s = z + beta s (multSelfAndAdd :229-242)
@@ -324,9 +355,9 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(SolveArgs S, const uin
                                                           const float* __restrict__ b, const double* __restrict__ x,
                                                           float* __restrict__ r, int replace, double* __restrict__ partials) {
 #pragma clang fp contract(off)
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0, mx = 0.0;
-    if (i < S.nc) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.nc; i += stride) {
         uint16_t info = nbinfo[i];
         if (info & NB_FLUID) {
             const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
@@ -339,12 +370,12 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(SolveArgs S, const uin
             for (int a = 0; a < 3; a++)
                 if (info & (NB_MX << a)) val += -S.scale * x[i - st[a]];
             double t = (double)b[i] - val;
-            ss = t * t;
-            mx = fabs(t);
+            ss += t * t;
+            mx = fmax(mx, fabs(t));
             if (replace) r[i] = (float)t;
         }
     }
-    block_reduce_store(ss, mx, partials, true);
+    block_reduce_store(ss, mx, partials, AUX_MAX);
 }
 
 // applyPressureToVelocities (:295-327), FP64 pressure
@@ -385,6 +416,24 @@ static SolveArgs solve_args(const fsim_ctx* ctx, double dt) {
     return S;
 }
 
+// A wavefront kernel whose boundary poll ran out of spins sets wave_err; its
+// results are then invalid and the call fails.
+static int check_wave_err(fsim_ctx* ctx) {
+    if (ctx->sweep_mode != 0) return FSIM_OK;
+    int err = 0;
+    FSIM_HIP(ctx, hipMemcpyAsync(&err, ctx->wave_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) {
+        FSIM_HIP(ctx, hipMemsetAsync(ctx->wave_err, 0, sizeof(int), ctx->stream));
+        return set_err(ctx, FSIM_ERR_TIMEOUT, "wavefront sweep boundary wait timed out");
+    }
+    return FSIM_OK;
+}
+
+// Reductions use at most this many blocks (grid-stride), so the single-block
+// finalize sums few partials, in a fixed order.
+static unsigned red_blocks(int64_t n) { return std::min<unsigned>(nblocks(n), 1024u); }
+
 static int read_scalars(fsim_ctx* ctx) {
     FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -392,12 +441,14 @@ static int read_scalars(fsim_ctx* ctx) {
 }
 
 static int finalize(fsim_ctx* ctx, int nparts, int op) {
+    FSIM_PROF(ctx, KID_FINALIZE);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, nparts, op, ctx->scal);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
 
 static int build_nbinfo(fsim_ctx* ctx, const SolveArgs& S, bool with_rhs) {
+    FSIM_PROF(ctx, KID_RHS);
     RhsArgs A{};
     A.S = S;
     A.label = ctx->label;
@@ -412,17 +463,19 @@ static int build_nbinfo(fsim_ctx* ctx, const SolveArgs& S, bool with_rhs) {
     A.pk = ctx->prm.pressure_k;
     A.pressure_enabled = ctx->prm.pressure_enabled;
     A.partials = ctx->partials;
-    hipLaunchKernelGGL(k_nbinfo_rhs, dim3(nblocks(S.nc)), dim3(kBlock), 0, ctx->stream, A, with_rhs ? 1 : 0);
+    hipLaunchKernelGGL(k_nbinfo_rhs, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, A, with_rhs ? 1 : 0);
     FSIM_HIP(ctx, hipGetLastError());
-    if (with_rhs) FSIM_TRY(finalize(ctx, (int)nblocks(S.nc), FIN_BB));
+    if (with_rhs) FSIM_TRY(finalize(ctx, (int)red_blocks(S.nc), FIN_BB));
     return FSIM_OK;
 }
 
 static int nplane_threads(const SolveArgs& S) { return (S.nx - 2) * (S.ny - 2); }
 
 static int mic_factor(fsim_ctx* ctx, const SolveArgs& S) {
+    FSIM_PROF(ctx, KID_MIC_FACTOR);
     FSIM_TRY(fill_f64(ctx, ctx->P64, 0.0, S.nc));
     FSIM_TRY(fill_f32(ctx, ctx->P32, 0.0f, S.nc));
+    if (ctx->sweep_mode == 0) return wave_launch(ctx, 0, nullptr, nullptr, S.scale);
     PlaneArgs A{S, ctx->nbinfo, 0};
     const int nt = nplane_threads(S);
     const int lmax = (S.nx - 2) + (S.ny - 2) + (S.nz - 2);
@@ -430,12 +483,21 @@ static int mic_factor(fsim_ctx* ctx, const SolveArgs& S) {
         A.level = L;
         hipLaunchKernelGGL(k_mic_factor_plane, dim3(nblocks(nt)), dim3(kBlock), 0, ctx->stream, A, ctx->P64, ctx->P32);
     }
+    hipLaunchKernelGGL(k_pack_w, dim3(nblocks(S.nc)), dim3(kBlock), 0, ctx->stream, S.nc, ctx->nbinfo, ctx->P32, ctx->W);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
 
 // z = M^-1 r (q is scratch)
 static int mic_apply(fsim_ctx* ctx, const SolveArgs& S, const float* r, float* q, float* z) {
+    if (ctx->sweep_mode == 0) {
+        {
+            FSIM_PROF(ctx, KID_SWEEP_FWD);
+            FSIM_TRY(wave_launch(ctx, 1, r, q, S.scale));
+        }
+        FSIM_PROF(ctx, KID_SWEEP_BWD);
+        return wave_launch(ctx, 2, q, z, S.scale);
+    }
     PlaneArgs A{S, ctx->nbinfo, 0};
     const int nt = nplane_threads(S);
     const int lmax = (S.nx - 2) + (S.ny - 2) + (S.nz - 2);
@@ -452,10 +514,13 @@ static int mic_apply(fsim_ctx* ctx, const SolveArgs& S, const float* r, float* q
 }
 
 static int true_residual(fsim_ctx* ctx, const SolveArgs& S, bool replace) {
-    hipLaunchKernelGGL(k_true_residual, dim3(nblocks(S.nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->b, ctx->x,
+    {
+    FSIM_PROF(ctx, KID_TRUE_RES);
+    hipLaunchKernelGGL(k_true_residual, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->b, ctx->x,
                        ctx->r, replace ? 1 : 0, ctx->partials);
+    }
     FSIM_HIP(ctx, hipGetLastError());
-    return finalize(ctx, (int)nblocks(S.nc), FIN_TRUE);
+    return finalize(ctx, (int)red_blocks(S.nc), FIN_TRUE);
 }
 
 static bool stop_met(const fsim_ctx* ctx, double ss, double mx, double bnorm) {
@@ -467,6 +532,7 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     const SolveArgs S = solve_args(ctx, dt);
     const int64_t nc = S.nc;
     const unsigned nb = nblocks(nc);
+    const unsigned nr = red_blocks(nc);
     ctx->last_dt = dt;
     fsim_step_stats& st = ctx->stats;
     st.early_exit = 0;
@@ -480,6 +546,7 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     const double bb = ctx->scal_host->bb;
     const double bnorm = std::sqrt(bb);
     st.b_norm_l2 = bnorm;
+    st.num_fluid_cells = (int64_t)ctx->scal_host->nfluid;
     if (bb < 1e-7) {   // early exit without pressure (bridsonSolverGrid.cpp:254-258)
         st.early_exit = 1;
         FSIM_TRY(true_residual(ctx, S, false));
@@ -494,8 +561,11 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     FSIM_TRY(fill_f32(ctx, ctx->z, 0.0f, nc));
     FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
     hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->z, ctx->s);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->r, ctx->partials);
-    FSIM_TRY(finalize(ctx, nb, FIN_SIGMA0));
+    {
+        FSIM_PROF(ctx, KID_DOT);
+        hipLaunchKernelGGL(k_dot, dim3(nr), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->r, ctx->partials);
+    }
+    FSIM_TRY(finalize(ctx, nr, FIN_SIGMA0));
     const int max_it = ctx->prm.max_iterations;
     const int max_reliable = 64;
     // Reliable updates (restarted mixed-precision PCG): when the FP32
@@ -507,13 +577,20 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     double last_true = bnorm;
     int it = 0;
     for (; it < max_it; it++) {
-        hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->s, ctx->q, ctx->partials);
-        FSIM_TRY(finalize(ctx, nb, FIN_ALPHA));
+        {
+            FSIM_PROF(ctx, KID_SPMV);
+            hipLaunchKernelGGL(k_spmv_dot, dim3(nr), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->s, ctx->q,
+                               ctx->partials);
+        }
+        FSIM_TRY(finalize(ctx, nr, FIN_ALPHA));
         FSIM_TRY(read_scalars(ctx));
         if (ctx->scal_host->nan_flag & 1) break;
-        hipLaunchKernelGGL(k_update, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->s, ctx->q, ctx->x, ctx->r,
-                           ctx->scal, ctx->partials);
-        FSIM_TRY(finalize(ctx, nb, FIN_RES));
+        {
+            FSIM_PROF(ctx, KID_UPDATE);
+            hipLaunchKernelGGL(k_update, dim3(nr), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->s, ctx->q, ctx->x,
+                               ctx->r, ctx->scal, ctx->partials);
+        }
+        FSIM_TRY(finalize(ctx, nr, FIN_RES));
         FSIM_TRY(read_scalars(ctx));
         const double rr = ctx->scal_host->rr, rmax = ctx->scal_host->rmax;
         bool restart = false;
@@ -531,29 +608,68 @@ int stage_solve(fsim_ctx* ctx, double dt) {
             }
         }
         FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
-        hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->r, ctx->partials);
+        {
+            FSIM_PROF(ctx, KID_DOT);
+            hipLaunchKernelGGL(k_dot, dim3(nr), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->r, ctx->partials);
+        }
         if (restart) {
-            FSIM_TRY(finalize(ctx, nb, FIN_SIGMA0));
+            FSIM_TRY(finalize(ctx, nr, FIN_SIGMA0));
             hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->z, ctx->s);
             FSIM_HIP(ctx, hipGetLastError());
             continue;
         }
-        FSIM_TRY(finalize(ctx, nb, FIN_BETA));
+        FSIM_TRY(finalize(ctx, nr, FIN_BETA));
         FSIM_TRY(read_scalars(ctx));
         if (ctx->scal_host->nan_flag & 2) break;
-        hipLaunchKernelGGL(k_xpay, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->s, ctx->scal);
+        {
+            FSIM_PROF(ctx, KID_XPAY);
+            hipLaunchKernelGGL(k_xpay, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->s, ctx->scal);
+        }
         FSIM_HIP(ctx, hipGetLastError());
     }
     st.pcg_iterations = it;
     const double sc = dt / (ctx->prm.fluid_density * ctx->g.cd);
-    hipLaunchKernelGGL(k_apply_pressure, dim3(nb), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->label, ctx->x, ctx->u2,
-                       ctx->v2, ctx->w2, sc);
+    {
+        FSIM_PROF(ctx, KID_PRESSURE);
+        hipLaunchKernelGGL(k_apply_pressure, dim3(nb), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->label, ctx->x,
+                           ctx->u2, ctx->v2, ctx->w2, sc);
+    }
     FSIM_HIP(ctx, hipGetLastError());
     FSIM_TRY(true_residual(ctx, S, false));
     FSIM_TRY(read_scalars(ctx));
     st.residual_abs_inf = ctx->scal_host->tr_max;
     st.residual_rel_l2 = bnorm > 0 ? std::sqrt(ctx->scal_host->tr_ss) / bnorm : 0.0;
+    FSIM_TRY(check_wave_err(ctx));
     if (ctx->scal_host->nan_flag) return set_err(ctx, FSIM_ERR_NUMERIC, "NaN in the pressure solve");
+    return FSIM_OK;
+}
+
+int wave_init(fsim_ctx* ctx) {
+    WaveArgs& A = ctx->wave;
+    A.nx = ctx->g.nx;
+    A.ny = ctx->g.ny;
+    A.nz = ctx->g.nz;
+    A.TJ = (A.ny - 2 + WF_TY - 1) / WF_TY;
+    A.TK = (A.nz - 2 + WF_TZ - 1) / WF_TZ;
+    const int64_t ny_pk = 2 * (int64_t)A.TJ * A.nx * A.nz, nz_pk = 2 * (int64_t)A.TK * A.nx * A.ny;
+    if (hipMalloc(&ctx->W, sizeof(uint64_t) * ctx->g.nc) != hipSuccess ||
+        hipMalloc(&ctx->ybnd, sizeof(uint64_t) * ny_pk) != hipSuccess ||
+        hipMalloc(&ctx->zbnd, sizeof(uint64_t) * nz_pk) != hipSuccess ||
+        hipMalloc(&ctx->wave_ticket, sizeof(unsigned long long) * 3) != hipSuccess ||
+        hipMalloc(&ctx->wave_err, sizeof(int)) != hipSuccess)
+        return set_err(ctx, FSIM_ERR_OOM, "wavefront buffer allocation failed");
+    FSIM_HIP(ctx, hipMemsetAsync(ctx->W, 0, sizeof(uint64_t) * ctx->g.nc, ctx->stream));
+    FSIM_HIP(ctx, hipMemsetAsync(ctx->ybnd, 0, sizeof(uint64_t) * ny_pk, ctx->stream));
+    FSIM_HIP(ctx, hipMemsetAsync(ctx->zbnd, 0, sizeof(uint64_t) * nz_pk, ctx->stream));
+    FSIM_HIP(ctx, hipMemsetAsync(ctx->wave_ticket, 0, sizeof(unsigned long long) * 3, ctx->stream));
+    FSIM_HIP(ctx, hipMemsetAsync(ctx->wave_err, 0, sizeof(int), ctx->stream));
+    A.nbinfo = ctx->nbinfo;
+    A.P64 = ctx->P64;
+    A.P32 = ctx->P32;
+    A.W = ctx->W;
+    A.ybnd = ctx->ybnd;
+    A.zbnd = ctx->zbnd;
+    A.err = ctx->wave_err;
     return FSIM_OK;
 }
 
@@ -564,6 +680,7 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
     std::vector<float> hin(nc), hout(nc);
     if (op == FSIM_OP_FACTOR_MIC0) {
         FSIM_TRY(mic_factor(ctx, S));
+        FSIM_TRY(check_wave_err(ctx));
         FSIM_HIP(ctx, hipMemcpyAsync(out, ctx->P64, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
         FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         return FSIM_OK;
@@ -571,7 +688,7 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
     for (int64_t i = 0; i < nc; i++) hin[i] = (float)in[i];
     FSIM_HIP(ctx, hipMemcpyAsync(ctx->r, hin.data(), sizeof(float) * nc, hipMemcpyHostToDevice, ctx->stream));
     if (op == FSIM_OP_APPLY_A) {
-        hipLaunchKernelGGL(k_spmv_dot, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->r, ctx->q,
+        hipLaunchKernelGGL(k_spmv_dot, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->r, ctx->q,
                            ctx->partials);
         FSIM_HIP(ctx, hipGetLastError());
         FSIM_HIP(ctx, hipMemcpyAsync(hout.data(), ctx->q, sizeof(float) * nc, hipMemcpyDeviceToHost, ctx->stream));
@@ -580,6 +697,7 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
         FSIM_TRY(fill_f32(ctx, ctx->y, 0.0f, nc));
         FSIM_TRY(fill_f32(ctx, ctx->z, 0.0f, nc));
         FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
+        FSIM_TRY(check_wave_err(ctx));
         FSIM_HIP(ctx, hipMemcpyAsync(hout.data(), ctx->z, sizeof(float) * nc, hipMemcpyDeviceToHost, ctx->stream));
     } else {
         return set_err(ctx, FSIM_ERR_INVALID, "unknown solver op");

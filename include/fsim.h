@@ -222,6 +222,24 @@ int fsim_get_grid(fsim_ctx* ctx, int field, void* dst, size_t bytes);
 int fsim_set_grid(fsim_ctx* ctx, int field, const void* src, size_t bytes);
 int fsim_solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out);
 
+/* ---- implementation options (no reference counterpart) ------------------- */
+typedef enum fsim_option {
+    /* 0 (default): persistent wavefront kernels for the MIC(0) factor and
+     * sweeps; 1: one kernel launch per i+j+k plane (same arithmetic, slower;
+     * used to cross-check the wavefront kernels). */
+    FSIM_OPT_SWEEP_MODE = 0,
+    /* 1: time every kernel with HIP events (fsim_get_kernel_time) */
+    FSIM_OPT_PROFILE = 1
+} fsim_option;
+int fsim_set_option(fsim_ctx* ctx, int option, int value);
+
+/* Per-kernel GPU time (HIP events on the context's stream) accumulated while
+ * FSIM_OPT_PROFILE is on.  Kernel ids run 0 .. fsim_num_kernels()-1. */
+int fsim_num_kernels(void);
+const char* fsim_kernel_name(int kernel);
+int fsim_get_kernel_time(fsim_ctx* ctx, int kernel, double* total_ms, int64_t* launches);
+int fsim_reset_kernel_times(fsim_ctx* ctx);
+
 /* ---- synthetic scenes (host-only, no GPU needed) -------------------------- */
 /* Dam-break generator: a cell (not on the domain border, not inside any
  * sphere/box obstacle by the voxelisation rule of addObstacle) whose centre
