@@ -427,7 +427,23 @@ int fsim_get_grid(fsim_ctx* ctx, int field, void* dst, size_t bytes) {
     }
     if (bytes < need) return set_err(ctx, FSIM_ERR_INVALID, "destination too small");
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    FSIM_HIP(ctx, hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+    const bool rotated = field == FSIM_FIELD_PRESSURE || field == FSIM_FIELD_RHS || field == FSIM_FIELD_PRECON;
+    if (!rotated) {
+        FSIM_HIP(ctx, hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+        return FSIM_OK;
+    }
+    // solver vectors are stored in the rotated layout: gather into flat order
+    const size_t elem = need / nc;
+    std::vector<unsigned char> tmp(need);
+    FSIM_HIP(ctx, hipMemcpy(tmp.data(), src, need, hipMemcpyDeviceToHost));
+    unsigned char* out = static_cast<unsigned char*>(dst);
+    const GridDims& g = ctx->g;
+    for (int x = 0; x < g.nx; x++)
+        for (int y = 0; y < g.ny; y++)
+            for (int z = 0; z < g.nz; z++) {
+                const int64_t i = ((int64_t)x * g.ny + y) * g.nz + z;
+                std::memcpy(out + i * elem, tmp.data() + ridx(x, y, z, g.nx, g.ny, g.nz) * elem, elem);
+            }
     return FSIM_OK;
 }
 
@@ -480,6 +496,8 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
     switch (option) {
     case FSIM_OPT_SWEEP_MODE:
         if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "sweep mode is 0 or 1");
+        if (value == 0 && ctx->g.nc > kWaveMaxCells)
+            return set_err(ctx, FSIM_ERR_INVALID, "grid too large for the wavefront sweeps (32-bit offsets)");
         ctx->sweep_mode = value;
         return FSIM_OK;
     case FSIM_OPT_PROFILE:

@@ -2,14 +2,14 @@
 //
 // Device layout (one context, one GPU):
 //   * grid fields are dense arrays over all Nc = Nx*Ny*Nz cells in the
-//     reference's flat order idx = (x*Ny + y)*Nz + z (z fastest, so a
-//     wavefront's lanes run along z and their accesses coalesce);
+//     reference's flat order idx = (x*Ny + y)*Nz + z (z fastest);
 //   * face velocities are FP32 SoA: u,v,w (P2G result after boundary
 //     conditions, the reference's Face::v) and u2,v2,w2 (Face::v2);
 //   * labels are uint8 fsim_cell_type; nbinfo (uint16) caches, per WATER cell,
 //     which of the 6 neighbours are WATER and how many are not SOLID;
 //   * pressure-solve vectors are dense FP32 (b, r, z, s, q, y) with FP64
 //     pressure x and FP64 MIC(0) diagonal; entries outside WATER cells are 0;
+//     they, nbinfo and the packed W use the rotated layout of ridx() below;
 //   * particles are FP32 SoA (pos, vel, APIC C) plus the int32 particle id
 //     (its index in the reference's particle vector), kept sorted by cell
 //     after every BIN stage.
@@ -41,15 +41,41 @@ constexpr int NB_NONSOLID_SHIFT = 7;        // bits 7..9: number of non-SOLID ne
 // +((a+2)%3) neighbours are WATER (the MIC(0) factor's "other" terms)
 constexpr int NB_UP_SHIFT = 10;
 
-// persistent wavefront kernels: tile of WF_TY x WF_TZ (y, z) columns, one
-// compute wave per y row plus two communication waves
-constexpr int WF_TY = 14;
+// Persistent wavefront kernels (MIC(0) factor and triangular sweeps): a tile
+// of WF_TY x WF_TZ (y, z) columns per workgroup, WF_NW compute waves of WF_R
+// rows each plus two communication waves.
+constexpr int WF_R = 2;
+constexpr int WF_NW = 8;
+constexpr int WF_TY = WF_R * WF_NW;
 constexpr int WF_TZ = 64;
-constexpr int WF_THREADS = (WF_TY + 2) * WF_TZ;
+constexpr int WF_THREADS = (WF_NW + 2) * WF_TZ;
+// the wavefront kernels address cells with 32-bit byte offsets into arrays
+// of up to 8-byte words; larger grids use the plane-by-plane kernels
+constexpr int64_t kWaveMaxCells = ((int64_t(1) << 31) - 1) / 8;
+
+// Solver-vector layout ("rotated"): cell (x, y, z) of the pressure solve is
+// stored at plane (x + ((z - 1) mod 64)) mod nx of a dense nx*ny*nz array,
+// row y, column z.  The map is a bijection on every (y, z) column, and the
+// 64 cells of a z-tile [1 + 64k, 64 + 64k] that one wavefront step visits
+// (x = c - ((z - 1) mod 64)) share one plane, so a step reads and writes 64
+// contiguous words.  Stencil neighbours stay contiguous chunks too.
+__host__ __device__ inline int rot_of(int z) { return (z - 1) & (WF_TZ - 1); }
+__host__ __device__ inline int rot_plane(int x, int z, int nx) {
+    const int p = x + rot_of(z);
+    return p < nx ? p : p % nx;
+}
+__host__ __device__ inline int unrot_x(int p, int z, int nx) {
+    const int x = p - rot_of(z);
+    return x >= 0 ? x : (x % nx + nx) % nx;
+}
+__host__ __device__ inline int64_t ridx(int x, int y, int z, int nx, int ny, int nz) {
+    return ((int64_t)rot_plane(x, z, nx) * ny + y) * nz + z;
+}
 
 struct WaveArgs {
     int nx, ny, nz;
     int TJ, TK;                   // tiles along y and z
+    int SP;                       // boundary-packet rows (steps) per tile
     const uint16_t* nbinfo;
     const float* in;
     float* out;
@@ -58,8 +84,8 @@ struct WaveArgs {
     double* P64;
     float* P32;
     uint64_t* W;                  // packed (nbinfo << 32 | P32 bits) per cell
-    uint64_t* ybnd;               // boundary packets across y tile edges
-    uint64_t* zbnd;               // boundary packets across z tile edges
+    uint64_t* ybnd;               // packets across y tile edges: [tile J*TK+K][step][64][words]
+    uint64_t* zbnd;               // packets across z tile edges: [tile J*TK+K][step][WF_TY][words]
     unsigned long long* ticket;
     unsigned long long ticket_base;
     uint32_t epoch;

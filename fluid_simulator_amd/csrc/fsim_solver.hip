@@ -33,7 +33,24 @@ struct SolveArgs {
     double scale;
 };
 
+// flat index of the grid fields (labels, velocities, density)
 __device__ inline int64_t sidx(const SolveArgs& S, int x, int y, int z) { return ((int64_t)x * S.ny + y) * S.nz + z; }
+// index of the solver vectors, nbinfo and W (rotated layout)
+__device__ inline int64_t rsidx(const SolveArgs& S, int x, int y, int z) { return ridx(x, y, z, S.nx, S.ny, S.nz); }
+// cell (x, y, z) stored at rotated index i
+__device__ inline void rcell(const SolveArgs& S, int64_t i, int& x, int& y, int& z) {
+    z = (int)(i % S.nz);
+    const int64_t t = i / S.nz;
+    y = (int)(t % S.ny);
+    x = unrot_x((int)(t / S.ny), z, S.nx);
+}
+// rotated indices of the +a and -a neighbours of interior cell (x, y, z)
+__device__ inline int64_t rnb_plus(const SolveArgs& S, int x, int y, int z, int a) {
+    return rsidx(S, x + (a == 0), y + (a == 1), z + (a == 2));
+}
+__device__ inline int64_t rnb_minus(const SolveArgs& S, int x, int y, int z, int a) {
+    return rsidx(S, x - (a == 0), y - (a == 1), z - (a == 2));
+}
 
 // ---- block reductions (FP64, fixed order) ------------------------------------
 __device__ inline double wave_sum(double v) {
@@ -186,8 +203,9 @@ __global__ void __launch_bounds__(kBlock) k_nbinfo_rhs(RhsArgs A, int with_rhs) 
                 bv = (float)bd;
             }
         }
-        A.nbinfo[i] = info;
-        if (with_rhs) A.b[i] = bv;
+        const int64_t ri = rsidx(S, x, y, z);
+        A.nbinfo[ri] = info;
+        if (with_rhs) A.b[ri] = bv;
     }
     if (with_rhs) block_reduce_store(bsq, nfl, A.partials, AUX_SUM);
 }
@@ -214,15 +232,14 @@ __global__ void __launch_bounds__(kBlock) k_mic_factor_plane(PlaneArgs A, double
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int x, y, z;
     if (!plane_cell(A, t, x, y, z)) return;
-    int64_t i = sidx(S, x, y, z);
+    int64_t i = rsidx(S, x, y, z);
     uint16_t info = A.nbinfo[i];
     if (!(info & NB_FLUID)) return;
     const double tau = 0.97, sigma = 0.25, scale = S.scale;
-    const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
     double eneg = 0, eneg_tau = 0;
     for (int a = 0; a < 3; a++) {
         if (!(info & (NB_MX << a))) continue;
-        int64_t j = i - st[a];
+        int64_t j = rnb_minus(S, x, y, z, a);
         double pj = P64[j];
         double ap = -scale * pj;   // xWater of the -neighbour times its precon
         double o1 = (info & (1u << (NB_UP_SHIFT + 2 * a))) ? -scale : 0.0;
@@ -257,15 +274,17 @@ __global__ void __launch_bounds__(kBlock) k_fwd_plane(PlaneArgs A, const float* 
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int x, y, z;
     if (!plane_cell(A, t, x, y, z)) return;
-    int64_t i = sidx(S, x, y, z);
+    int64_t i = rsidx(S, x, y, z);
     uint16_t info = A.nbinfo[i];
     if (!(info & NB_FLUID)) return;
-    const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
     const float ms = -S.scale_f;
     float qneg = 0.f;
 #pragma unroll
     for (int a = 0; a < 3; a++)
-        if (info & (NB_MX << a)) qneg += ms * (q[i - st[a]] * P[i - st[a]]);
+        if (info & (NB_MX << a)) {
+            const int64_t j = rnb_minus(S, x, y, z, a);
+            qneg += ms * (q[j] * P[j]);
+        }
     q[i] = (r[i] - qneg) * P[i];
 }
 
@@ -276,15 +295,14 @@ __global__ void __launch_bounds__(kBlock) k_bwd_plane(PlaneArgs A, const float* 
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int x, y, zc;
     if (!plane_cell(A, t, x, y, zc)) return;
-    int64_t i = sidx(S, x, y, zc);
+    int64_t i = rsidx(S, x, y, zc);
     uint16_t info = A.nbinfo[i];
     if (!(info & NB_FLUID)) return;
-    const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
     const float ms = -S.scale_f;
     float tneg = 0.f;
 #pragma unroll
     for (int a = 0; a < 3; a++)
-        if (info & (NB_PX << a)) tneg += ms * z[i + st[a]];
+        if (info & (NB_PX << a)) tneg += ms * z[rnb_plus(S, x, y, zc, a)];
     z[i] = (q[i] - tneg * P[i]) * P[i];
 }
 
@@ -298,16 +316,17 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(SolveArgs S, const uint16_t
         uint16_t info = nbinfo[i];
         float val = 0.f;
         if (info & NB_FLUID) {
-            const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
+            int x, y, z;
+            rcell(S, i, x, y, z);
             const float ms = -S.scale_f;
             int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
             val = (S.scale_f * (float)cnt) * s[i];
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                if (info & (NB_PX << a)) val += ms * s[i + st[a]];
+                if (info & (NB_PX << a)) val += ms * s[rnb_plus(S, x, y, z, a)];
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                if (info & (NB_MX << a)) val += ms * s[i - st[a]];
+                if (info & (NB_MX << a)) val += ms * s[rnb_minus(S, x, y, z, a)];
             d += (double)s[i] * (double)val;
         }
         q[i] = val;
@@ -360,15 +379,16 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(SolveArgs S, const uin
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.nc; i += stride) {
         uint16_t info = nbinfo[i];
         if (info & NB_FLUID) {
-            const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
+            int cx, cy, cz;
+            rcell(S, i, cx, cy, cz);
             int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
             double diag = 0;
             for (int k = 0; k < cnt; k++) diag += S.scale;
             double val = diag * x[i];
             for (int a = 0; a < 3; a++)
-                if (info & (NB_PX << a)) val += -S.scale * x[i + st[a]];
+                if (info & (NB_PX << a)) val += -S.scale * x[rnb_plus(S, cx, cy, cz, a)];
             for (int a = 0; a < 3; a++)
-                if (info & (NB_MX << a)) val += -S.scale * x[i - st[a]];
+                if (info & (NB_MX << a)) val += -S.scale * x[rnb_minus(S, cx, cy, cz, a)];
             double t = (double)b[i] - val;
             ss += t * t;
             mx = fmax(mx, fabs(t));
@@ -386,15 +406,17 @@ __global__ void __launch_bounds__(kBlock) k_apply_pressure(SolveArgs S, const ui
 #pragma clang fp contract(off)
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S.nc) return;
-    if (!(nbinfo[i] & NB_FLUID)) return;
+    const int cz = (int)(i % S.nz), cy = (int)((i / S.nz) % S.ny), cx = (int)(i / ((int64_t)S.nz * S.ny));
+    const int64_t ri = rsidx(S, cx, cy, cz);
+    if (!(nbinfo[ri] & NB_FLUID)) return;
     const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
     float* f[3] = {u2, v2, w2};
-    const double pi = x[i];
+    const double pi = x[ri];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         uint8_t lp = label[i + st[a]];
         if (lp == FSIM_AIR) f[a][i] = (float)((double)f[a][i] + sc * pi);
-        else if (lp == FSIM_WATER) f[a][i] = (float)((double)f[a][i] + sc * (pi - x[i + st[a]]));
+        else if (lp == FSIM_WATER) f[a][i] = (float)((double)f[a][i] + sc * (pi - x[rnb_plus(S, cx, cy, cz, a)]));
         if (label[i - st[a]] == FSIM_AIR) f[a][i - st[a]] = (float)((double)f[a][i - st[a]] - sc * pi);
     }
 }
@@ -651,7 +673,13 @@ int wave_init(fsim_ctx* ctx) {
     A.nz = ctx->g.nz;
     A.TJ = (A.ny - 2 + WF_TY - 1) / WF_TY;
     A.TK = (A.nz - 2 + WF_TZ - 1) / WF_TZ;
-    const int64_t ny_pk = 2 * (int64_t)A.TJ * A.nx * A.nz, nz_pk = 2 * (int64_t)A.TK * A.nx * A.ny;
+    // packet rows: one per step of a tile (the step count rounded up to the
+    // kernels' 8-step blocks, plus slack); two words per value (the factor's
+    // FP64), the sweeps use one
+    A.SP = (A.nx - 2 + WF_TY + WF_TZ - 2 + 7) / 8 * 8 + 8;
+    const int64_t ny_pk = 2 * (int64_t)A.TJ * A.TK * A.SP * WF_TZ;   // per tile (J, K)
+    const int64_t nz_pk = 2 * (int64_t)A.TJ * A.TK * A.SP * WF_TY;
+    if (ctx->g.nc > kWaveMaxCells) ctx->sweep_mode = 1;
     if (hipMalloc(&ctx->W, sizeof(uint64_t) * ctx->g.nc) != hipSuccess ||
         hipMalloc(&ctx->ybnd, sizeof(uint64_t) * ny_pk) != hipSuccess ||
         hipMalloc(&ctx->zbnd, sizeof(uint64_t) * nz_pk) != hipSuccess ||
@@ -677,15 +705,22 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
     const SolveArgs S = solve_args(ctx, dt);
     const int64_t nc = S.nc;
     FSIM_TRY(build_nbinfo(ctx, S, false));
+    // host vectors are in flat order, device solver vectors in the rotated one
+    std::vector<int64_t> rmap(nc);
+    for (int x = 0; x < S.nx; x++)
+        for (int y = 0; y < S.ny; y++)
+            for (int z = 0; z < S.nz; z++) rmap[((int64_t)x * S.ny + y) * S.nz + z] = ridx(x, y, z, S.nx, S.ny, S.nz);
     std::vector<float> hin(nc), hout(nc);
     if (op == FSIM_OP_FACTOR_MIC0) {
         FSIM_TRY(mic_factor(ctx, S));
         FSIM_TRY(check_wave_err(ctx));
-        FSIM_HIP(ctx, hipMemcpyAsync(out, ctx->P64, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<double> hp(nc);
+        FSIM_HIP(ctx, hipMemcpyAsync(hp.data(), ctx->P64, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
         FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int64_t i = 0; i < nc; i++) out[i] = hp[rmap[i]];
         return FSIM_OK;
     }
-    for (int64_t i = 0; i < nc; i++) hin[i] = (float)in[i];
+    for (int64_t i = 0; i < nc; i++) hin[rmap[i]] = (float)in[i];
     FSIM_HIP(ctx, hipMemcpyAsync(ctx->r, hin.data(), sizeof(float) * nc, hipMemcpyHostToDevice, ctx->stream));
     if (op == FSIM_OP_APPLY_A) {
         hipLaunchKernelGGL(k_spmv_dot, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->r, ctx->q,
@@ -703,7 +738,7 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
         return set_err(ctx, FSIM_ERR_INVALID, "unknown solver op");
     }
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int64_t i = 0; i < nc; i++) out[i] = hout[i];
+    for (int64_t i = 0; i < nc; i++) out[i] = hout[rmap[i]];
     return FSIM_OK;
 }
 

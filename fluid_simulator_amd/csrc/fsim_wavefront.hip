@@ -3,31 +3,45 @@
 //
 // Each cell depends only on its three upstream neighbours (-x,-y,-z for the
 // factor and the forward sweep, +x,+y,+z for the backward sweep), so the
-// serial lexicographic loop of the reference can run on i+j+k planes.
+// serial lexicographic loop of the reference can run on i+j+k planes; every
+// cell performs the same operations in the same order as the serial loop, so
+// the result is bit-identical to it whatever the schedule.
 //
-// Decomposition.  One workgroup owns a tile of WF_TY x WF_TZ interior (y, z)
-// columns and marches along x.  Lane (ty, tz) of the tile processes
+// Decomposition.  One workgroup owns a tile of WF_TY x 64 interior (y, z)
+// columns and marches along x.  Lane tz of tile row ty processes
 // x = t - ty - tz at step t, so its three upstream values were produced at
 // step t-1: the -x value by the lane itself, the -z value by lane tz-1 of the
-// same wave (DPP wave shift) and the -y value by wave ty-1 (LDS).
+// same row (DPP wave shift) and the -y value by row ty-1: a register when the
+// two rows belong to the same wave (WF_R rows per wave), otherwise the LDS
+// ring below.  The backward sweep mirrors y, and z inside each 64-wide
+// z-tile, so that its upstream is also ty-1, tz-1, x-1 in local terms.
 //
-// Inside a workgroup the WF_TY compute waves form a pipeline along y with
-// point-to-point synchronisation: wave ty publishes its row of step t into an
-// LDS ring and then its progress counter; wave ty+1 waits only for that
-// counter, and wave ty waits for wave ty+1's counter before reusing a ring
-// slot.  There is no workgroup barrier in the main loop.
+// Memory.  The solver vectors use the rotated layout (ridx): the 64 cells a
+// row visits at one step share one rotated plane, so every load and store
+// of a step covers 64 contiguous words.  Each row keeps its plane offset in
+// a scalar register (advanced and wrapped per step) and each lane a constant
+// in-plane offset, so addressing costs no vector instructions; lanes outside
+// the grid carry an out-of-range buffer offset (loads read 0, stores drop).
+//
+// Inside a workgroup the compute waves form a pipeline along y.  The last
+// row of each wave is written into an LDS ring as 64-bit words
+// (step << 32 | value bits); the next wave reads the word it needs and
+// accepts it when the step tag matches, so a step costs one LDS round trip
+// and needs no counters, fences or drains (an aligned 64-bit LDS access is
+// never seen half-written).  Ring reuse is checked every kBpEvery steps
+// against the downstream wave's own tags.
 //
 // Across workgroups, values at the tile edges travel through boundary
 // buffers in HBM as 64-bit packets (launch epoch << 32 | value bits) written
-// with device-scope stores; a reader accepts a packet when its epoch tag
-// matches, so no flags or fences are needed and a packet is never seen
-// half-written.  Two communication waves per workgroup (one for the -y edge,
-// one for the -z edge) read the upstream tile's packets in batches and
-// republish them into LDS rings tagged with the step that needs them.  This
-// keeps every global read whose result is needed "now" out of the compute
-// waves: on gfx9 the memory counter completes in order, so such a read would
-// drain the compute waves' prefetch stream (per-cell inputs are loaded
-// kPrefetch steps ahead into registers).
+// with device-scope stores, one packet row per step of the consuming tile
+// (ybnd[tile][s][64], zbnd[tile][s][WF_TY] of the producing tile), so a
+// step's packets are contiguous.
+// Two communication waves per workgroup read the upstream tile's packets in
+// batches of steps and republish them into LDS rings tagged with the step.
+// This keeps every global read whose result is needed "now" out of the
+// compute waves: on gfx9 the memory counter completes in order, so such a
+// read would drain the compute waves' prefetch stream (per-cell inputs are
+// loaded kPrefetch steps ahead, each prefetch slot in a fixed register).
 //
 // Every wait is bounded: past kSpinLimit polls, or as soon as any wave in the
 // grid has given up, a wave raises the error flag and carries on, so the
@@ -36,28 +50,36 @@
 // workgroups already running.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fsim_internal.hpp"
 
 namespace fsim {
 
 constexpr int kSpinLimit = 1 << 21;
+constexpr int kBusyPolls = 32;   // polls before a waiting wave starts to sleep
 constexpr int kPrefetch = 8;     // compute-wave register prefetch (steps)
-constexpr int kRingV = 8;        // LDS ring of exchanged values between compute waves (steps)
-constexpr int kRingP = 64;       // LDS ring of boundary packets (steps)
+constexpr int kRingV = 8;        // LDS ring of row values between compute waves (steps)
+constexpr int kBpEvery = 4;      // ring-reuse check period (steps); allowed lag kRingV - kBpEvery
+constexpr uint32_t kOob = 0x80000000u;        // buffer offset beyond every array: reads 0, drops stores
+constexpr int kRawBufferFlags = 0x00020000;   // buffer resource word 3 (raw 32-bit access, gfx9)
+constexpr int kDeviceScope = 16;              // buffer cache-policy bit sc1: device-scope coherence
 
 template <int KIND>
-struct KindTraits {   // 1 = forward sweep, 2 = backward sweep
+struct Kind {   // 1 = forward sweep, 2 = backward sweep (FP32)
     using T = float;
     static constexpr bool FWD = KIND == 1;
-    static constexpr int NP = 1;      // packets per boundary value
-    static constexpr int BATCH = 32;  // comm-wave batch (steps)
+    static constexpr int NP = 1;       // 32-bit words per value
+    static constexpr int BATCH = 32;   // comm-wave batch (steps)
+    static constexpr int RINGP = 64;   // LDS ring of boundary packets (steps)
 };
 template <>
-struct KindTraits<0> {   // MIC(0) factor
+struct Kind<0> {   // MIC(0) factor (FP64)
     using T = double;
     static constexpr bool FWD = true;
     static constexpr int NP = 2;
     static constexpr int BATCH = 16;
+    static constexpr int RINGP = 32;
 };
 
 // f(integral_constant<int, 0>), ..., f(integral_constant<int, N-1>) in order
@@ -70,7 +92,7 @@ __device__ __forceinline__ void unroll(F&& f) {
 }
 
 __device__ inline uint64_t pack(uint32_t bits, uint32_t tag) { return ((uint64_t)tag << 32) | bits; }
-__device__ inline void gput(uint64_t* p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline int tag_of(uint64_t v) { return (int)(uint32_t)(v >> 32); }
 __device__ inline uint64_t gpeek(const uint64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ inline void lput(uint64_t* p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline uint64_t lpeek(const uint64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
@@ -78,40 +100,25 @@ __device__ inline void lput32(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC
 __device__ inline int lpeek32(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline bool gave_up(const int* err) { return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0; }
 
-// Wait for all of this wave's LDS operations (vector memory counter untouched).
-__device__ inline void lds_drain() {
-    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0), vmcnt/expcnt at their maxima
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const void* p, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, kRawBufferFlags);
 }
-
-// Spin until the LDS counter *p reaches v (bounded).
-__device__ inline void wait_counter(const int* p, int v, int* err) {
-    int spins = 0;
-    while (lpeek32(p) < v) {
-        if (++spins > kSpinLimit) {
-            atomicOr(err, 1);
-            break;
-        }
-        if ((spins & 1023) == 0 && gave_up(err)) break;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+__device__ inline uint64_t bload64(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
-
-// Spin until the LDS packet slot carries step tag t; returns its payload.
-__device__ inline uint32_t wait_slot(const uint64_t* p, int t, int* err) {
-    uint64_t v = lpeek(p);
-    int spins = 0;
-    while ((int)(uint32_t)(v >> 32) != t) {
-        if (++spins > kSpinLimit) {
-            atomicOr(err, 1);
-            break;
-        }
-        if ((spins & 1023) == 0 && gave_up(err)) break;
-        __builtin_amdgcn_s_sleep(1);
-        v = lpeek(p);
-    }
-    return (uint32_t)v;
+__device__ inline float bload32f(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ inline uint32_t bload16(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+}
+template <int AUX = 0>
+__device__ inline void bstore64(uint64_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, AUX);
+}
+__device__ inline void bstore32(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
 }
 
 // lane i receives lane i-1's value, lane 0 receives 0 (DPP wave_shr:1)
@@ -126,8 +133,10 @@ __device__ inline double shr1(double v) {
 
 __device__ inline uint32_t word(float v, int) { return __float_as_uint(v); }
 __device__ inline uint32_t word(double v, int k) { return k ? (uint32_t)__double2hiint(v) : (uint32_t)__double2loint(v); }
-__device__ inline void from_words(float& v, const uint32_t* w) { v = __uint_as_float(w[0]); }
-__device__ inline void from_words(double& v, const uint32_t* w) { v = __hiloint2double((int)w[1], (int)w[0]); }
+__device__ inline void from_words(float& v, const uint64_t* w) { v = __uint_as_float((uint32_t)w[0]); }
+__device__ inline void from_words(double& v, const uint64_t* w) {
+    v = __hiloint2double((int)(uint32_t)w[1], (int)(uint32_t)w[0]);
+}
 
 // Ticket in [0, TJ*TK), or -1 (with *err raised) if the counter is out of
 // step with the launch (only after a failed launch); the caller then exits.
@@ -145,91 +154,79 @@ __device__ inline int take_tile(const WaveArgs& A) {
     return s_tile;
 }
 
-// Geometry of one tile position (local coordinates mirrored for the backward
-// direction so that "upstream" is always ty-1, tz-1, xl-1).
-struct Pos {
-    int y, z;
-    bool col;
-};
+// Grid y and z of local row ty, lane tz of tile (J, K).
 template <bool FWD>
-__device__ inline Pos pos_of(const WaveArgs& A, int J, int K, int ty, int tz) {
-    Pos P;
-    const int ly = J * WF_TY + ty, lz = K * WF_TZ + tz;
-    P.col = ly < A.ny - 2 && lz < A.nz - 2;
-    P.y = FWD ? 1 + ly : A.ny - 2 - ly;
-    P.z = FWD ? 1 + lz : A.nz - 2 - lz;
-    return P;
+__device__ inline int y_of(const WaveArgs& A, int J, int ty) { return FWD ? 1 + J * WF_TY + ty : A.ny - 2 - (J * WF_TY + ty); }
+template <bool FWD>
+__device__ inline int z_of(const WaveArgs& A, int K, int tz) {
+    return FWD ? 1 + K * WF_TZ + tz : 1 + (A.TK - 1 - K) * WF_TZ + (WF_TZ - 1 - tz);
 }
-template <bool FWD>
-__device__ inline int x_of(const WaveArgs& A, int xl) { return FWD ? 1 + xl : A.nx - 2 - xl; }
 
 template <int KIND>
 struct WaveLds {
-    using T = typename KindTraits<KIND>::T;
-    static constexpr int NP = KindTraits<KIND>::NP;
-    T v[kRingV][WF_TY][WF_TZ];
-    uint64_t yp[kRingP][NP][WF_TZ];
-    uint64_t zp[kRingP][NP][WF_TY];
-    int prog[WF_TY];
+    static constexpr int NP = Kind<KIND>::NP, RINGP = Kind<KIND>::RINGP;
+    uint64_t v[kRingV][WF_NW][NP][WF_TZ];   // tagged last-row values of each compute wave
+    uint64_t yp[RINGP][NP][WF_TZ];          // tagged -y packets for row 0
+    uint64_t zp[RINGP][NP][WF_TY];          // tagged -z packets for lane 0 of each row
+    int prog[WF_NW];                        // a step whose packets the compute wave has consumed
 };
 
 // ---- communication wave: upstream tile edge -> LDS packet ring ------------
-// Lane l serves one consumer lane: for the -y edge the consumer is (ty 0,
-// tz l) of this tile, for the -z edge it is (ty l, tz 0).  The consumer at
-// step s needs the upstream value at x-index xl = s - l.
+// The -y edge carries 64 values per step (row 0, lane l), the -z edge WF_TY
+// (lane 0 of row l).  The upstream tile writes every packet of a step at
+// once, so the wave forwards whole steps: step s once all its packets carry
+// this launch's epoch and the ring slot s mod RINGP is free.
 template <int KIND, bool YEDGE>
 __device__ void comm_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
-    using TR = KindTraits<KIND>;
-    constexpr bool FWD = TR::FWD;
-    constexpr int NP = TR::NP, B = TR::BATCH;
+    using KD = Kind<KIND>;
+    constexpr int NP = KD::NP, B = KD::BATCH, RINGP = KD::RINGP;
+    constexpr int NL = YEDGE ? WF_TZ : WF_TY;    // values per step
+    if (YEDGE ? J == 0 : K == 0) return;
     const int l = threadIdx.x % 64;
+    const bool on = l < NL;
+    const int li = on ? l : 0;
     const int nxi = A.nx - 2;
-    const bool has_up = YEDGE ? J > 0 : K > 0;
-    const int cty = YEDGE ? 0 : l, ctz = YEDGE ? l : 0;
-    const bool lane_ok = has_up && (YEDGE || l < WF_TY) && pos_of<FWD>(A, J, K, cty, ctz).col;
-    const Pos P = pos_of<FWD>(A, J, K, min(cty, WF_TY - 1), ctz);
-    const uint64_t* base = YEDGE ? A.ybnd + (int64_t)NP * ((int64_t)(J - 1) * A.nx) * A.nz
-                                 : A.zbnd + (int64_t)NP * ((int64_t)(K - 1) * A.nx) * A.ny;
-    const int64_t stride = YEDGE ? A.nz : A.ny;          // per x
-    const int64_t off = YEDGE ? P.z : P.y;
-    const int* cprog = &S.prog[YEDGE ? 0 : min(l, WF_TY - 1)];
-    int s = lane_ok ? l : 0;                              // next step to publish
-    const int end = lane_ok ? l + nxi : 0;
-    int idle = 0;
-    while (__any(s < end)) {
+    const int end = NL - 1 + nxi;                 // steps 0 .. end-1 are consumed
+    // packets written by the upstream tile (J-1, K) or (J, K-1)
+    const int64_t up = YEDGE ? (int64_t)(J - 1) * A.TK + K : (int64_t)J * A.TK + K - 1;
+    const uint64_t* base = YEDGE ? A.ybnd + up * A.SP * WF_TZ * NP : A.zbnd + up * A.SP * WF_TY * NP;
+    // the consumer of lane l: wave 0 for the -y edge, the wave holding row l for the -z edge
+    const int* cprog = &S.prog[YEDGE ? 0 : li / WF_R];
+    int s0 = 0, idle = 0;
+    while (s0 < end) {
         uint64_t buf[B][NP];
 #pragma unroll
         for (int b = 0; b < B; b++) {
-            const int xl = min(max(s + b - l, 0), nxi - 1);
-            const uint64_t* p = lane_ok ? base + NP * ((int64_t)x_of<FWD>(A, xl) * stride + off) : A.ybnd;
+            const int sb = min(s0 + b, end - 1);
+            const uint64_t* p = base + ((int64_t)sb * NL + li) * NP;
 #pragma unroll
-            for (int k = 0; k < NP; k++) buf[b][k] = gpeek(p + (lane_ok ? k : 0));
+            for (int k = 0; k < NP; k++) buf[b][k] = gpeek(p + k);
         }
-        const int limit = lpeek32(cprog) + kRingP;        // slots free up to this step
+        const int limit = lpeek32(cprog) + RINGP;   // slot of step sb is free if sb - RINGP was consumed
         int published = 0;
         bool stop = false;
 #pragma unroll
         for (int b = 0; b < B; b++) {
-            const int sb = s + b;
-            bool ok = !stop && sb < end && sb <= limit;
+            const int sb = s0 + b;
+            bool ok = !on || sb <= limit;
 #pragma unroll
-            for (int k = 0; k < NP; k++) ok = ok && (uint32_t)(buf[b][k] >> 32) == A.epoch;
-            if (ok) {
-                uint64_t* slot = YEDGE ? &S.yp[sb % kRingP][0][l] : &S.zp[sb % kRingP][0][min(l, WF_TY - 1)];
+            for (int k = 0; k < NP; k++) ok = ok && (!on || (uint32_t)(buf[b][k] >> 32) == A.epoch);
+            stop = stop || sb >= end || !__all(ok);
+            if (!stop) {
+                if (on) {
+                    uint64_t* slot = YEDGE ? &S.yp[sb % RINGP][0][li] : &S.zp[sb % RINGP][0][li];
 #pragma unroll
-                for (int k = 0; k < NP; k++)
-                    lput(slot + k * (YEDGE ? WF_TZ : WF_TY), pack((uint32_t)buf[b][k], (uint32_t)sb));
+                    for (int k = 0; k < NP; k++) lput(slot + k * NL, pack((uint32_t)buf[b][k], (uint32_t)sb));
+                }
                 published++;
-            } else {
-                stop = true;
             }
         }
-        s += published;
-        if (__any(published > 0)) {
+        s0 += published;
+        if (published > 0) {
             idle = 0;
         } else {
             if (++idle > kSpinLimit / 64) {
-                atomicOr(A.err, 1);
+                if (l == 0) atomicOr(A.err, 1);
                 break;
             }
             if ((idle & 15) == 0 && gave_up(A.err)) break;
@@ -238,94 +235,198 @@ __device__ void comm_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
     }
 }
 
-// ---- compute waves -----------------------------------------------------------
+// ---- compute waves -------------------------------------------------------------
+// Factor (KIND 0, FP64):   e = diag - sum_a (A_a p_a)^2 - tau sum_a A_a (o1 + o2) p_a^2,
+//                          guarded by sigma, p = 1/sqrt(e)  (calculatePreconditioner)
+// forward sweep (KIND 1):  q = (r - sum_{-a fluid} A_a q_a p_a) * p, wave value q*p
+// backward sweep (KIND 2): z = (q - p * sum_{+a fluid} A_a z_a) * p, wave value z
+// with A_a = -scale, products and sums in the reference's order (x, y, z).
+// In the sweeps a neighbour that is not WATER carries exactly +0, whose term
+// -scale * (+0) = -0 leaves any partial sum unchanged, so they sum all three.
 template <int KIND>
 __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
 #pragma clang fp contract(off)
-    using TR = KindTraits<KIND>;
-    using T = typename TR::T;
-    constexpr bool FWD = TR::FWD;
-    constexpr int NP = TR::NP;
-    const int ty = threadIdx.x / 64, tz = threadIdx.x % 64;
-    const Pos P = pos_of<FWD>(A, J, K, ty, tz);
-    const int nxi = A.nx - 2;
-    const int nsteps = nxi + WF_TY + WF_TZ - 2;
-    const bool cy = ty == 0 && J > 0, cz = tz == 0 && K > 0;
-    uint64_t* ybout = A.ybnd + (int64_t)NP * ((int64_t)J * A.nx) * A.nz;
-    uint64_t* zbout = A.zbnd + (int64_t)NP * ((int64_t)K * A.nx) * A.ny;
-    const uint32_t mx = FWD ? NB_MX : NB_PX, my = FWD ? NB_MY : NB_PY, mz = FWD ? NB_MZ : NB_PZ;
+    using KD = Kind<KIND>;
+    using T = typename KD::T;
+    constexpr bool FWD = KD::FWD;
+    constexpr int NP = KD::NP, RINGP = KD::RINGP, R = WF_R, NW = WF_NW, TY = WF_TY;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int tz = threadIdx.x % 64;
+    const int nx = A.nx, nxi = A.nx - 2;
+    // whole blocks of kPrefetch steps (the extra steps are inactive everywhere),
+    // so no path through the loop skips a prefetch
+    const int nsteps = (nxi + TY + WF_TZ - 2 + kPrefetch - 1) / kPrefetch * kPrefetch;
+    const int plane = A.ny * A.nz;
+    const int64_t nc = (int64_t)nx * plane;
+    const uint32_t epoch = A.epoch;
+    // buffer resources: per-cell arrays and this tile's outgoing packet rows
+    const __amdgpu_buffer_rsrc_t rIn0 = KIND == 0 ? rsrc(A.nbinfo, nc * 2) : rsrc(A.W, nc * 8);
+    const __amdgpu_buffer_rsrc_t rIn1 = rsrc(A.in, nc * 4);
+    const __amdgpu_buffer_rsrc_t rOut0 = KIND == 0 ? rsrc(A.W, nc * 8) : rsrc(A.out, nc * 4);
+    const __amdgpu_buffer_rsrc_t rP64 = rsrc(A.P64, nc * 8), rP32 = rsrc(A.P32, nc * 4);
+    const int64_t tile = (int64_t)J * A.TK + K;
+    const __amdgpu_buffer_rsrc_t ryb = rsrc(A.ybnd + tile * A.SP * WF_TZ * NP, (int64_t)A.SP * WF_TZ * NP * 8);
+    const __amdgpu_buffer_rsrc_t rzb = rsrc(A.zbnd + tile * A.SP * TY * NP, (int64_t)A.SP * TY * NP * 8);
     const float ms = A.ms;
     const double scale = A.scale;
-    // per-cell inputs: packed (nbinfo << 32 | P32) words for the sweeps (plus
-    // the input vector), nbinfo for the factor; loads always issued
-    // (clamped to a valid cell) so the compiler can count them statically
-    uint64_t wring[kPrefetch];
-    float iring[kPrefetch];
-    auto fetch = [&](int xl, uint64_t& w, float& in) {
-        const int xc = min(max(xl, 0), nxi - 1);
-        const int64_t i = ((int64_t)x_of<FWD>(A, xc) * A.ny + (P.col ? P.y : 1)) * A.nz + (P.col ? P.z : 1);
-        if (KIND == 0) {
-            w = A.nbinfo[i];
-            in = 0.f;
+
+    // per row: in-plane lane offset (elements; kOob marks lanes outside the
+    // grid), rotated plane of the current step and of the prefetch step
+    uint32_t lo[R];
+    int pc[R], pf[R];
+    bool colr[R], zin[R];
+    const int z = z_of<FWD>(A, K, tz);
+    unroll<R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        const int ty = w * R + r;
+        const int y = y_of<FWD>(A, J, ty);
+        colr[r] = J * TY + ty < A.ny - 2 && z <= A.nz - 2;
+        lo[r] = colr[r] ? (uint32_t)(y * A.nz + z) : kOob;
+        // FWD: x + rot(z) = 1 + t - ty;  BWD: x + rot(z) = nx + 61 + ty - t
+        const int p0 = FWD ? ((1 - ty) % nx + nx) % nx : (61 + ty) % nx;
+        pc[r] = p0;
+        pf[r] = p0;
+        // lane 0 of this row consumes -z packets when K > 0 and the row is in the grid
+        zin[r] = K > 0 && J * TY + ty < A.ny - 2;
+    });
+    auto step_plane = [&](int& p, int n) {   // advance a rotated plane by n steps
+        p = FWD ? p + n : p - n;
+        if (p >= nx) p -= nx;
+        if (p < 0) p += nx;
+    };
+    // byte offsets for 8-, 4- and 2-byte elements
+    auto off8 = [](uint32_t e) { return e == kOob ? kOob : e * 8u; };
+    auto off4 = [](uint32_t e) { return e == kOob ? kOob : e * 4u; };
+    auto off2 = [](uint32_t e) { return e == kOob ? kOob : e * 2u; };
+    uint32_t lo8[R], lo4[R], lo2[R];
+    unroll<R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        lo8[r] = off8(lo[r]);
+        lo4[r] = off4(lo[r]);
+        lo2[r] = off2(lo[r]);
+    });
+    // prefetch ring: per-cell inputs of the next kPrefetch steps
+    uint64_t wring[kPrefetch][R];
+    float iring[kPrefetch][R];
+    auto fetch = [&](int d, int r) {
+        const uint32_t pe = (uint32_t)pf[r] * (uint32_t)plane;
+        if constexpr (KIND == 0) {
+            wring[d][r] = bload16(rIn0, lo2[r], pe * 2u);
+            iring[d][r] = 0.f;
         } else {
-            w = A.W[i];
-            in = A.in[i];
+            wring[d][r] = bload64(rIn0, lo8[r], pe * 8u);
+            iring[d][r] = bload32f(rIn1, lo4[r], pe * 4u);
         }
     };
-#pragma unroll
-    for (int d = 0; d < kPrefetch; d++) fetch(d - ty - tz, wring[d], iring[d]);
-    T vprev = 0;
+    // issue order pinned, and each prologue step issues as many memory
+    // operations as the loop steps do at least (2R stores, here dropped as
+    // out of range, then the loads): the wait for slot d then allows the same
+    // number of younger operations on entry as at every later step
+    unroll<kPrefetch>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        unroll<2 * R>([&](auto) { bstore32(0u, rOut0, kOob, 0); });
+        unroll<R>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            fetch(d, r);
+            step_plane(pf[r], 1);
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    constexpr int kSlotV = NW * NP * WF_TZ;   // words per value-ring slot
+    constexpr int kWordV = WF_TZ;             // distance between the words of one value
+    const uint64_t* up_row = &S.v[0][w > 0 ? w - 1 : 0][0][tz];
+    const uint64_t* dn_row = &S.v[0][w < NW - 1 ? w + 1 : w][0][tz];
+    uint64_t* my_row = &S.v[0][w][0][tz];
+    const bool cy = w == 0 && J > 0;
+    T prev[R];
+    unroll<R>([&](auto rc) { prev[decltype(rc)::value] = 0; });
     for (int t0 = 0; t0 < nsteps; t0 += kPrefetch) {
-        // compile-time step index, so each ring slot is its own register
+        // compile-time step index, so each prefetch slot stays in one register
         unroll<kPrefetch>([&](auto dc) {
             constexpr int d = decltype(dc)::value;
             const int t = t0 + d;
-            const int xl = t - ty - tz;
-            const bool active = P.col && xl >= 0 && xl < nxi && t < nsteps;
-            const uint64_t w = wring[d];
-            const float in = iring[d];
-            fetch(xl + kPrefetch, wring[d], iring[d]);
-            T vz = shr1(vprev);
-            T vy = 0;
-            if (ty > 0) {
-                wait_counter(&S.prog[ty - 1], t - 1, A.err);
-                vy = S.v[(t + kRingV - 1) % kRingV][ty - 1][tz];
+            // ---- receive
+            bool act[R];
+            unroll<R>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                act[r] = colr[r] && (unsigned)(t - (w * R + r) - tz) < (unsigned)nxi;
+            });
+            const bool need_up = w > 0 && t > 0;
+            const bool need_y = cy && act[0];
+            bool need_z[R];
+            unroll<R>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                need_z[r] = zin[r] && (unsigned)(t - (w * R + r)) < (unsigned)nxi;   // lane 0 of the row active
+            });
+            const bool need_bp = (d % kBpEvery) == 0 && w < NW - 1 && t >= kRingV - kBpEvery;
+            const int t_bp = t - (kRingV - kBpEvery);
+            const int pslot = t % RINGP;
+            uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
+            auto poll = [&]() {
+#pragma unroll
+                for (int k = 0; k < NP; k++) {
+                    a[k] = need_up ? lpeek(up_row + ((t + kRingV - 1) % kRingV) * kSlotV + k * kWordV) : 0;
+                    c[k] = cy ? lpeek(&S.yp[pslot][k][tz]) : 0;
+                }
+                unroll<R>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+#pragma unroll
+                    for (int k = 0; k < NP; k++) zq[r][k] = K > 0 ? lpeek(&S.zp[pslot][k][w * R + r]) : 0;
+                });
+                if (need_bp) bp = lpeek(dn_row + (t_bp % kRingV) * kSlotV);
+            };
+            auto ready = [&]() {
+                bool ok = !need_bp || tag_of(bp) >= t_bp;
+#pragma unroll
+                for (int k = 0; k < NP; k++) {
+                    ok = ok && (!need_up || tag_of(a[k]) == t - 1) && (!need_y || tag_of(c[k]) == t);
+                    unroll<R>([&](auto rc) {
+                        constexpr int r = decltype(rc)::value;
+                        ok = ok && (!need_z[r] || tag_of(zq[r][k]) == t);
+                    });
+                }
+                return ok;
+            };
+            poll();
+            if (!__all(ready())) {
+                int spins = 0;
+                do {
+                    if (++spins > kSpinLimit) {
+                        if (tz == 0) atomicOr(A.err, 1);
+                        break;
+                    }
+                    if ((spins & 1023) == 0 && gave_up(A.err)) break;
+                    if (spins > kBusyPolls) __builtin_amdgcn_s_sleep(1);
+                    poll();
+                } while (!__all(ready()));
             }
-            T vnew = 0;
-            if (active) {
-                const int x = x_of<FWD>(A, xl);
-                const int64_t i = ((int64_t)x * A.ny + P.y) * A.nz + P.z;
-                if (tz == 0) {
-                    vz = 0;
-                    if (cz) {
-                        uint32_t wd[NP];
-#pragma unroll
-                        for (int k = 0; k < NP; k++) wd[k] = wait_slot(&S.zp[t % kRingP][k][ty], t, A.err);
-                        from_words(vz, wd);
-                    }
+            // ---- compute the R rows (all upstream values are from step t-1)
+            T vnew[R];
+            unroll<R>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                T vy = 0, vz = shr1(prev[r]), vin;
+                if constexpr (r > 0) {
+                    vy = prev[r - 1];
+                } else {
+                    if (w > 0) from_words(vin, a), vy = vin;
+                    else if (need_y) from_words(vin, c), vy = vin;
                 }
-                if (ty == 0) {
-                    vy = 0;
-                    if (cy) {
-                        uint32_t wd[NP];
-#pragma unroll
-                        for (int k = 0; k < NP; k++) wd[k] = wait_slot(&S.yp[t % kRingP][k][tz], t, A.err);
-                        from_words(vy, wd);
-                    }
-                }
-                const uint32_t info = KIND == 0 ? (uint32_t)w : (uint32_t)(w >> 32);
-                if (KIND == 0) {
+                if (tz == 0 && need_z[r]) from_words(vin, zq[r]), vz = vin;
+                const uint32_t pe = (uint32_t)pc[r] * (uint32_t)plane;
+                if constexpr (KIND == 0) {
+                    const uint32_t info = act[r] ? (uint32_t)wring[d][r] : 0u;
+                    double v = 0;
                     if (info & NB_FLUID) {
                         const double tau = 0.97, sigma = 0.25;
-                        const double up[3] = {(double)vprev, (double)vy, (double)vz};
+                        const double upv[3] = {prev[r], vy, vz};
                         double eneg = 0, eneg_tau = 0;
 #pragma unroll
-                        for (int a = 0; a < 3; a++) {
-                            if (!(info & (NB_MX << a))) continue;
-                            const double pj = up[a];
+                        for (int ax = 0; ax < 3; ax++) {
+                            if (!(info & (NB_MX << ax))) continue;
+                            const double pj = upv[ax];
                             const double ap = -scale * pj;
-                            const double o1 = (info & (1u << (NB_UP_SHIFT + 2 * a))) ? -scale : 0.0;
-                            const double o2 = (info & (1u << (NB_UP_SHIFT + 2 * a + 1))) ? -scale : 0.0;
+                            const double o1 = (info & (1u << (NB_UP_SHIFT + 2 * ax))) ? -scale : 0.0;
+                            const double o2 = (info & (1u << (NB_UP_SHIFT + 2 * ax + 1))) ? -scale : 0.0;
                             eneg += ap * ap;
                             eneg_tau += ap * (o1 + o2) * pj;
                         }
@@ -334,66 +435,85 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                         for (int k = 0; k < cnt; k++) diag += scale;
                         double e = diag - eneg - eneg_tau * tau;
                         if (e < sigma * diag) e = diag < 1e-6 ? 1.0 : diag;
-                        const double p = 1.0 / sqrt(e);
-                        vnew = (T)p;
-                        A.P64[i] = p;
-                        const float p32 = (float)p;
-                        A.P32[i] = p32;
-                        A.W[i] = ((uint64_t)info << 32) | __float_as_uint(p32);
-                    } else {
-                        A.W[i] = (uint64_t)info << 32;
+                        v = 1.0 / sqrt(e);
                     }
-                } else if (info & NB_FLUID) {
-                    const float p = __uint_as_float((uint32_t)w);
+                    const bool fl = (info & NB_FLUID) != 0;
+                    const float p32 = (float)v;
+                    vnew[r] = v;
+                    // W for every cell of the sweep, P64/P32 for WATER cells
+                    bstore64(((uint64_t)info << 32) | (fl ? __float_as_uint(p32) : 0u), rOut0, act[r] ? lo8[r] : kOob,
+                             pe * 8u);
+                    bstore64(__builtin_bit_cast(uint64_t, v), rP64, fl ? lo8[r] : kOob, pe * 8u);
+                    bstore32(__float_as_uint(p32), rP32, fl ? lo4[r] : kOob, pe * 4u);
+                } else {
+                    const uint64_t wd = wring[d][r];
+                    const float p = __uint_as_float((uint32_t)wd);
+                    const bool fl = act[r] && ((uint32_t)(wd >> 32) & NB_FLUID) != 0;
                     float acc = 0.f;
-                    if (info & mx) acc += ms * (float)vprev;
-                    if (info & my) acc += ms * (float)vy;
-                    if (info & mz) acc += ms * (float)vz;
+                    acc += ms * prev[r];
+                    acc += ms * vy;
+                    acc += ms * vz;
+                    float outv, v;
                     if (FWD) {
-                        const float q = (in - acc) * p;
-                        A.out[i] = q;
-                        vnew = (T)(q * p);
+                        outv = (iring[d][r] - acc) * p;
+                        v = outv * p;
                     } else {
-                        const float zz = (in - acc * p) * p;
-                        A.out[i] = zz;
-                        vnew = (T)zz;
+                        outv = (iring[d][r] - acc * p) * p;
+                        v = outv;
                     }
+                    vnew[r] = fl ? v : 0.f;
+                    bstore32(__float_as_uint(outv), rOut0, fl ? lo4[r] : kOob, pe * 4u);
                 }
-                if (ty == WF_TY - 1) {
+                // -z edge packet of this row (lane 63) for the tile K+1, step t - 63 of its lane 0
+                if (t >= WF_TZ - 1) {
+                    const uint32_t so = (uint32_t)(t - (WF_TZ - 1)) * (TY * NP * 8u);
+                    const uint32_t vo = tz == WF_TZ - 1 ? (uint32_t)((w * R + r) * NP * 8) : kOob;
 #pragma unroll
                     for (int k = 0; k < NP; k++)
-                        gput(ybout + NP * ((int64_t)x * A.nz + P.z) + k, pack(word(vnew, k), A.epoch));
+                        bstore64<kDeviceScope>(pack(word(vnew[r], k), epoch), rzb, vo + (vo == kOob ? 0u : k * 8u), so);
                 }
-                if (tz == WF_TZ - 1) {
+            });
+            // -y edge packets of the tile's last row for the tile J+1, step t - (TY-1) of its row 0
+            if (w == NW - 1 && t >= TY - 1) {
+                const uint32_t so = (uint32_t)(t - (TY - 1)) * (WF_TZ * NP * 8u);
 #pragma unroll
-                    for (int k = 0; k < NP; k++)
-                        gput(zbout + NP * ((int64_t)x * A.ny + P.y) + k, pack(word(vnew, k), A.epoch));
-                }
+                for (int k = 0; k < NP; k++)
+                    bstore64<kDeviceScope>(pack(word(vnew[R - 1], k), epoch), ryb, (uint32_t)(tz * NP + k) * 8u, so);
             }
-            // reuse of ring slot t: the downstream wave must have consumed step t - kRingV
-            if (ty < WF_TY - 1) wait_counter(&S.prog[ty + 1], t - kRingV + 1, A.err);
-            S.v[t % kRingV][ty][tz] = vnew;
-            lds_drain();
-            if (tz == 0) lput32(&S.prog[ty], t);
-            vprev = vnew;
+            // ---- publish the last row for wave w+1, advance, prefetch step t + kPrefetch
+#pragma unroll
+            for (int k = 0; k < NP; k++)
+                lput(my_row + (t % kRingV) * kSlotV + k * kWordV, pack(word(vnew[R - 1], k), (uint32_t)t));
+            // the packet reads of steps <= t have completed (their values were used above)
+            if (d == kPrefetch - 1 && tz == 0) lput32(&S.prog[w], t);
+            unroll<R>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                prev[r] = vnew[r];
+                step_plane(pc[r], 1);
+                fetch(d, r);
+                step_plane(pf[r], 1);
+            });
         });
     }
+    if (tz == 0) lput32(&S.prog[w], nsteps);
 }
 
 template <int KIND>
 __global__ void __launch_bounds__(WF_THREADS) k_wave(WaveArgs A) {
+    using KD = Kind<KIND>;
+    constexpr int NP = KD::NP, RINGP = KD::RINGP;
     __shared__ WaveLds<KIND> S;
     const int tile = take_tile(A);
     if (tile < 0) return;
     const int J = tile / A.TK, K = tile % A.TK;
-    constexpr int NP = KindTraits<KIND>::NP;
-    for (int i = threadIdx.x; i < kRingP * NP * WF_TZ; i += WF_THREADS) (&S.yp[0][0][0])[i] = ~0ull;
-    for (int i = threadIdx.x; i < kRingP * NP * WF_TY; i += WF_THREADS) (&S.zp[0][0][0])[i] = ~0ull;
-    if (threadIdx.x < WF_TY) S.prog[threadIdx.x] = -1;
+    for (int i = threadIdx.x; i < kRingV * WF_NW * NP * WF_TZ; i += WF_THREADS) (&S.v[0][0][0][0])[i] = ~0ull;
+    for (int i = threadIdx.x; i < RINGP * NP * WF_TZ; i += WF_THREADS) (&S.yp[0][0][0])[i] = ~0ull;
+    for (int i = threadIdx.x; i < RINGP * NP * WF_TY; i += WF_THREADS) (&S.zp[0][0][0])[i] = ~0ull;
+    if (threadIdx.x < WF_NW) S.prog[threadIdx.x] = -1;
     __syncthreads();
     const int wave = threadIdx.x / 64;
-    if (wave < WF_TY) compute_wave<KIND>(A, S, J, K);
-    else if (wave == WF_TY) comm_wave<KIND, true>(A, S, J, K);
+    if (wave < WF_NW) compute_wave<KIND>(A, S, J, K);
+    else if (wave == WF_NW) comm_wave<KIND, true>(A, S, J, K);
     else comm_wave<KIND, false>(A, S, J, K);
 }
 
