@@ -44,8 +44,14 @@ constexpr int NB_UP_SHIFT = 10;
 // Persistent wavefront kernels (MIC(0) factor and triangular sweeps): a tile
 // of WF_TY x WF_TZ (y, z) columns per workgroup, WF_NW compute waves of WF_R
 // rows each plus two communication waves.
-constexpr int WF_R = 2;
-constexpr int WF_NW = 8;
+#ifndef FSIM_WF_R
+#define FSIM_WF_R 2
+#endif
+#ifndef FSIM_WF_NW
+#define FSIM_WF_NW 8
+#endif
+constexpr int WF_R = FSIM_WF_R;
+constexpr int WF_NW = FSIM_WF_NW;
 constexpr int WF_TY = WF_R * WF_NW;
 constexpr int WF_TZ = 64;
 constexpr int WF_THREADS = (WF_NW + 2) * WF_TZ;

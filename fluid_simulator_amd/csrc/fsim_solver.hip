@@ -37,12 +37,31 @@ struct SolveArgs {
 __device__ inline int64_t sidx(const SolveArgs& S, int x, int y, int z) { return ((int64_t)x * S.ny + y) * S.nz + z; }
 // index of the solver vectors, nbinfo and W (rotated layout)
 __device__ inline int64_t rsidx(const SolveArgs& S, int x, int y, int z) { return ridx(x, y, z, S.nx, S.ny, S.nz); }
-// cell (x, y, z) stored at rotated index i
-__device__ inline void rcell(const SolveArgs& S, int64_t i, int& x, int& y, int& z) {
-    z = (int)(i % S.nz);
-    const int64_t t = i / S.nz;
-    y = (int)(t % S.ny);
-    x = unrot_x((int)(t / S.ny), z, S.nx);
+// Rotated indices of the six neighbours of the interior cell stored at
+// plane p, row y, column z (rotated index i): +-y are +-nz; +-x are the
+// next/previous plane; +-z move one column and one plane, except across a
+// 64-wide z-tile edge where rot(z) wraps 63 <-> 0 and the plane moves by 63.
+struct RNeighbours {
+    int64_t px, mx, py, my, pz, mz;
+};
+__device__ inline int pmod(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+__device__ inline RNeighbours rneighbours(const SolveArgs& S, int64_t i, int p, int y, int z) {
+    const int64_t plane = (int64_t)S.ny * S.nz;
+    const int64_t yz = (int64_t)y * S.nz + z;
+    const int rz = rot_of(z);
+    const int pn = p + 1 == S.nx ? 0 : p + 1;
+    const int pp = p == 0 ? S.nx - 1 : p - 1;
+    RNeighbours n;
+    n.px = pn * plane + yz;
+    n.mx = pp * plane + yz;
+    n.py = i + S.nz;
+    n.my = i - S.nz;
+    n.pz = (rz == WF_TZ - 1 ? pmod(p - (WF_TZ - 1), S.nx) : pn) * plane + yz + 1;
+    n.mz = (rz == 0 ? pmod(p + (WF_TZ - 1), S.nx) : pp) * plane + yz - 1;
+    return n;
 }
 // rotated indices of the +a and -a neighbours of interior cell (x, y, z)
 __device__ inline int64_t rnb_plus(const SolveArgs& S, int x, int y, int z, int a) {
@@ -158,11 +177,12 @@ __global__ void __launch_bounds__(kBlock) k_nbinfo_rhs(RhsArgs A, int with_rhs) 
 #pragma clang fp contract(off)
     const SolveArgs& S = A.S;
     double bsq = 0.0, nfl = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.nc; i += stride) {
-        const int z = (int)(i % S.nz);
-        const int y = (int)((i / S.nz) % S.ny);
-        const int x = (int)(i / ((int64_t)S.nz * S.ny));
+    const int nrows = S.nx * S.ny;
+    // one grid row (x, y) per block iteration, lanes along z
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x)
+    for (int z = threadIdx.x; z < S.nz; z += blockDim.x) {
+        const int x = row / S.ny, y = row - x * S.ny;
+        const int64_t i = (int64_t)row * S.nz + z;
         uint16_t info = 0;
         float bv = 0.f;
         if (is_fluid(S, A.label, x, y, z)) {
@@ -311,25 +331,29 @@ __global__ void __launch_bounds__(kBlock) k_spmv_dot(SolveArgs S, const uint16_t
                                                      const float* __restrict__ s, float* __restrict__ q,
                                                      double* __restrict__ partials) {
     double d = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.nc; i += stride) {
-        uint16_t info = nbinfo[i];
-        float val = 0.f;
-        if (info & NB_FLUID) {
-            int x, y, z;
-            rcell(S, i, x, y, z);
-            const float ms = -S.scale_f;
-            int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
-            val = (S.scale_f * (float)cnt) * s[i];
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-                if (info & (NB_PX << a)) val += ms * s[rnb_plus(S, x, y, z, a)];
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-                if (info & (NB_MX << a)) val += ms * s[rnb_minus(S, x, y, z, a)];
-            d += (double)s[i] * (double)val;
+    const int nrows = S.nx * S.ny;
+    const float ms = -S.scale_f;
+    // one rotated row (plane p, row y) per block iteration, lanes along z
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const int p = row / S.ny, y = row - p * S.ny;
+        for (int z = threadIdx.x; z < S.nz; z += blockDim.x) {
+            const int64_t i = (int64_t)row * S.nz + z;
+            const uint16_t info = nbinfo[i];
+            float val = 0.f;
+            if (info & NB_FLUID) {
+                const RNeighbours n = rneighbours(S, i, p, y, z);
+                const int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
+                val = (S.scale_f * (float)cnt) * s[i];
+                if (info & NB_PX) val += ms * s[n.px];
+                if (info & NB_PY) val += ms * s[n.py];
+                if (info & NB_PZ) val += ms * s[n.pz];
+                if (info & NB_MX) val += ms * s[n.mx];
+                if (info & NB_MY) val += ms * s[n.my];
+                if (info & NB_MZ) val += ms * s[n.mz];
+                d += (double)s[i] * (double)val;
+            }
+            q[i] = val;
         }
-        q[i] = val;
     }
     block_reduce_store(d, 0.0, partials, AUX_NONE);
 }
@@ -375,21 +399,25 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(SolveArgs S, const uin
                                                           float* __restrict__ r, int replace, double* __restrict__ partials) {
 #pragma clang fp contract(off)
     double ss = 0.0, mx = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.nc; i += stride) {
-        uint16_t info = nbinfo[i];
-        if (info & NB_FLUID) {
-            int cx, cy, cz;
-            rcell(S, i, cx, cy, cz);
-            int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
+    const int nrows = S.nx * S.ny;
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const int p = row / S.ny, y = row - p * S.ny;
+        for (int z = threadIdx.x; z < S.nz; z += blockDim.x) {
+            const int64_t i = (int64_t)row * S.nz + z;
+            const uint16_t info = nbinfo[i];
+            if (!(info & NB_FLUID)) continue;
+            const RNeighbours n = rneighbours(S, i, p, y, z);
+            const int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
             double diag = 0;
             for (int k = 0; k < cnt; k++) diag += S.scale;
             double val = diag * x[i];
-            for (int a = 0; a < 3; a++)
-                if (info & (NB_PX << a)) val += -S.scale * x[rnb_plus(S, cx, cy, cz, a)];
-            for (int a = 0; a < 3; a++)
-                if (info & (NB_MX << a)) val += -S.scale * x[rnb_minus(S, cx, cy, cz, a)];
-            double t = (double)b[i] - val;
+            if (info & NB_PX) val += -S.scale * x[n.px];
+            if (info & NB_PY) val += -S.scale * x[n.py];
+            if (info & NB_PZ) val += -S.scale * x[n.pz];
+            if (info & NB_MX) val += -S.scale * x[n.mx];
+            if (info & NB_MY) val += -S.scale * x[n.my];
+            if (info & NB_MZ) val += -S.scale * x[n.mz];
+            const double t = (double)b[i] - val;
             ss += t * t;
             mx = fmax(mx, fabs(t));
             if (replace) r[i] = (float)t;
@@ -452,9 +480,10 @@ static int check_wave_err(fsim_ctx* ctx) {
     return FSIM_OK;
 }
 
-// Reductions use at most this many blocks (grid-stride), so the single-block
-// finalize sums few partials, in a fixed order.
-static unsigned red_blocks(int64_t n) { return std::min<unsigned>(nblocks(n), 1024u); }
+// Reductions use at most this many blocks (grid-stride): enough waves to
+// keep HBM busy (8 blocks of 4 waves per CU), few enough partials for the
+// single-block finalize, which sums them in a fixed order.
+static unsigned red_blocks(int64_t n) { return std::min<unsigned>(nblocks(n), 8192u); }
 
 static int read_scalars(fsim_ctx* ctx) {
     FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));

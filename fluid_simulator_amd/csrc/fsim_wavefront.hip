@@ -256,66 +256,75 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     // whole blocks of kPrefetch steps (the extra steps are inactive everywhere),
     // so no path through the loop skips a prefetch
     const int nsteps = (nxi + TY + WF_TZ - 2 + kPrefetch - 1) / kPrefetch * kPrefetch;
-    const int plane = A.ny * A.nz;
-    const int64_t nc = (int64_t)nx * plane;
+    const uint32_t plane = (uint32_t)(A.ny * A.nz);
+    const uint32_t nc = (uint32_t)nx * plane;   // below 2^28 (kWaveMaxCells)
+    const uint32_t plane8 = plane * 8u, nc8 = nc * 8u;
     const uint32_t epoch = A.epoch;
-    // buffer resources: per-cell arrays and this tile's outgoing packet rows
-    const __amdgpu_buffer_rsrc_t rIn0 = KIND == 0 ? rsrc(A.nbinfo, nc * 2) : rsrc(A.W, nc * 8);
-    const __amdgpu_buffer_rsrc_t rIn1 = rsrc(A.in, nc * 4);
-    const __amdgpu_buffer_rsrc_t rOut0 = KIND == 0 ? rsrc(A.W, nc * 8) : rsrc(A.out, nc * 4);
-    const __amdgpu_buffer_rsrc_t rP64 = rsrc(A.P64, nc * 8), rP32 = rsrc(A.P32, nc * 4);
+    // buffer resources.  factor: rA nbinfo, rB W, rC P64, rD P32;  sweeps: rA W, rB in, rC out
+    __amdgpu_buffer_rsrc_t rA, rB, rC, rD;
+    if constexpr (KIND == 0) {
+        rA = rsrc(A.nbinfo, (int64_t)nc * 2);
+        rB = rsrc(A.W, (int64_t)nc * 8);
+        rC = rsrc(A.P64, (int64_t)nc * 8);
+        rD = rsrc(A.P32, (int64_t)nc * 4);
+    } else {
+        rA = rsrc(A.W, (int64_t)nc * 8);
+        rB = rsrc(A.in, (int64_t)nc * 4);
+        rC = rsrc(A.out, (int64_t)nc * 4);
+        rD = rC;
+    }
     const int64_t tile = (int64_t)J * A.TK + K;
     const __amdgpu_buffer_rsrc_t ryb = rsrc(A.ybnd + tile * A.SP * WF_TZ * NP, (int64_t)A.SP * WF_TZ * NP * 8);
     const __amdgpu_buffer_rsrc_t rzb = rsrc(A.zbnd + tile * A.SP * TY * NP, (int64_t)A.SP * TY * NP * 8);
     const float ms = A.ms;
     const double scale = A.scale;
 
-    // per row: in-plane lane offset (elements; kOob marks lanes outside the
-    // grid), rotated plane of the current step and of the prefetch step
-    uint32_t lo[R];
-    int pc[R], pf[R];
-    bool colr[R], zin[R];
+    // Per row: the lane's in-plane byte offsets for 8-, 4- and 2-byte
+    // elements (kOob, shifted, stays out of range for every array), the
+    // step at which the lane enters the grid along x (t - base = x-index),
+    // and the byte offsets (8-byte elements) of the rotated plane of the
+    // current step and of the prefetch step, kept in scalar registers.
+    // Planes: FWD x + rot(z) = 1 + t - ty;  BWD x + rot(z) = nx + 61 + ty - t.
+    uint32_t lo8[R], lo4[R], lo2[R], zo[R], oc[R], op[R];
+    int base[R];
+    bool colr[R], colz[R];
     const int z = z_of<FWD>(A, K, tz);
     unroll<R>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         const int ty = w * R + r;
         const int y = y_of<FWD>(A, J, ty);
-        colr[r] = J * TY + ty < A.ny - 2 && z <= A.nz - 2;
-        lo[r] = colr[r] ? (uint32_t)(y * A.nz + z) : kOob;
-        // FWD: x + rot(z) = 1 + t - ty;  BWD: x + rot(z) = nx + 61 + ty - t
+        const bool rowin = J * TY + ty < A.ny - 2;
+        colr[r] = rowin && z <= A.nz - 2;
+        lo8[r] = colr[r] ? (uint32_t)(y * A.nz + z) * 8u : kOob;
+        lo4[r] = lo8[r] >> 1;
+        lo2[r] = lo8[r] >> 2;
+        base[r] = ty + tz;
+        // lane 63 writes the row's -z edge packet (every row, so each packet step is complete)
+        zo[r] = tz == WF_TZ - 1 ? (uint32_t)(ty * NP * 8) : kOob;
+        // lane 0 of the row reads -z packets when there is an upstream z tile
+        colz[r] = K > 0 && rowin;
         const int p0 = FWD ? ((1 - ty) % nx + nx) % nx : (61 + ty) % nx;
-        pc[r] = p0;
-        pf[r] = p0;
-        // lane 0 of this row consumes -z packets when K > 0 and the row is in the grid
-        zin[r] = K > 0 && J * TY + ty < A.ny - 2;
+        oc[r] = (uint32_t)p0 * plane8;
+        op[r] = oc[r];
     });
-    auto step_plane = [&](int& p, int n) {   // advance a rotated plane by n steps
-        p = FWD ? p + n : p - n;
-        if (p >= nx) p -= nx;
-        if (p < 0) p += nx;
+    auto advance = [&](uint32_t& o) {   // the rotated plane of the next step
+        if (FWD) {
+            o += plane8;
+            if (o >= nc8) o -= nc8;
+        } else {
+            o = o >= plane8 ? o - plane8 : o + nc8 - plane8;
+        }
     };
-    // byte offsets for 8-, 4- and 2-byte elements
-    auto off8 = [](uint32_t e) { return e == kOob ? kOob : e * 8u; };
-    auto off4 = [](uint32_t e) { return e == kOob ? kOob : e * 4u; };
-    auto off2 = [](uint32_t e) { return e == kOob ? kOob : e * 2u; };
-    uint32_t lo8[R], lo4[R], lo2[R];
-    unroll<R>([&](auto rc) {
-        constexpr int r = decltype(rc)::value;
-        lo8[r] = off8(lo[r]);
-        lo4[r] = off4(lo[r]);
-        lo2[r] = off2(lo[r]);
-    });
     // prefetch ring: per-cell inputs of the next kPrefetch steps
     uint64_t wring[kPrefetch][R];
     float iring[kPrefetch][R];
     auto fetch = [&](int d, int r) {
-        const uint32_t pe = (uint32_t)pf[r] * (uint32_t)plane;
         if constexpr (KIND == 0) {
-            wring[d][r] = bload16(rIn0, lo2[r], pe * 2u);
+            wring[d][r] = bload16(rA, lo2[r], op[r] >> 2);
             iring[d][r] = 0.f;
         } else {
-            wring[d][r] = bload64(rIn0, lo8[r], pe * 8u);
-            iring[d][r] = bload32f(rIn1, lo4[r], pe * 4u);
+            wring[d][r] = bload64(rA, lo8[r], op[r]);
+            iring[d][r] = bload32f(rB, lo4[r], op[r] >> 1);
         }
     };
     // issue order pinned, and each prologue step issues as many memory
@@ -324,71 +333,73 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     // number of younger operations on entry as at every later step
     unroll<kPrefetch>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
-        unroll<2 * R>([&](auto) { bstore32(0u, rOut0, kOob, 0); });
+        unroll<2 * R>([&](auto) { bstore32(0u, rC, kOob, 0); });
         unroll<R>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
             fetch(d, r);
-            step_plane(pf[r], 1);
+            advance(op[r]);
         });
         __builtin_amdgcn_sched_barrier(0);
     });
     constexpr int kSlotV = NW * NP * WF_TZ;   // words per value-ring slot
-    constexpr int kWordV = WF_TZ;             // distance between the words of one value
-    const uint64_t* up_row = &S.v[0][w > 0 ? w - 1 : 0][0][tz];
-    const uint64_t* dn_row = &S.v[0][w < NW - 1 ? w + 1 : w][0][tz];
-    uint64_t* my_row = &S.v[0][w][0][tz];
+    const uint64_t* up_base = &S.v[0][w > 0 ? w - 1 : 0][0][tz];
+    const uint64_t* dn_base = &S.v[0][w < NW - 1 ? w + 1 : w][0][tz];
+    uint64_t* my_base = &S.v[0][w][0][tz];
+    const uint64_t* yp_base = &S.yp[0][0][tz];
+    const uint64_t* zp_base = &S.zp[0][0][w * R];
     const bool cy = w == 0 && J > 0;
     T prev[R];
     unroll<R>([&](auto rc) { prev[decltype(rc)::value] = 0; });
     for (int t0 = 0; t0 < nsteps; t0 += kPrefetch) {
         // compile-time step index, so each prefetch slot stays in one register
+        // and the value-ring slots are constants
         unroll<kPrefetch>([&](auto dc) {
             constexpr int d = decltype(dc)::value;
+            constexpr int s_up = (d + kRingV - 1) % kRingV, s_bp = (d + kBpEvery) % kRingV;
+            constexpr bool bp_step = d % kBpEvery == 0;
             const int t = t0 + d;
-            // ---- receive
-            bool act[R];
-            unroll<R>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                act[r] = colr[r] && (unsigned)(t - (w * R + r) - tz) < (unsigned)nxi;
-            });
+            const int ps = t % RINGP;
+            // ---- receive: unconditional LDS reads of valid slots, then tag checks
             const bool need_up = w > 0 && t > 0;
-            const bool need_y = cy && act[0];
-            bool need_z[R];
+            const bool need_bp = bp_step && w < NW - 1 && t >= kRingV - kBpEvery;
+            const int t_bp = t - (kRingV - kBpEvery);
+            bool act[R], need_z[R];
             unroll<R>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                need_z[r] = zin[r] && (unsigned)(t - (w * R + r)) < (unsigned)nxi;   // lane 0 of the row active
+                act[r] = colr[r] && (unsigned)(t - base[r]) < (unsigned)nxi;
+                need_z[r] = colz[r] && (unsigned)(t - (w * R + r)) < (unsigned)nxi;
             });
-            const bool need_bp = (d % kBpEvery) == 0 && w < NW - 1 && t >= kRingV - kBpEvery;
-            const int t_bp = t - (kRingV - kBpEvery);
-            const int pslot = t % RINGP;
+            const bool need_y = cy && act[0];
             uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
-            auto poll = [&]() {
+            auto rd = [&]() {
 #pragma unroll
                 for (int k = 0; k < NP; k++) {
-                    a[k] = need_up ? lpeek(up_row + ((t + kRingV - 1) % kRingV) * kSlotV + k * kWordV) : 0;
-                    c[k] = cy ? lpeek(&S.yp[pslot][k][tz]) : 0;
+                    a[k] = lpeek(up_base + s_up * kSlotV + k * WF_TZ);
+                    c[k] = lpeek(yp_base + ps * NP * WF_TZ + k * WF_TZ);
                 }
                 unroll<R>([&](auto rc) {
                     constexpr int r = decltype(rc)::value;
 #pragma unroll
-                    for (int k = 0; k < NP; k++) zq[r][k] = K > 0 ? lpeek(&S.zp[pslot][k][w * R + r]) : 0;
+                    for (int k = 0; k < NP; k++) zq[r][k] = lpeek(zp_base + ps * NP * TY + k * TY + r);
                 });
-                if (need_bp) bp = lpeek(dn_row + (t_bp % kRingV) * kSlotV);
+                if constexpr (bp_step) bp = lpeek(dn_base + s_bp * kSlotV);
             };
             auto ready = [&]() {
-                bool ok = !need_bp || tag_of(bp) >= t_bp;
+                bool lane = !need_bp || tag_of(bp) >= t_bp;
+                bool uni = true;
 #pragma unroll
                 for (int k = 0; k < NP; k++) {
-                    ok = ok && (!need_up || tag_of(a[k]) == t - 1) && (!need_y || tag_of(c[k]) == t);
+                    lane = lane && (!need_up || tag_of(a[k]) == t - 1) && (!need_y || tag_of(c[k]) == t);
                     unroll<R>([&](auto rc) {
                         constexpr int r = decltype(rc)::value;
-                        ok = ok && (!need_z[r] || tag_of(zq[r][k]) == t);
+                        // -z packets are read by every lane from one slot: test one copy
+                        uni = uni && (!need_z[r] || __builtin_amdgcn_readfirstlane((int)(zq[r][k] >> 32)) == t);
                     });
                 }
-                return ok;
+                return uni && __all(lane);
             };
-            poll();
-            if (!__all(ready())) {
+            rd();
+            if (!ready()) {
                 int spins = 0;
                 do {
                     if (++spins > kSpinLimit) {
@@ -397,8 +408,8 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     }
                     if ((spins & 1023) == 0 && gave_up(A.err)) break;
                     if (spins > kBusyPolls) __builtin_amdgcn_s_sleep(1);
-                    poll();
-                } while (!__all(ready()));
+                    rd();
+                } while (!ready());
             }
             // ---- compute the R rows (all upstream values are from step t-1)
             T vnew[R];
@@ -412,7 +423,6 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     else if (need_y) from_words(vin, c), vy = vin;
                 }
                 if (tz == 0 && need_z[r]) from_words(vin, zq[r]), vz = vin;
-                const uint32_t pe = (uint32_t)pc[r] * (uint32_t)plane;
                 if constexpr (KIND == 0) {
                     const uint32_t info = act[r] ? (uint32_t)wring[d][r] : 0u;
                     double v = 0;
@@ -441,10 +451,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     const float p32 = (float)v;
                     vnew[r] = v;
                     // W for every cell of the sweep, P64/P32 for WATER cells
-                    bstore64(((uint64_t)info << 32) | (fl ? __float_as_uint(p32) : 0u), rOut0, act[r] ? lo8[r] : kOob,
-                             pe * 8u);
-                    bstore64(__builtin_bit_cast(uint64_t, v), rP64, fl ? lo8[r] : kOob, pe * 8u);
-                    bstore32(__float_as_uint(p32), rP32, fl ? lo4[r] : kOob, pe * 4u);
+                    bstore64(((uint64_t)info << 32) | (fl ? __float_as_uint(p32) : 0u), rB, act[r] ? lo8[r] : kOob, oc[r]);
+                    bstore64(__builtin_bit_cast(uint64_t, v), rC, fl ? lo8[r] : kOob, oc[r]);
+                    bstore32(__float_as_uint(p32), rD, fl ? lo4[r] : kOob, oc[r] >> 1);
                 } else {
                     const uint64_t wd = wring[d][r];
                     const float p = __uint_as_float((uint32_t)wd);
@@ -462,15 +471,14 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                         v = outv;
                     }
                     vnew[r] = fl ? v : 0.f;
-                    bstore32(__float_as_uint(outv), rOut0, fl ? lo4[r] : kOob, pe * 4u);
+                    bstore32(__float_as_uint(outv), rC, fl ? lo4[r] : kOob, oc[r] >> 1);
                 }
                 // -z edge packet of this row (lane 63) for the tile K+1, step t - 63 of its lane 0
                 if (t >= WF_TZ - 1) {
                     const uint32_t so = (uint32_t)(t - (WF_TZ - 1)) * (TY * NP * 8u);
-                    const uint32_t vo = tz == WF_TZ - 1 ? (uint32_t)((w * R + r) * NP * 8) : kOob;
 #pragma unroll
                     for (int k = 0; k < NP; k++)
-                        bstore64<kDeviceScope>(pack(word(vnew[r], k), epoch), rzb, vo + (vo == kOob ? 0u : k * 8u), so);
+                        bstore64<kDeviceScope>(pack(word(vnew[r], k), epoch), rzb, zo[r] + k * 8u, so);
                 }
             });
             // -y edge packets of the tile's last row for the tile J+1, step t - (TY-1) of its row 0
@@ -482,16 +490,15 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             }
             // ---- publish the last row for wave w+1, advance, prefetch step t + kPrefetch
 #pragma unroll
-            for (int k = 0; k < NP; k++)
-                lput(my_row + (t % kRingV) * kSlotV + k * kWordV, pack(word(vnew[R - 1], k), (uint32_t)t));
+            for (int k = 0; k < NP; k++) lput(my_base + d * kSlotV + k * WF_TZ, pack(word(vnew[R - 1], k), (uint32_t)t));
             // the packet reads of steps <= t have completed (their values were used above)
             if (d == kPrefetch - 1 && tz == 0) lput32(&S.prog[w], t);
             unroll<R>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
                 prev[r] = vnew[r];
-                step_plane(pc[r], 1);
+                advance(oc[r]);
                 fetch(d, r);
-                step_plane(pf[r], 1);
+                advance(op[r]);
             });
         });
     }
