@@ -55,6 +55,15 @@ constexpr int WF_NW = FSIM_WF_NW;
 constexpr int WF_TY = WF_R * WF_NW;
 constexpr int WF_TZ = 64;
 constexpr int WF_THREADS = (WF_NW + 2) * WF_TZ;
+constexpr int WF_PREFETCH = 8;   // steps of per-cell inputs loaded ahead (and the loop unroll)
+// Steps of one tile: all rows of compute wave w are at x-index t - w - tz at
+// step t, so the last cell (wave WF_NW-1, lane 63, x-index nxi-1) is at step
+// nxi + WF_NW + WF_TZ - 3; rounded up to whole blocks of WF_PREFETCH steps
+// (the extra steps are inactive everywhere) so that no path through the step
+// loop skips a prefetch.
+__host__ __device__ inline int wave_steps(int nxi) {
+    return (nxi + WF_NW + WF_TZ - 2 + WF_PREFETCH - 1) / WF_PREFETCH * WF_PREFETCH;
+}
 // the wavefront kernels address cells with 32-bit byte offsets into arrays
 // of up to 8-byte words; larger grids use the plane-by-plane kernels
 constexpr int64_t kWaveMaxCells = ((int64_t(1) << 31) - 1) / 8;

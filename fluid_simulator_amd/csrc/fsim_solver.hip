@@ -124,13 +124,18 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
     lds_s[threadIdx.x] = s;
     lds_m[threadIdx.x] = m;
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    s = 0;
-    m = 0;
-    for (int i = 0; i < kBlock; i++) {
-        s += lds_s[i];
-        m = aux_sum ? m + lds_m[i] : fmax(m, lds_m[i]);
+    // pairwise tree over the threads' sums, fixed order
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            lds_s[threadIdx.x] += lds_s[threadIdx.x + off];
+            lds_m[threadIdx.x] = aux_sum ? lds_m[threadIdx.x] + lds_m[threadIdx.x + off]
+                                         : fmax(lds_m[threadIdx.x], lds_m[threadIdx.x + off]);
+        }
+        __syncthreads();
     }
+    if (threadIdx.x != 0) return;
+    s = lds_s[0];
+    m = lds_m[0];
     switch (op) {
     case FIN_BB: sc->bb = s; sc->nfluid = m; break;
     case FIN_SIGMA0: sc->sigma = s; break;
@@ -702,10 +707,9 @@ int wave_init(fsim_ctx* ctx) {
     A.nz = ctx->g.nz;
     A.TJ = (A.ny - 2 + WF_TY - 1) / WF_TY;
     A.TK = (A.nz - 2 + WF_TZ - 1) / WF_TZ;
-    // packet rows: one per step of a tile (the step count rounded up to the
-    // kernels' 8-step blocks, plus slack); two words per value (the factor's
+    // packet rows: one per step of a tile; two words per value (the factor's
     // FP64), the sweeps use one
-    A.SP = (A.nx - 2 + WF_TY + WF_TZ - 2 + 7) / 8 * 8 + 8;
+    A.SP = wave_steps(A.nx - 2);
     const int64_t ny_pk = 2 * (int64_t)A.TJ * A.TK * A.SP * WF_TZ;   // per tile (J, K)
     const int64_t nz_pk = 2 * (int64_t)A.TJ * A.TK * A.SP * WF_TY;
     if (ctx->g.nc > kWaveMaxCells) ctx->sweep_mode = 1;

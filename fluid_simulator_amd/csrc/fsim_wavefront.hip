@@ -58,7 +58,7 @@ namespace fsim {
 
 constexpr int kSpinLimit = 1 << 21;
 constexpr int kBusyPolls = 32;   // polls before a waiting wave starts to sleep
-constexpr int kPrefetch = 8;     // compute-wave register prefetch (steps)
+constexpr int kPrefetch = WF_PREFETCH;   // compute-wave register prefetch (steps)
 constexpr int kRingV = 8;        // LDS ring of row values between compute waves (steps)
 constexpr int kBpEvery = 4;      // ring-reuse check period (steps); allowed lag kRingV - kBpEvery
 constexpr uint32_t kOob = 0x80000000u;        // buffer offset beyond every array: reads 0, drops stores
@@ -186,7 +186,9 @@ __device__ void comm_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
     const bool on = l < NL;
     const int li = on ? l : 0;
     const int nxi = A.nx - 2;
-    const int end = NL - 1 + nxi;                 // steps 0 .. end-1 are consumed
+    // steps 0 .. end-1 are consumed: row 0 lane l at l .. l+nxi-1 (-y edge),
+    // lane 0 of the rows of wave w at w .. w+nxi-1 (-z edge)
+    const int end = (YEDGE ? WF_TZ - 1 : WF_NW - 1) + nxi;
     // packets written by the upstream tile (J-1, K) or (J, K-1)
     const int64_t up = YEDGE ? (int64_t)(J - 1) * A.TK + K : (int64_t)J * A.TK + K - 1;
     const uint64_t* base = YEDGE ? A.ybnd + up * A.SP * WF_TZ * NP : A.zbnd + up * A.SP * WF_TY * NP;
@@ -253,9 +255,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int tz = threadIdx.x % 64;
     const int nx = A.nx, nxi = A.nx - 2;
-    // whole blocks of kPrefetch steps (the extra steps are inactive everywhere),
-    // so no path through the loop skips a prefetch
-    const int nsteps = (nxi + TY + WF_TZ - 2 + kPrefetch - 1) / kPrefetch * kPrefetch;
+    const int nsteps = wave_steps(nxi);
     const uint32_t plane = (uint32_t)(A.ny * A.nz);
     const uint32_t nc = (uint32_t)nx * plane;   // below 2^28 (kWaveMaxCells)
     const uint32_t plane8 = plane * 8u, nc8 = nc * 8u;
@@ -279,16 +279,20 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     const float ms = A.ms;
     const double scale = A.scale;
 
+    // All R rows of wave w process x-index xl = t - w - tz at step t (row r
+    // uses row r-1's value of the same step), so the lane enters the grid
+    // along x at step w + tz, and the wave's cells of one step lie in one
+    // rotated plane: FWD x + rot(z) = 1 + t - w;  BWD x + rot(z) = nx + 61 + w - t.
     // Per row: the lane's in-plane byte offsets for 8-, 4- and 2-byte
-    // elements (kOob, shifted, stays out of range for every array), the
-    // step at which the lane enters the grid along x (t - base = x-index),
-    // and the byte offsets (8-byte elements) of the rotated plane of the
-    // current step and of the prefetch step, kept in scalar registers.
-    // Planes: FWD x + rot(z) = 1 + t - ty;  BWD x + rot(z) = nx + 61 + ty - t.
-    uint32_t lo8[R], lo4[R], lo2[R], zo[R], oc[R], op[R];
-    int base[R];
+    // elements (kOob, shifted, stays out of range for every array).  Per
+    // wave: the byte offsets (8-byte elements) of the rotated plane of the
+    // current step and of the prefetch step, in scalar registers.
+    uint32_t lo8[R], lo4[R], lo2[R], zo[R];
     bool colr[R], colz[R];
     const int z = z_of<FWD>(A, K, tz);
+    const int base = w + tz;
+    uint32_t oc = (uint32_t)(FWD ? ((1 - w) % nx + nx) % nx : (61 + w) % nx) * plane8;
+    uint32_t op = oc;
     unroll<R>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         const int ty = w * R + r;
@@ -298,14 +302,10 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
         lo8[r] = colr[r] ? (uint32_t)(y * A.nz + z) * 8u : kOob;
         lo4[r] = lo8[r] >> 1;
         lo2[r] = lo8[r] >> 2;
-        base[r] = ty + tz;
         // lane 63 writes the row's -z edge packet (every row, so each packet step is complete)
         zo[r] = tz == WF_TZ - 1 ? (uint32_t)(ty * NP * 8) : kOob;
         // lane 0 of the row reads -z packets when there is an upstream z tile
         colz[r] = K > 0 && rowin;
-        const int p0 = FWD ? ((1 - ty) % nx + nx) % nx : (61 + ty) % nx;
-        oc[r] = (uint32_t)p0 * plane8;
-        op[r] = oc[r];
     });
     auto advance = [&](uint32_t& o) {   // the rotated plane of the next step
         if (FWD) {
@@ -320,11 +320,11 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     float iring[kPrefetch][R];
     auto fetch = [&](int d, int r) {
         if constexpr (KIND == 0) {
-            wring[d][r] = bload16(rA, lo2[r], op[r] >> 2);
+            wring[d][r] = bload16(rA, lo2[r], op >> 2);
             iring[d][r] = 0.f;
         } else {
-            wring[d][r] = bload64(rA, lo8[r], op[r]);
-            iring[d][r] = bload32f(rB, lo4[r], op[r] >> 1);
+            wring[d][r] = bload64(rA, lo8[r], op);
+            iring[d][r] = bload32f(rB, lo4[r], op >> 1);
         }
     };
     // issue order pinned, and each prologue step issues as many memory
@@ -334,11 +334,8 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     unroll<kPrefetch>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
         unroll<2 * R>([&](auto) { bstore32(0u, rC, kOob, 0); });
-        unroll<R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            fetch(d, r);
-            advance(op[r]);
-        });
+        unroll<R>([&](auto rc) { fetch(d, decltype(rc)::value); });
+        advance(op);
         __builtin_amdgcn_sched_barrier(0);
     });
     constexpr int kSlotV = NW * NP * WF_TZ;   // words per value-ring slot
@@ -363,11 +360,15 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             const bool need_up = w > 0 && t > 0;
             const bool need_bp = bp_step && w < NW - 1 && t >= kRingV - kBpEvery;
             const int t_bp = t - (kRingV - kBpEvery);
+            // the lane's x-index is in the grid (same for all rows of the wave),
+            // and the same test for lane 0 (the -z packet consumer)
+            const bool xin = (unsigned)(t - base) < (unsigned)nxi;
+            const bool xin0 = (unsigned)(t - w) < (unsigned)nxi;
             bool act[R], need_z[R];
             unroll<R>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                act[r] = colr[r] && (unsigned)(t - base[r]) < (unsigned)nxi;
-                need_z[r] = colz[r] && (unsigned)(t - (w * R + r)) < (unsigned)nxi;
+                act[r] = colr[r] && xin;
+                need_z[r] = colz[r] && xin0;
             });
             const bool need_y = cy && act[0];
             uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
@@ -417,7 +418,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                 constexpr int r = decltype(rc)::value;
                 T vy = 0, vz = shr1(prev[r]), vin;
                 if constexpr (r > 0) {
-                    vy = prev[r - 1];
+                    vy = vnew[r - 1];   // the row above, same step (same x)
                 } else {
                     if (w > 0) from_words(vin, a), vy = vin;
                     else if (need_y) from_words(vin, c), vy = vin;
@@ -451,9 +452,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     const float p32 = (float)v;
                     vnew[r] = v;
                     // W for every cell of the sweep, P64/P32 for WATER cells
-                    bstore64(((uint64_t)info << 32) | (fl ? __float_as_uint(p32) : 0u), rB, act[r] ? lo8[r] : kOob, oc[r]);
-                    bstore64(__builtin_bit_cast(uint64_t, v), rC, fl ? lo8[r] : kOob, oc[r]);
-                    bstore32(__float_as_uint(p32), rD, fl ? lo4[r] : kOob, oc[r] >> 1);
+                    bstore64(((uint64_t)info << 32) | (fl ? __float_as_uint(p32) : 0u), rB, act[r] ? lo8[r] : kOob, oc);
+                    bstore64(__builtin_bit_cast(uint64_t, v), rC, fl ? lo8[r] : kOob, oc);
+                    bstore32(__float_as_uint(p32), rD, fl ? lo4[r] : kOob, oc >> 1);
                 } else {
                     const uint64_t wd = wring[d][r];
                     const float p = __uint_as_float((uint32_t)wd);
@@ -471,7 +472,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                         v = outv;
                     }
                     vnew[r] = fl ? v : 0.f;
-                    bstore32(__float_as_uint(outv), rC, fl ? lo4[r] : kOob, oc[r] >> 1);
+                    bstore32(__float_as_uint(outv), rC, fl ? lo4[r] : kOob, oc >> 1);
                 }
                 // -z edge packet of this row (lane 63) for the tile K+1, step t - 63 of its lane 0
                 if (t >= WF_TZ - 1) {
@@ -481,9 +482,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                         bstore64<kDeviceScope>(pack(word(vnew[r], k), epoch), rzb, zo[r] + k * 8u, so);
                 }
             });
-            // -y edge packets of the tile's last row for the tile J+1, step t - (TY-1) of its row 0
-            if (w == NW - 1 && t >= TY - 1) {
-                const uint32_t so = (uint32_t)(t - (TY - 1)) * (WF_TZ * NP * 8u);
+            // -y edge packets of the tile's last row for the tile J+1, step t - (NW-1) of its row 0
+            if (w == NW - 1 && t >= NW - 1) {
+                const uint32_t so = (uint32_t)(t - (NW - 1)) * (WF_TZ * NP * 8u);
 #pragma unroll
                 for (int k = 0; k < NP; k++)
                     bstore64<kDeviceScope>(pack(word(vnew[R - 1], k), epoch), ryb, (uint32_t)(tz * NP + k) * 8u, so);
@@ -496,10 +497,10 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             unroll<R>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
                 prev[r] = vnew[r];
-                advance(oc[r]);
                 fetch(d, r);
-                advance(op[r]);
             });
+            advance(oc);
+            advance(op);
         });
     }
     if (tz == 0) lput32(&S.prog[w], nsteps);
