@@ -48,7 +48,7 @@ constexpr int NB_UP_SHIFT = 10;
 #define FSIM_WF_R 2
 #endif
 #ifndef FSIM_WF_NW
-#define FSIM_WF_NW 8
+#define FSIM_WF_NW 4
 #endif
 constexpr int WF_R = FSIM_WF_R;
 constexpr int WF_NW = FSIM_WF_NW;

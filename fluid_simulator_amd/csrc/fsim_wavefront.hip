@@ -70,7 +70,7 @@ struct Kind {   // 1 = forward sweep, 2 = backward sweep (FP32)
     using T = float;
     static constexpr bool FWD = KIND == 1;
     static constexpr int NP = 1;       // 32-bit words per value
-    static constexpr int BATCH = 32;   // comm-wave batch (steps)
+    static constexpr int BATCH = 4;    // comm-wave poll batch (steps): short polls keep hop latency low
     static constexpr int RINGP = 64;   // LDS ring of boundary packets (steps)
 };
 template <>
@@ -78,7 +78,7 @@ struct Kind<0> {   // MIC(0) factor (FP64)
     using T = double;
     static constexpr bool FWD = true;
     static constexpr int NP = 2;
-    static constexpr int BATCH = 16;
+    static constexpr int BATCH = 4;
     static constexpr int RINGP = 32;
 };
 
@@ -385,19 +385,19 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                 });
                 if constexpr (bp_step) bp = lpeek(dn_base + s_bp * kSlotV);
             };
+            // branch-free: per-lane masks, one ballot (the -z packet slots are
+            // read by every lane from one address, so their test is uniform)
             auto ready = [&]() {
-                bool lane = !need_bp || tag_of(bp) >= t_bp;
-                bool uni = true;
+                bool lane = (tag_of(bp) >= t_bp) | !need_bp;
 #pragma unroll
                 for (int k = 0; k < NP; k++) {
-                    lane = lane && (!need_up || tag_of(a[k]) == t - 1) && (!need_y || tag_of(c[k]) == t);
+                    lane = lane & ((tag_of(a[k]) == t - 1) | !need_up) & ((tag_of(c[k]) == t) | !need_y);
                     unroll<R>([&](auto rc) {
                         constexpr int r = decltype(rc)::value;
-                        // -z packets are read by every lane from one slot: test one copy
-                        uni = uni && (!need_z[r] || __builtin_amdgcn_readfirstlane((int)(zq[r][k] >> 32)) == t);
+                        lane = lane & ((tag_of(zq[r][k]) == t) | !need_z[r]);
                     });
                 }
-                return uni && __all(lane);
+                return __builtin_amdgcn_ballot_w64(!lane) == 0;
             };
             rd();
             if (!ready()) {
