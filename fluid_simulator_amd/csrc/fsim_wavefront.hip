@@ -307,13 +307,16 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
         // lane 0 of the row reads -z packets when there is an upstream z tile
         colz[r] = K > 0 && rowin;
     });
-    auto advance = [&](uint32_t& o) {   // the rotated plane of the next step
+    // the rotated plane of the next step; readfirstlane keeps the offset in a
+    // scalar register (a per-lane copy turns each buffer access into a loop)
+    auto advance = [&](uint32_t& o) {
         if (FWD) {
             o += plane8;
             if (o >= nc8) o -= nc8;
         } else {
             o = o >= plane8 ? o - plane8 : o + nc8 - plane8;
         }
+        o = __builtin_amdgcn_readfirstlane(o);
     };
     // prefetch ring: per-cell inputs of the next kPrefetch steps
     uint64_t wring[kPrefetch][R];
