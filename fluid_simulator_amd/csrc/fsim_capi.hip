@@ -104,11 +104,12 @@ static void free_all(fsim_ctx* ctx) {
     void* ptrs[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed, ctx->u, ctx->v, ctx->w,
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
                     ctx->scan_tmp, ctx->nbinfo, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
-                    ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->W, ctx->ybnd, ctx->zbnd,
+                    ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->W, ctx->ybnd, ctx->zbnd, ctx->s2, ctx->red_count,
                     ctx->wave_ticket, ctx->wave_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
+    if (ctx->res_ev) (void)hipEventDestroy(ctx->res_ev);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->prof_pool) (void)hipEventDestroy(e);
@@ -171,7 +172,8 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
         (rc = dalloc(ctx, &ctx->density, nc)) || (rc = dalloc(ctx, &ctx->cell_count, nc + 1)) ||
         (rc = dalloc(ctx, &ctx->cell_start, nc + 2)) || (rc = dalloc(ctx, &ctx->nbinfo, nc)) ||
         (rc = dalloc(ctx, &ctx->b, nc)) || (rc = dalloc(ctx, &ctx->r, nc)) || (rc = dalloc(ctx, &ctx->z, nc)) ||
-        (rc = dalloc(ctx, &ctx->s, nc)) || (rc = dalloc(ctx, &ctx->q, nc)) || (rc = dalloc(ctx, &ctx->y, nc)) ||
+        (rc = dalloc(ctx, &ctx->s, nc)) || (rc = dalloc(ctx, &ctx->s2, nc)) || (rc = dalloc(ctx, &ctx->q, nc)) ||
+        (rc = dalloc(ctx, &ctx->y, nc)) || (rc = dalloc(ctx, &ctx->red_count, 1)) ||
         (rc = dalloc(ctx, &ctx->P32, nc)) || (rc = dalloc(ctx, &ctx->P64, nc)) || (rc = dalloc(ctx, &ctx->x, nc)) ||
         (rc = dalloc(ctx, &ctx->scal, 1)) || (rc = dalloc(ctx, &ctx->obs_dev, kMaxObstacles)))
         return fail(rc);
@@ -183,15 +185,17 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
         return fail(FSIM_ERR_OOM);
     for (auto& e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(FSIM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->res_ev, hipEventDisableTiming) != hipSuccess) return fail(FSIM_ERR_HIP);
     // initial grid state: zero faces, AIR labels (MacGrid ctor, macGrid.cpp:9-17)
-    float* f32s[] = {ctx->u, ctx->v, ctx->w, ctx->u2, ctx->v2, ctx->w2, ctx->density, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q,
-                     ctx->y, ctx->P32};
+    float* f32s[] = {ctx->u, ctx->v, ctx->w, ctx->u2, ctx->v2, ctx->w2, ctx->density, ctx->b, ctx->r, ctx->z, ctx->s, ctx->s2,
+                     ctx->q, ctx->y, ctx->P32};
     for (float* p : f32s)
         if (fill_f32(ctx, p, 0.0f, nc)) return fail(FSIM_ERR_HIP);
     if (fill_u8(ctx, ctx->label, FSIM_AIR, nc) || fill_f64(ctx, ctx->x, 0.0, nc) || fill_f64(ctx, ctx->P64, 0.0, nc) ||
         fill_i32(ctx, ctx->cell_count, 0, nc + 1) || fill_i32(ctx, ctx->cell_start, 0, nc + 2))
         return fail(FSIM_ERR_HIP);
     if (hipMemsetAsync(ctx->nbinfo, 0, sizeof(uint16_t) * nc, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(ctx->red_count, 0, sizeof(unsigned), ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(FSIM_ERR_HIP);
     if (particles_reserve(ctx, 1024) != FSIM_OK) return fail(FSIM_ERR_OOM);

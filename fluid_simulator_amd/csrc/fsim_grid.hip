@@ -96,42 +96,121 @@ struct P2GArgs {
     int fuse_mark;   // 1: also label WATER where the bin count is > 0
 };
 
+// Gather form of the transfer: the thread of cell (x,y,z) owns that cell's
+// three faces and sums, in a fixed order (neighbour cells dx, dy, dz, then
+// particles in sorted order), every particle whose 2x2x2 face stencil of an
+// axis holds the face (base from face_base, as in the reference's scatter).
+//
+// A block owns 256 consecutive cells.  For one (dx, dy) the three dz cells of
+// every thread are consecutive, so the particles the block needs are one
+// contiguous range of the sorted arrays.  The block stages that range in LDS
+// in chunks (coalesced loads; each particle's (int)(p / dx) and half-cell
+// bits computed once), then every thread walks its own sub-range of the
+// chunk in order.
+constexpr int P2G_CAP_FLIP = 1024, P2G_CAP_APIC = 512;   // particles per staged chunk
+
+template <int APIC>
 __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
+    constexpr int CAP = APIC ? P2G_CAP_APIC : P2G_CAP_FLIP;
+    constexpr int NC = APIC ? 9 : 1;
+    __shared__ float s_pos[3][CAP], s_vel[3][CAP];
+    __shared__ int s_t[3][CAP];          // (int)(p_d / dx)
+    __shared__ uint8_t s_h[CAP];          // bit d: (int)(p_d / dx - 0.5) == (int)(p_d / dx)
+    __shared__ float s_c[NC][APIC ? CAP : 1];
     const GridArgs& G = A.G;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G.nc) return;
-    const int z = (int)(i % G.nz);
-    const int y = (int)((i / G.nz) % G.ny);
-    const int x = (int)(i / ((int64_t)G.nz * G.ny));
-    float fpos[3][3];
+    const int64_t c0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t i = c0 + threadIdx.x;
+    const bool valid = i < G.nc;
+    const int64_t ic = valid ? i : G.nc - 1;
+    const int z = (int)(ic % G.nz);
+    const int y = (int)((ic / G.nz) % G.ny);
+    const int x = (int)(ic / ((int64_t)G.nz * G.ny));
+    // face centres: component d of axis a's face is "aligned" (x + 1) for d == a, else the cell centre
+    float fal[3], fce[3];
+    {
+        float f[3][3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) face_pos(G, a, x, y, z, fpos[a]);
+        for (int a = 0; a < 3; a++) face_pos(G, a, x, y, z, f[a]);
+        fal[0] = f[0][0], fal[1] = f[1][1], fal[2] = f[2][2];
+        fce[0] = f[1][0], fce[1] = f[0][1], fce[2] = f[0][2];
+    }
+    const int cc[3] = {x, y, z};
     float swv[3] = {0.f, 0.f, 0.f}, sw[3] = {0.f, 0.f, 0.f};
+    const int64_t plane = (int64_t)G.ny * G.nz;
     for (int dx = -1; dx <= 1; dx++) {
-        int cx = x + dx;
-        if (cx < 0 || cx >= G.nx) continue;
         for (int dy = -1; dy <= 1; dy++) {
-            int cy = y + dy;
-            if (cy < 0 || cy >= G.ny) continue;
-            for (int dz = -1; dz <= 1; dz++) {
-                int cz = z + dz;
-                if (cz < 0 || cz >= G.nz) continue;
-                int64_t n = cidx(G, cx, cy, cz);
-                int b = A.start[n], e = A.start[n + 1];
-                for (int j = b; j < e; j++) {
-                    const float px = A.pos[0][j], py = A.pos[1][j], pz = A.pos[2][j];
+            const int64_t o = dx * plane + dy * G.nz;
+            // the block's particle range: cells c0 + o - 1 .. c0 + o + kBlock (clamped)
+            const int64_t lo = min(max(c0 + o - 1, (int64_t)0), G.nc);
+            const int64_t hi = min(max(c0 + o + kBlock + 1, (int64_t)0), G.nc);
+            const int u0 = A.start[lo], u1 = A.start[hi];
+            if (u0 >= u1) continue;   // block-uniform
+            // this thread's range: cells (x+dx, y+dy, z-1 .. z+1) inside the grid
+            int j0 = 0, j1 = 0;
+            const int cx = x + dx, cy = y + dy;
+            if (valid && cx >= 0 && cx < G.nx && cy >= 0 && cy < G.ny) {
+                const int64_t n = ic + o;
+                j0 = A.start[z > 0 ? n - 1 : n];
+                j1 = A.start[z < G.nz - 1 ? n + 2 : n + 1];
+            }
+            for (int base = u0; base < u1; base += CAP) {
+                const int cnt = min(CAP, u1 - base);
+                __syncthreads();   // the previous chunk is consumed
+                for (int k = threadIdx.x; k < cnt; k += kBlock) {
+                    const int j = base + k;
+                    uint8_t h = 0;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const float pv = A.pos[d][j];
+                        const double v = (double)pv * G.cdinv;
+                        const int t = (int)v;
+                        s_pos[d][k] = pv;
+                        s_vel[d][k] = A.vel[d][j];
+                        s_t[d][k] = t;
+                        if ((int)(v - 0.5) == t) h |= (uint8_t)(1u << d);
+                    }
+                    s_h[k] = h;
+                    if (APIC) {
+#pragma unroll
+                        for (int m = 0; m < NC; m++) s_c[m][k] = A.c[m][j];
+                    }
+                }
+                __syncthreads();
+                const int e = min(j1, base + cnt);
+                for (int j = max(j0, base); j < e; j++) {
+                    const int k = j - base;
+                    const float p[3] = {s_pos[0][k], s_pos[1][k], s_pos[2][k]};
+                    const uint32_t h = s_h[k];
+                    // face_base: b_a = t_a - 1, b_d = (int)(p_d/dx - 0.5) = t_d - 1 + h_d (d != a);
+                    // the face of this cell is in the stencil when cell - b is 0 or 1 on every axis
+                    int ua[3], uc[3];
+                    float tal[3], tce[3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        ua[d] = cc[d] - s_t[d][k] + 1;
+                        uc[d] = ua[d] - (int)((h >> d) & 1u);
+                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * G.invf);
+                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * G.invf);
+                    }
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
-                        int bs[3];
-                        face_base(G, px, py, pz, a, bs);
-                        unsigned ox = (unsigned)(x - bs[0]), oy = (unsigned)(y - bs[1]), oz = (unsigned)(z - bs[2]);
-                        if (ox > 1u || oy > 1u || oz > 1u) continue;
-                        float w = tent3(fpos[a], px, py, pz, G.invf);
-                        float va = A.vel[a][j];
-                        if (A.apic) {
+                        bool in = true;
+                        float tw[3];
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            const int u = d == a ? ua[d] : uc[d];
+                            in = in && (unsigned)u <= 1u;
+                            tw[d] = d == a ? tal[d] : tce[d];
+                        }
+                        if (!in) continue;
+                        const float w = tw[0] * tw[1] * tw[2];
+                        float va = s_vel[a][k];
+                        if (APIC) {
                             // (v_a + c[a] . (face - p)) * w, simulator.cpp:326-328
-                            float d0 = fpos[a][0] - px, d1 = fpos[a][1] - py, d2 = fpos[a][2] - pz;
-                            va = va + (A.c[3 * a][j] * d0 + A.c[3 * a + 1][j] * d1 + A.c[3 * a + 2][j] * d2);
+                            const float f0 = a == 0 ? fal[0] : fce[0], f1 = a == 1 ? fal[1] : fce[1],
+                                        f2 = a == 2 ? fal[2] : fce[2];
+                            const float d0 = f0 - p[0], d1 = f1 - p[1], d2 = f2 - p[2];
+                            va = va + (s_c[3 * a][k] * d0 + s_c[3 * a + 1][k] * d1 + s_c[3 * a + 2][k] * d2);
                         }
                         swv[a] += va * w;
                         sw[a] += w;
@@ -140,6 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
             }
         }
     }
+    if (!valid) return;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         A.uvw[a][i] = sw[a] > 1e-6f ? swv[a] / sw[a] : 0.0f;
@@ -171,7 +251,8 @@ static int p2g(fsim_ctx* ctx, int fuse_mark) {
     A.density = ctx->density;
     A.apic = ctx->prm.transfer == FSIM_APIC;
     A.fuse_mark = fuse_mark;
-    hipLaunchKernelGGL(k_p2g_gather, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
+    if (A.apic) hipLaunchKernelGGL(k_p2g_gather<1>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
+    else hipLaunchKernelGGL(k_p2g_gather<0>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }

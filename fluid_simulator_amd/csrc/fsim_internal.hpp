@@ -175,10 +175,13 @@ struct fsim_ctx {
 
     // pressure solve
     float *b = nullptr, *r = nullptr, *z = nullptr, *s = nullptr, *q = nullptr, *y = nullptr;
+    float* s2 = nullptr;              // second search-direction buffer (swapped with s)
     float* P32 = nullptr;
     double* P64 = nullptr;
     double* x = nullptr;
     double* partials = nullptr;       // block partial sums (FP64)
+    unsigned* red_count = nullptr;    // blocks finished in the running reduction (0 between launches)
+    hipEvent_t res_ev = nullptr;      // PCG: the iteration's scalars are on the host
     int partials_cap = 0;
     fsim::SolveScalars* scal = nullptr;   // device
     fsim::SolveScalars* scal_host = nullptr;  // pinned host mirror

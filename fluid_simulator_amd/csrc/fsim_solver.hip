@@ -28,6 +28,7 @@ namespace fsim {
 
 struct SolveArgs {
     int nx, ny, nz;
+    int off63;          // 63 mod nx: the plane shift of a z step across a 64-wide z-tile edge
     int64_t nc;
     float scale_f;      // dt / (rho dx^2)
     double scale;
@@ -44,10 +45,6 @@ __device__ inline int64_t rsidx(const SolveArgs& S, int x, int y, int z) { retur
 struct RNeighbours {
     int64_t px, mx, py, my, pz, mz;
 };
-__device__ inline int pmod(int a, int n) {
-    a %= n;
-    return a < 0 ? a + n : a;
-}
 __device__ inline RNeighbours rneighbours(const SolveArgs& S, int64_t i, int p, int y, int z) {
     const int64_t plane = (int64_t)S.ny * S.nz;
     const int64_t yz = (int64_t)y * S.nz + z;
@@ -59,8 +56,9 @@ __device__ inline RNeighbours rneighbours(const SolveArgs& S, int64_t i, int p, 
     n.mx = pp * plane + yz;
     n.py = i + S.nz;
     n.my = i - S.nz;
-    n.pz = (rz == WF_TZ - 1 ? pmod(p - (WF_TZ - 1), S.nx) : pn) * plane + yz + 1;
-    n.mz = (rz == 0 ? pmod(p + (WF_TZ - 1), S.nx) : pp) * plane + yz - 1;
+    const int o = S.off63;
+    n.pz = (rz == WF_TZ - 1 ? (p >= o ? p - o : p - o + S.nx) : pn) * plane + yz + 1;
+    n.mz = (rz == 0 ? (p + o >= S.nx ? p + o - S.nx : p + o) : pp) * plane + yz - 1;
     return n;
 }
 // rotated indices of the +a and -a neighbours of interior cell (x, y, z)
@@ -82,79 +80,114 @@ __device__ inline double wave_max(double v) {
     for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_down(v, d, 64));
     return v;
 }
-// Writes the block's sum to partials[blockIdx.x] and, for aux_mode 1 (max)
-// or 2 (sum), the reduced aux value to partials[gridDim.x + blockIdx.x].
+// ---- in-kernel finish of a reduction -------------------------------------------------
+// Each block reduces its values to one partial (plus an aux value: max or sum)
+// and stores it with device-scope stores; once they are acknowledged the block
+// takes a ticket from a device-scope counter.  The block with the last ticket
+// sums all partials in a fixed order (thread t: partials t, t + kBlock, ...,
+// then a pairwise tree), applies the reduction's scalar update and resets the
+// counter.  The sum's order does not depend on which block is last, so the
+// result is deterministic, and no separate finalize launch is needed.
 constexpr int AUX_NONE = 0, AUX_MAX = 1, AUX_SUM = 2;
-__device__ inline void block_reduce_store(double sum, double mx, double* partials, int aux_mode) {
-    __shared__ double lds_s[kBlock / 64], lds_m[kBlock / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool with_max = aux_mode != AUX_NONE;
-    sum = wave_sum(sum);
-    if (aux_mode == AUX_MAX) mx = wave_max(mx);
-    if (aux_mode == AUX_SUM) mx = wave_sum(mx);
-    if (lane == 0) {
-        lds_s[wave] = sum;
-        lds_m[wave] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0, m = 0;
-        for (int w = 0; w < kBlock / 64; w++) {
-            s += lds_s[w];
-            m = aux_mode == AUX_SUM ? m + lds_m[w] : fmax(m, lds_m[w]);
-        }
-        partials[blockIdx.x] = s;
-        if (with_max) partials[gridDim.x + blockIdx.x] = m;
-    }
-}
+constexpr unsigned kMaxParts = 2048;   // reduction grids have at most this many blocks, a multiple of 8
+enum FinOp { FIN_NONE = -1, FIN_BB = 0, FIN_SIGMA0 = 1, FIN_ALPHA = 2, FIN_RES = 3, FIN_BETA = 4, FIN_TRUE = 5 };
 
-enum FinOp { FIN_BB = 0, FIN_SIGMA0 = 1, FIN_ALPHA = 2, FIN_RES = 3, FIN_BETA = 4, FIN_TRUE = 5 };
+struct Reduce {
+    double* partials;   // 2 x gridDim.x
+    unsigned* count;    // zero between launches
+    SolveScalars* sc;
+    int op;             // FinOp
+};
 
-__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ partials, int nparts, int op,
-                                                     SolveScalars* __restrict__ sc) {
-    __shared__ double lds_s[kBlock], lds_m[kBlock];
-    double s = 0, m = 0;
-    const bool with_max = op == FIN_RES || op == FIN_TRUE;
-    const bool aux_sum = op == FIN_BB;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) {
-        s += partials[i];
-        if (with_max) m = fmax(m, partials[nparts + i]);
-        if (aux_sum) m += partials[nparts + i];
-    }
-    lds_s[threadIdx.x] = s;
-    lds_m[threadIdx.x] = m;
-    __syncthreads();
-    // pairwise tree over the threads' sums, fixed order
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) {
-            lds_s[threadIdx.x] += lds_s[threadIdx.x + off];
-            lds_m[threadIdx.x] = aux_sum ? lds_m[threadIdx.x] + lds_m[threadIdx.x + off]
-                                         : fmax(lds_m[threadIdx.x], lds_m[threadIdx.x + off]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    s = lds_s[0];
-    m = lds_m[0];
+__device__ inline void apply_fin(int op, double s, double m, SolveScalars* sc) {
     switch (op) {
     case FIN_BB: sc->bb = s; sc->nfluid = m; break;
-    case FIN_SIGMA0: sc->sigma = s; break;
-    case FIN_ALPHA: {
+    case FIN_SIGMA0: sc->zr = s; break;
+    case FIN_ALPHA: {   // sigma is the last z.r (initial, restart or beta step)
+        sc->sigma = sc->zr;
         sc->sq = s;
-        double alpha = sc->sigma / s;
+        const double alpha = sc->sigma / s;
         sc->alpha = alpha;
         if (alpha != alpha) sc->nan_flag |= 1;
         break;
     }
     case FIN_RES: sc->rr = s; sc->rmax = m; break;
     case FIN_BETA: {
-        sc->zr = s;
         if (sc->sigma != sc->sigma) sc->nan_flag |= 2;   // the reference tests the old sigma (:285-286)
+        sc->zr = s;
         sc->beta = s / sc->sigma;
-        sc->sigma = s;
         break;
     }
     case FIN_TRUE: sc->tr_ss = s; sc->tr_max = m; break;
+    default: break;
+    }
+}
+
+__device__ inline void put_dev(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double get_dev(const double* p) {
+    return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <int AUX>
+__device__ void block_finish(double sum, double aux, const Reduce& R) {
+    __shared__ double lds_s[kBlock], lds_m[kBlock];
+    __shared__ int last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    sum = wave_sum(sum);
+    if (AUX == AUX_MAX) aux = wave_max(aux);
+    if (AUX == AUX_SUM) aux = wave_sum(aux);
+    if (lane == 0) {
+        lds_s[wave] = sum;
+        lds_m[wave] = aux;
+    }
+    __syncthreads();
+    const unsigned nparts = gridDim.x;
+    if (threadIdx.x == 0) {
+        double s = 0, m = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            s += lds_s[w];
+            m = AUX == AUX_SUM ? m + lds_m[w] : fmax(m, lds_m[w]);
+        }
+        put_dev(&R.partials[blockIdx.x], s);
+        if (AUX != AUX_NONE) put_dev(&R.partials[nparts + blockIdx.x], m);
+        __builtin_amdgcn_s_waitcnt(0);   // the partials are acknowledged before the ticket
+        last = __hip_atomic_fetch_add(R.count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nparts - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    // all loads first (at most kMaxParts / kBlock per thread), then the sums in order
+    constexpr int kPer = kMaxParts / kBlock;
+    double vs[kPer], vm[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const unsigned i = threadIdx.x + k * kBlock;
+        vs[k] = i < nparts ? get_dev(&R.partials[i]) : 0.0;
+        vm[k] = AUX != AUX_NONE && i < nparts ? get_dev(&R.partials[nparts + i]) : 0.0;
+    }
+    double s = 0, m = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        s += vs[k];
+        m = AUX == AUX_SUM ? m + vm[k] : fmax(m, vm[k]);
+    }
+    lds_s[threadIdx.x] = s;
+    lds_m[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            lds_s[threadIdx.x] += lds_s[threadIdx.x + off];
+            lds_m[threadIdx.x] = AUX == AUX_SUM ? lds_m[threadIdx.x] + lds_m[threadIdx.x + off]
+                                                : fmax(lds_m[threadIdx.x], lds_m[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        apply_fin(R.op, lds_s[0], lds_m[0], R.sc);
+        __hip_atomic_store(R.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -171,7 +204,7 @@ struct RhsArgs {
     double inv_dx;
     double avg_p, pk;
     int pressure_enabled;
-    double* partials;
+    Reduce red;   // FIN_BB: ||b||^2 and the WATER cell count
 };
 
 __device__ inline bool is_fluid(const SolveArgs& S, const uint8_t* label, int x, int y, int z) {
@@ -232,7 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_nbinfo_rhs(RhsArgs A, int with_rhs) 
         A.nbinfo[ri] = info;
         if (with_rhs) A.b[ri] = bv;
     }
-    if (with_rhs) block_reduce_store(bsq, nfl, A.partials, AUX_SUM);
+    if (with_rhs) block_finish<AUX_SUM>(bsq, nfl, A.red);
 }
 
 // ---- MIC(0) factor, one i+j+k plane per launch (FP64) -----------------------------
@@ -331,104 +364,231 @@ __global__ void __launch_bounds__(kBlock) k_bwd_plane(PlaneArgs A, const float* 
     z[i] = (q[i] - tneg * P[i]) * P[i];
 }
 
-// ---- SpMV (applyAMatrix :165-198) with the s.q partial dot -------------------------
-__global__ void __launch_bounds__(kBlock) k_spmv_dot(SolveArgs S, const uint16_t* __restrict__ nbinfo,
-                                                     const float* __restrict__ s, float* __restrict__ q,
-                                                     double* __restrict__ partials) {
-    double d = 0.0;
-    const int nrows = S.nx * S.ny;
-    const float ms = -S.scale_f;
-    // one rotated row (plane p, row y) per block iteration, lanes along z
-    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
-        const int p = row / S.ny, y = row - p * S.ny;
-        for (int z = threadIdx.x; z < S.nz; z += blockDim.x) {
-            const int64_t i = (int64_t)row * S.nz + z;
-            const uint16_t info = nbinfo[i];
-            float val = 0.f;
-            if (info & NB_FLUID) {
-                const RNeighbours n = rneighbours(S, i, p, y, z);
-                const int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
-                val = (S.scale_f * (float)cnt) * s[i];
-                if (info & NB_PX) val += ms * s[n.px];
-                if (info & NB_PY) val += ms * s[n.py];
-                if (info & NB_PZ) val += ms * s[n.pz];
-                if (info & NB_MX) val += ms * s[n.mx];
-                if (info & NB_MY) val += ms * s[n.my];
-                if (info & NB_MZ) val += ms * s[n.mz];
-                d += (double)s[i] * (double)val;
-            }
-            q[i] = val;
-        }
-    }
-    block_reduce_store(d, 0.0, partials, AUX_NONE);
+// ---- SpMV (applyAMatrix :165-198) with the s.q dot ---------------------------------
+constexpr int SPMV_PLAIN = 0, SPMV_COPY = 1, SPMV_XPAY = 2;
+
+// Work split of the stencil kernels: one work item is a 256-cell chunk of an
+// interior rotated row (plane p, row y in 1..ny-2, z in 1..nz-2), handled by
+// one wave with 4 cells per lane (z = z0 + lane + 64k, coalesced for each k).
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx mod 8), so
+// the items are cut into 8 contiguous ranges, one per XCD: the +-x/+-y
+// neighbour rows a wave reads were loaded by waves of the same XCD (its L2).
+constexpr int kRowCells = 4 * 64;
+struct RowItems {
+    int nchunk, ny2;
+    int end;           // end of this XCD's item range (items fit in 32 bits: nc < 2^31)
+    int first, step;   // this wave's first item and stride
+};
+__device__ inline RowItems row_items(const SolveArgs& S) {
+    RowItems it;
+    it.nchunk = (S.nz - 2 + kRowCells - 1) / kRowCells;
+    it.ny2 = S.ny - 2;
+    const int total = S.nx * it.ny2 * it.nchunk;
+    const int per_xcd = (total + 7) / 8;
+    const int xcd = blockIdx.x & 7;
+    const int begin = xcd * per_xcd;
+    it.end = min(total, begin + per_xcd);
+    const int waves_per_block = blockDim.x / 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    it.first = begin + (int)(blockIdx.x >> 3) * waves_per_block + wave;
+    it.step = (int)(gridDim.x >> 3) * waves_per_block;
+    return it;
+}
+// (plane, row, first z) of item k
+__device__ inline void item_cells(const RowItems& it, int k, int& p, int& y, int& z0) {
+    const int c = k % it.nchunk;
+    const int rest = k / it.nchunk;
+    y = 1 + rest % it.ny2;
+    p = rest / it.ny2;
+    z0 = 1 + c * kRowCells;
 }
 
-// x += alpha s (FP64), r -= alpha q, with ||r||^2 and max|r| partials
-__global__ void __launch_bounds__(kBlock) k_update(int64_t n, const uint16_t* __restrict__ nbinfo, const float* __restrict__ s,
-                                                   const float* __restrict__ q, double* __restrict__ x, float* __restrict__ r,
-                                                   const SolveScalars* __restrict__ sc, double* __restrict__ partials) {
-    double ss = 0.0, mx = 0.0;
-    const double alpha = sc->alpha;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        if (!(nbinfo[i] & NB_FLUID)) continue;
-        x[i] += (double)s[i] * alpha;
-        float rn = r[i] + q[i] * (float)(-alpha);
-        r[i] = rn;
-        ss += (double)rn * (double)rn;
-        mx = fmax(mx, fabs((double)rn));
+// SPMV_PLAIN: q = A s.  SPMV_COPY: s' = z, q = A s'.  SPMV_XPAY: s' = s * beta + z
+// (multSelfAndAdd :229-242), q = A s'.  The new search direction s' goes to a
+// second buffer and is recomputed for the neighbours from z and the old s, so
+// no block reads a value another block has already replaced.  With
+// pending_x the previous iteration's x += alpha s (FP64) is applied here,
+// before FIN_ALPHA replaces alpha.  Only WATER cells are written: q is zeroed
+// before a solve and never read elsewhere.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __restrict__ nbinfo,
+                                                 const float* __restrict__ z, const float* __restrict__ s,
+                                                 float* __restrict__ s_new, float* __restrict__ q, double* __restrict__ x,
+                                                 int pending_x, Reduce red) {
+    double d = 0.0;
+    const float ms = -S.scale_f;
+    const float beta = MODE == SPMV_XPAY ? (float)red.sc->beta : 0.f;
+    const double alpha = red.sc->alpha;
+    auto dir = [&](int64_t j) {
+        if constexpr (MODE == SPMV_PLAIN) return s[j];
+        else if constexpr (MODE == SPMV_COPY) return z[j];
+        else return s[j] * beta + z[j];
+    };
+    const RowItems it = row_items(S);
+    const int lane = threadIdx.x & 63;
+    for (int k = it.first; k < it.end; k += it.step) {
+        int p, y, z0;
+        item_cells(it, k, p, y, z0);
+        const int64_t rowbase = ((int64_t)p * S.ny + y) * S.nz;
+        uint16_t info[4];
+        int zc[4];
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            zc[c] = z0 + lane + 64 * c;
+            info[c] = zc[c] < S.nz - 1 ? nbinfo[rowbase + zc[c]] : (uint16_t)0;
+            zc[c] = min(zc[c], S.nz - 2);   // lanes past the row gather an interior cell, unused
+            any |= (info[c] & NB_FLUID) != 0;
+        }
+        if (__builtin_amdgcn_ballot_w64(any) == 0) continue;   // no WATER cell in the wave's chunk
+        // all gathers of the 4 cells first (interior cells: every neighbour index is in range),
+        // then the sums with the reference's term order, by mask
+        float v[4][7];
+        double xo[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int64_t i = rowbase + zc[c];
+            const RNeighbours n = rneighbours(S, i, p, y, zc[c]);
+            v[c][0] = dir(i);
+            v[c][1] = dir(n.px);
+            v[c][2] = dir(n.py);
+            v[c][3] = dir(n.pz);
+            v[c][4] = dir(n.mx);
+            v[c][5] = dir(n.my);
+            v[c][6] = dir(n.mz);
+            xo[c] = pending_x ? x[i] : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint16_t f = info[c];
+            if (!(f & NB_FLUID)) continue;
+            const int64_t i = rowbase + zc[c];
+            const int cnt = (f >> NB_NONSOLID_SHIFT) & 7;
+            const float si = v[c][0];
+            float val = (S.scale_f * (float)cnt) * si;
+            val = (f & NB_PX) ? val + ms * v[c][1] : val;
+            val = (f & NB_PY) ? val + ms * v[c][2] : val;
+            val = (f & NB_PZ) ? val + ms * v[c][3] : val;
+            val = (f & NB_MX) ? val + ms * v[c][4] : val;
+            val = (f & NB_MY) ? val + ms * v[c][5] : val;
+            val = (f & NB_MZ) ? val + ms * v[c][6] : val;
+            d += (double)si * (double)val;
+            q[i] = val;
+            if (MODE != SPMV_PLAIN) s_new[i] = si;
+            if (pending_x) x[i] = xo[c] + (double)s[i] * alpha;
+        }
     }
-    block_reduce_store(ss, mx, partials, AUX_MAX);
+    block_finish<AUX_NONE>(d, 0.0, red);
+}
+
+// Streaming kernels: 4 consecutive cells per thread and iteration (16-byte
+// vector loads, all issued before use); the n % 4 tail cells go to the first
+// threads of block 0.
+__device__ inline uint16_t info_at(const uint2& w, int c) {
+    const uint32_t h = c < 2 ? w.x : w.y;
+    return (uint16_t)((c & 1) ? h >> 16 : h & 0xffffu);
+}
+__device__ inline float& lane_of(float4& v, int c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
+
+// r -= alpha q, with ||r||^2 and max|r|
+__global__ void __launch_bounds__(kBlock) k_update_r(int64_t n, const uint16_t* __restrict__ nbinfo,
+                                                     const float* __restrict__ q, float* __restrict__ r, Reduce red) {
+    double ss = 0.0, mx = 0.0;
+    const float malpha = (float)(-red.sc->alpha);
+    auto cell = [&](uint16_t info, float qv, float& rv) {
+        if (!(info & NB_FLUID)) return false;
+        rv = rv + qv * malpha;
+        ss += (double)rv * (double)rv;
+        mx = fmax(mx, fabs((double)rv));
+        return true;
+    };
+    const int64_t n4 = n >> 2, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
+        const uint2 iw = reinterpret_cast<const uint2*>(nbinfo)[v];
+        float4 qv = reinterpret_cast<const float4*>(q)[v];
+        float4 rv = reinterpret_cast<const float4*>(r)[v];
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < 4; c++) any |= cell(info_at(iw, c), lane_of(qv, c), lane_of(rv, c));
+        if (any) reinterpret_cast<float4*>(r)[v] = rv;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+        const int64_t i = 4 * n4 + threadIdx.x;
+        float rv = r[i];
+        if (cell(nbinfo[i], q[i], rv)) r[i] = rv;
+    }
+    block_finish<AUX_MAX>(ss, mx, red);
+}
+
+// x += alpha s (FP64) for WATER cells
+__global__ void __launch_bounds__(kBlock) k_update_x(int64_t n, const uint16_t* __restrict__ nbinfo,
+                                                     const float* __restrict__ s, double* __restrict__ x,
+                                                     const SolveScalars* __restrict__ sc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(nbinfo[i] & NB_FLUID)) return;
+    x[i] += (double)s[i] * sc->alpha;
 }
 
 __global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const uint16_t* __restrict__ nbinfo, const float* __restrict__ a,
-                                                const float* __restrict__ b, double* __restrict__ partials) {
+                                                const float* __restrict__ b, Reduce red) {
     double d = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    const int64_t n4 = n >> 2, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
+        const uint2 iw = reinterpret_cast<const uint2*>(nbinfo)[v];
+        float4 av = reinterpret_cast<const float4*>(a)[v];
+        float4 bv = reinterpret_cast<const float4*>(b)[v];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (info_at(iw, c) & NB_FLUID) d += (double)lane_of(av, c) * (double)lane_of(bv, c);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+        const int64_t i = 4 * n4 + threadIdx.x;
         if (nbinfo[i] & NB_FLUID) d += (double)a[i] * (double)b[i];
-    block_reduce_store(d, 0.0, partials, AUX_NONE);
-}
-
-// s = z + beta s (multSelfAndAdd :229-242)
-__global__ void __launch_bounds__(kBlock) k_xpay(int64_t n, const uint16_t* __restrict__ nbinfo, const float* __restrict__ z,
-                                                 float* __restrict__ s, const SolveScalars* __restrict__ sc) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !(nbinfo[i] & NB_FLUID)) return;
-    s[i] = s[i] * (float)sc->beta + z[i];
+    }
+    block_finish<AUX_NONE>(d, 0.0, red);
 }
 
 // FP64 true residual t = b - A x; optionally r = (float)t.
 __global__ void __launch_bounds__(kBlock) k_true_residual(SolveArgs S, const uint16_t* __restrict__ nbinfo,
                                                           const float* __restrict__ b, const double* __restrict__ x,
-                                                          float* __restrict__ r, int replace, double* __restrict__ partials) {
+                                                          float* __restrict__ r, int replace, Reduce red) {
 #pragma clang fp contract(off)
     double ss = 0.0, mx = 0.0;
-    const int nrows = S.nx * S.ny;
-    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
-        const int p = row / S.ny, y = row - p * S.ny;
-        for (int z = threadIdx.x; z < S.nz; z += blockDim.x) {
-            const int64_t i = (int64_t)row * S.nz + z;
-            const uint16_t info = nbinfo[i];
-            if (!(info & NB_FLUID)) continue;
-            const RNeighbours n = rneighbours(S, i, p, y, z);
-            const int cnt = (info >> NB_NONSOLID_SHIFT) & 7;
+    const RowItems it = row_items(S);
+    const int lane = threadIdx.x & 63;
+    for (int k = it.first; k < it.end; k += it.step) {
+        int p, y, z0;
+        item_cells(it, k, p, y, z0);
+        const int64_t rowbase = ((int64_t)p * S.ny + y) * S.nz;
+        uint16_t info[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int zc = z0 + lane + 64 * c;
+            info[c] = zc < S.nz - 1 ? nbinfo[rowbase + zc] : (uint16_t)0;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (!(info[c] & NB_FLUID)) continue;
+            const int zc = z0 + lane + 64 * c;
+            const int64_t i = rowbase + zc;
+            const RNeighbours n = rneighbours(S, i, p, y, zc);
+            const int cnt = (info[c] >> NB_NONSOLID_SHIFT) & 7;
             double diag = 0;
-            for (int k = 0; k < cnt; k++) diag += S.scale;
+            for (int j = 0; j < cnt; j++) diag += S.scale;
             double val = diag * x[i];
-            if (info & NB_PX) val += -S.scale * x[n.px];
-            if (info & NB_PY) val += -S.scale * x[n.py];
-            if (info & NB_PZ) val += -S.scale * x[n.pz];
-            if (info & NB_MX) val += -S.scale * x[n.mx];
-            if (info & NB_MY) val += -S.scale * x[n.my];
-            if (info & NB_MZ) val += -S.scale * x[n.mz];
+            if (info[c] & NB_PX) val += -S.scale * x[n.px];
+            if (info[c] & NB_PY) val += -S.scale * x[n.py];
+            if (info[c] & NB_PZ) val += -S.scale * x[n.pz];
+            if (info[c] & NB_MX) val += -S.scale * x[n.mx];
+            if (info[c] & NB_MY) val += -S.scale * x[n.my];
+            if (info[c] & NB_MZ) val += -S.scale * x[n.mz];
             const double t = (double)b[i] - val;
             ss += t * t;
             mx = fmax(mx, fabs(t));
             if (replace) r[i] = (float)t;
         }
     }
-    block_reduce_store(ss, mx, partials, AUX_MAX);
+    block_finish<AUX_MAX>(ss, mx, red);
 }
 
 // applyPressureToVelocities (:295-327), FP64 pressure
@@ -466,6 +626,7 @@ static SolveArgs solve_args(const fsim_ctx* ctx, double dt) {
     S.ny = ctx->g.ny;
     S.nz = ctx->g.nz;
     S.nc = ctx->g.nc;
+    S.off63 = (WF_TZ - 1) % S.nx;
     S.scale = dt / (ctx->prm.fluid_density * ctx->g.cd * ctx->g.cd);
     S.scale_f = (float)S.scale;
     return S;
@@ -485,21 +646,16 @@ static int check_wave_err(fsim_ctx* ctx) {
     return FSIM_OK;
 }
 
-// Reductions use at most this many blocks (grid-stride): enough waves to
-// keep HBM busy (8 blocks of 4 waves per CU), few enough partials for the
-// single-block finalize, which sums them in a fixed order.
-static unsigned red_blocks(int64_t n) { return std::min<unsigned>(nblocks(n), 8192u); }
+// Reduction grids: at most kMaxParts blocks (grid-stride; 8 blocks of 4 waves
+// per CU keep HBM busy), a multiple of 8 for the XCD-aware row split, and
+// the last block sums the partials in one pass.
+static unsigned red_blocks(int64_t n) { return std::min<unsigned>((nblocks(n) + 7) / 8 * 8, kMaxParts); }
+
+static Reduce reduce_to(fsim_ctx* ctx, int op) { return Reduce{ctx->partials, ctx->red_count, ctx->scal, op}; }
 
 static int read_scalars(fsim_ctx* ctx) {
     FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return FSIM_OK;
-}
-
-static int finalize(fsim_ctx* ctx, int nparts, int op) {
-    FSIM_PROF(ctx, KID_FINALIZE);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, nparts, op, ctx->scal);
-    FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
 
@@ -518,10 +674,9 @@ static int build_nbinfo(fsim_ctx* ctx, const SolveArgs& S, bool with_rhs) {
     A.avg_p = ctx->prm.average_pressure;
     A.pk = ctx->prm.pressure_k;
     A.pressure_enabled = ctx->prm.pressure_enabled;
-    A.partials = ctx->partials;
+    A.red = reduce_to(ctx, FIN_BB);
     hipLaunchKernelGGL(k_nbinfo_rhs, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, A, with_rhs ? 1 : 0);
     FSIM_HIP(ctx, hipGetLastError());
-    if (with_rhs) FSIM_TRY(finalize(ctx, (int)red_blocks(S.nc), FIN_BB));
     return FSIM_OK;
 }
 
@@ -570,13 +725,46 @@ static int mic_apply(fsim_ctx* ctx, const SolveArgs& S, const float* r, float* q
 }
 
 static int true_residual(fsim_ctx* ctx, const SolveArgs& S, bool replace) {
-    {
     FSIM_PROF(ctx, KID_TRUE_RES);
     hipLaunchKernelGGL(k_true_residual, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->b, ctx->x,
-                       ctx->r, replace ? 1 : 0, ctx->partials);
+                       ctx->r, replace ? 1 : 0, reduce_to(ctx, FIN_TRUE));
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+static int dot_zr(fsim_ctx* ctx, int op) {
+    FSIM_PROF(ctx, KID_DOT);
+    hipLaunchKernelGGL(k_dot, dim3(red_blocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, ctx->g.nc, ctx->nbinfo, ctx->z,
+                       ctx->r, reduce_to(ctx, op));
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// New search direction s (z, or z + beta s) into the second buffer, q = A s,
+// s.q -> alpha; with pending_x also the previous iteration's x += alpha s.
+static int direction_step(fsim_ctx* ctx, const SolveArgs& S, bool copy, bool pending_x) {
+    {
+        FSIM_PROF(ctx, KID_SPMV);
+        const dim3 grid(red_blocks(S.nc)), block(kBlock);
+        const Reduce red = reduce_to(ctx, FIN_ALPHA);
+        if (copy)
+            hipLaunchKernelGGL(k_spmv<SPMV_COPY>, grid, block, 0, ctx->stream, S, ctx->nbinfo, ctx->z, ctx->s, ctx->s2,
+                               ctx->q, ctx->x, pending_x ? 1 : 0, red);
+        else
+            hipLaunchKernelGGL(k_spmv<SPMV_XPAY>, grid, block, 0, ctx->stream, S, ctx->nbinfo, ctx->z, ctx->s, ctx->s2,
+                               ctx->q, ctx->x, pending_x ? 1 : 0, red);
     }
     FSIM_HIP(ctx, hipGetLastError());
-    return finalize(ctx, (int)red_blocks(S.nc), FIN_TRUE);
+    std::swap(ctx->s, ctx->s2);
+    return FSIM_OK;
+}
+
+static int update_x(fsim_ctx* ctx) {
+    FSIM_PROF(ctx, KID_UPDATE);
+    hipLaunchKernelGGL(k_update_x, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, ctx->g.nc, ctx->nbinfo, ctx->s,
+                       ctx->x, ctx->scal);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
 }
 
 static bool stop_met(const fsim_ctx* ctx, double ss, double mx, double bnorm) {
@@ -588,7 +776,6 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     const SolveArgs S = solve_args(ctx, dt);
     const int64_t nc = S.nc;
     const unsigned nb = nblocks(nc);
-    const unsigned nr = red_blocks(nc);
     ctx->last_dt = dt;
     fsim_step_stats& st = ctx->stats;
     st.early_exit = 0;
@@ -615,13 +802,10 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->b, ctx->r);
     FSIM_TRY(fill_f32(ctx, ctx->q, 0.0f, nc));
     FSIM_TRY(fill_f32(ctx, ctx->z, 0.0f, nc));
+    FSIM_TRY(fill_f32(ctx, ctx->s, 0.0f, nc));
+    FSIM_TRY(fill_f32(ctx, ctx->s2, 0.0f, nc));
     FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
-    hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->z, ctx->s);
-    {
-        FSIM_PROF(ctx, KID_DOT);
-        hipLaunchKernelGGL(k_dot, dim3(nr), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->r, ctx->partials);
-    }
-    FSIM_TRY(finalize(ctx, nr, FIN_SIGMA0));
+    FSIM_TRY(dot_zr(ctx, FIN_SIGMA0));
     const int max_it = ctx->prm.max_iterations;
     const int max_reliable = 64;
     // Reliable updates (restarted mixed-precision PCG): when the FP32
@@ -629,60 +813,61 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     // the last FP64 true residual, the true residual b - A x is recomputed in
     // FP64.  The solve stops if it meets the tolerance; otherwise r takes its
     // value and the search direction restarts from z = M^-1 r.
+    //
+    // One host round trip per iteration: the iteration's scalars are copied
+    // after the r update, and the next preconditioner application and z.r
+    // run while the host waits for them.  If the solve stops or restarts,
+    // those results are discarded.  x += alpha s of an iteration that goes
+    // on is applied by the next direction step.
     const double kDelta = 1e-3;
     double last_true = bnorm;
+    bool copy = true, pending_x = false;
     int it = 0;
     for (; it < max_it; it++) {
-        {
-            FSIM_PROF(ctx, KID_SPMV);
-            hipLaunchKernelGGL(k_spmv_dot, dim3(nr), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->s, ctx->q,
-                               ctx->partials);
-        }
-        FSIM_TRY(finalize(ctx, nr, FIN_ALPHA));
-        FSIM_TRY(read_scalars(ctx));
-        if (ctx->scal_host->nan_flag & 1) break;
+        FSIM_TRY(direction_step(ctx, S, copy, pending_x));
+        pending_x = false;
         {
             FSIM_PROF(ctx, KID_UPDATE);
-            hipLaunchKernelGGL(k_update, dim3(nr), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->s, ctx->q, ctx->x,
-                               ctx->r, ctx->scal, ctx->partials);
+            hipLaunchKernelGGL(k_update_r, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->q,
+                               ctx->r, reduce_to(ctx, FIN_RES));
         }
-        FSIM_TRY(finalize(ctx, nr, FIN_RES));
-        FSIM_TRY(read_scalars(ctx));
-        const double rr = ctx->scal_host->rr, rmax = ctx->scal_host->rmax;
+        FSIM_HIP(ctx, hipGetLastError());
+        FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
+        FSIM_HIP(ctx, hipEventRecord(ctx->res_ev, ctx->stream));
+        FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
+        FSIM_TRY(dot_zr(ctx, FIN_BETA));
+        FSIM_HIP(ctx, hipEventSynchronize(ctx->res_ev));
+        const SolveScalars h = *ctx->scal_host;
+        // the previous iteration's old-sigma test, then this iteration's alpha test
+        if (h.nan_flag & 2) {
+            it--;
+            break;
+        }
+        if (h.nan_flag & 1) break;
         bool restart = false;
-        if (stop_met(ctx, rr, rmax, bnorm) || std::sqrt(rr) < kDelta * last_true) {
+        if (stop_met(ctx, h.rr, h.rmax, bnorm) || std::sqrt(h.rr) < kDelta * last_true) {
+            FSIM_TRY(update_x(ctx));
             const bool replace = st.reliable_updates < max_reliable;
             FSIM_TRY(true_residual(ctx, S, replace));
             FSIM_TRY(read_scalars(ctx));
             if (stop_met(ctx, ctx->scal_host->tr_ss, ctx->scal_host->tr_max, bnorm)) break;
             if (!replace) {
-                if (stop_met(ctx, rr, rmax, bnorm)) break;
+                if (stop_met(ctx, h.rr, h.rmax, bnorm)) break;
             } else {
                 st.reliable_updates++;
                 last_true = std::sqrt(ctx->scal_host->tr_ss);
                 restart = true;
             }
-        }
-        FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
-        {
-            FSIM_PROF(ctx, KID_DOT);
-            hipLaunchKernelGGL(k_dot, dim3(nr), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->r, ctx->partials);
+        } else {
+            pending_x = true;
         }
         if (restart) {
-            FSIM_TRY(finalize(ctx, nr, FIN_SIGMA0));
-            hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->z, ctx->s);
-            FSIM_HIP(ctx, hipGetLastError());
-            continue;
+            FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
+            FSIM_TRY(dot_zr(ctx, FIN_SIGMA0));
         }
-        FSIM_TRY(finalize(ctx, nr, FIN_BETA));
-        FSIM_TRY(read_scalars(ctx));
-        if (ctx->scal_host->nan_flag & 2) break;
-        {
-            FSIM_PROF(ctx, KID_XPAY);
-            hipLaunchKernelGGL(k_xpay, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->z, ctx->s, ctx->scal);
-        }
-        FSIM_HIP(ctx, hipGetLastError());
+        copy = restart;
     }
+    if (pending_x) FSIM_TRY(update_x(ctx));
     st.pcg_iterations = it;
     const double sc = dt / (ctx->prm.fluid_density * ctx->g.cd);
     {
@@ -756,8 +941,9 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
     for (int64_t i = 0; i < nc; i++) hin[rmap[i]] = (float)in[i];
     FSIM_HIP(ctx, hipMemcpyAsync(ctx->r, hin.data(), sizeof(float) * nc, hipMemcpyHostToDevice, ctx->stream));
     if (op == FSIM_OP_APPLY_A) {
-        hipLaunchKernelGGL(k_spmv_dot, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->r, ctx->q,
-                           ctx->partials);
+        FSIM_TRY(fill_f32(ctx, ctx->q, 0.0f, nc));   // the SpMV writes WATER cells only
+        hipLaunchKernelGGL(k_spmv<SPMV_PLAIN>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, nullptr,
+                           ctx->r, nullptr, ctx->q, nullptr, 0, reduce_to(ctx, FIN_NONE));
         FSIM_HIP(ctx, hipGetLastError());
         FSIM_HIP(ctx, hipMemcpyAsync(hout.data(), ctx->q, sizeof(float) * nc, hipMemcpyDeviceToHost, ctx->stream));
     } else if (op == FSIM_OP_APPLY_MIC0) {
