@@ -481,14 +481,21 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
     block_finish<AUX_NONE>(d, 0.0, red);
 }
 
-// Streaming kernels: 4 consecutive cells per thread and iteration (16-byte
-// vector loads, all issued before use); the n % 4 tail cells go to the first
-// threads of block 0.
+// Streaming kernels: 4 consecutive cells per thread and group (16-byte vector
+// loads).  A thread takes kGroups groups at a time (grid stride): their
+// nbinfo words first, then the vector loads of the groups holding a WATER
+// cell only, so all-AIR/SOLID stretches of the grid cost their nbinfo bytes
+// alone.  The n % 4 tail cells go to the first threads of block 0.
+constexpr int kGroups = 4;
 __device__ inline uint16_t info_at(const uint2& w, int c) {
     const uint32_t h = c < 2 ? w.x : w.y;
     return (uint16_t)((c & 1) ? h >> 16 : h & 0xffffu);
 }
 __device__ inline float& lane_of(float4& v, int c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
+__device__ inline bool any_fluid(const uint2& w) {
+    constexpr uint32_t m = (uint32_t)NB_FLUID | ((uint32_t)NB_FLUID << 16);
+    return ((w.x | w.y) & m) != 0;
+}
 
 // r -= alpha q, with ||r||^2 and max|r|
 __global__ void __launch_bounds__(kBlock) k_update_r(int64_t n, const uint16_t* __restrict__ nbinfo,
@@ -503,14 +510,28 @@ __global__ void __launch_bounds__(kBlock) k_update_r(int64_t n, const uint16_t* 
         return true;
     };
     const int64_t n4 = n >> 2, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
-        const uint2 iw = reinterpret_cast<const uint2*>(nbinfo)[v];
-        float4 qv = reinterpret_cast<const float4*>(q)[v];
-        float4 rv = reinterpret_cast<const float4*>(r)[v];
-        bool any = false;
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < n4; v0 += kGroups * stride) {
+        uint2 iw[kGroups];
+        float4 qv[kGroups], rv[kGroups];
 #pragma unroll
-        for (int c = 0; c < 4; c++) any |= cell(info_at(iw, c), lane_of(qv, c), lane_of(rv, c));
-        if (any) reinterpret_cast<float4*>(r)[v] = rv;
+        for (int g = 0; g < kGroups; g++) {
+            const int64_t v = v0 + g * stride;
+            iw[g] = v < n4 ? reinterpret_cast<const uint2*>(nbinfo)[v] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int g = 0; g < kGroups; g++) {
+            if (!any_fluid(iw[g])) continue;
+            const int64_t v = v0 + g * stride;
+            qv[g] = reinterpret_cast<const float4*>(q)[v];
+            rv[g] = reinterpret_cast<const float4*>(r)[v];
+        }
+#pragma unroll
+        for (int g = 0; g < kGroups; g++) {
+            if (!any_fluid(iw[g])) continue;
+#pragma unroll
+            for (int c = 0; c < 4; c++) cell(info_at(iw[g], c), lane_of(qv[g], c), lane_of(rv[g], c));
+            reinterpret_cast<float4*>(r)[v0 + g * stride] = rv[g];
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
         const int64_t i = 4 * n4 + threadIdx.x;
@@ -533,13 +554,28 @@ __global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const uint16_t* __res
                                                 const float* __restrict__ b, Reduce red) {
     double d = 0.0;
     const int64_t n4 = n >> 2, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
-        const uint2 iw = reinterpret_cast<const uint2*>(nbinfo)[v];
-        float4 av = reinterpret_cast<const float4*>(a)[v];
-        float4 bv = reinterpret_cast<const float4*>(b)[v];
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < n4; v0 += kGroups * stride) {
+        uint2 iw[kGroups];
+        float4 av[kGroups], bv[kGroups];
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            if (info_at(iw, c) & NB_FLUID) d += (double)lane_of(av, c) * (double)lane_of(bv, c);
+        for (int g = 0; g < kGroups; g++) {
+            const int64_t v = v0 + g * stride;
+            iw[g] = v < n4 ? reinterpret_cast<const uint2*>(nbinfo)[v] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int g = 0; g < kGroups; g++) {
+            if (!any_fluid(iw[g])) continue;
+            const int64_t v = v0 + g * stride;
+            av[g] = reinterpret_cast<const float4*>(a)[v];
+            bv[g] = reinterpret_cast<const float4*>(b)[v];
+        }
+#pragma unroll
+        for (int g = 0; g < kGroups; g++) {
+            if (!any_fluid(iw[g])) continue;
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (info_at(iw[g], c) & NB_FLUID) d += (double)lane_of(av[g], c) * (double)lane_of(bv[g], c);
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
         const int64_t i = 4 * n4 + threadIdx.x;
