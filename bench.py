@@ -35,8 +35,9 @@ ALGO_BYTES = {
     "sweep_fwd": ("f", 13.0),      # R r 4, R P 4, R label 1, W q 4
     "sweep_bwd": ("f", 13.0),      # R q 4, R P 4, R label 1, W z 4
     "mic_factor": ("f", 5.0),      # R label 1, W P 4
-    "spmv": ("f", 9.0),            # R s 4, R label 1, W q 4
-    "update": ("f", 32.0),         # R s 4, R q 4, R/W x 16 (FP64), R/W r 8
+    "spmv": ("f", 25.0),           # direction step: R label 1, R z 4, R s 4, W s' 4, W q 4, R/W x 8
+    "update": ("f", 13.0),         # R label 1, R q 4, R/W r 8
+    "dot": ("f", 8.0),             # R z 4, R r 4
     "p2g": ("p+c", (24.0, 12.0)),  # R particle pos+vel 24 B/p, W u,v,w 12 B/cell
     "g2p": ("p+c", (36.0, 24.0)),  # R pos,vel W vel 36 B/p, R u,v,w,u2,v2,w2 24 B/cell
     "move": ("p", 36.0),           # R pos,vel W pos (vel in place) 36 B/p
@@ -81,6 +82,43 @@ def cpu_baseline(scene, gpu_iterations: float, threads: int, pcg_cap: int) -> di
                        f"per iteration to the GPU's {gpu_iterations:.0f} iterations")}
 
 
+COPY_BYTES = 1 << 30   # bytes read (and written) by each calibration copy
+PMC_TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                "sweep_fwd_256_pmc_traffic.md")
+
+
+def copy_calibration() -> None:
+    """Three plain 1 GiB device-to-device copies: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    of this copy kernel give the counter's reading for a known number of bytes."""
+    import torch
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "copies": 3}), flush=True)
+
+
+def pmc_traffic(kernel: str, size: int) -> float | None:
+    """HBM bytes per launch of `kernel` at `size`^3 from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if it covers this run."""
+    try:
+        lines = open(PMC_TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    kv = {}
+    for ln in lines:
+        if ":" in ln and not ln.startswith("#"):
+            k, v = ln.split(":", 1)
+            kv[k.strip().lstrip("- ").strip("`")] = v.strip().strip("`")
+    if kv.get("kernel") != kernel or kv.get("size") != str(size):
+        return None
+    try:
+        return float(kv["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,7 +130,13 @@ def main():
     ap.add_argument("--cpu-pcg-cap", type=int, default=12)
     ap.add_argument("--profile-steps", type=int, default=None,
                     help="steps timed per kernel with HIP events after the timed region (default: --steps)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device-to-device copy of known size (for scaling rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU decomposition is described in DESIGN.md, not implemented)", file=sys.stderr)
@@ -142,7 +186,8 @@ def main():
     if b is not None and avg_ms > 0:
         achieved = b / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": pmc_traffic(dominant, args.size),
                     "avg_launch_ms": round(avg_ms, 4), "launches": launches,
                     "bytes_per_launch": b}
     breakdown = {k: round(v[0] / psteps, 3) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0]) if v[1] > 0}

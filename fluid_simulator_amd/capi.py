@@ -66,7 +66,8 @@ class StepStats(C.Structure):
     _fields_ = [("timer_ms", C.c_double * 8), ("step_ms", C.c_double), ("pcg_iterations", C.c_int),
                 ("early_exit", C.c_int), ("fallback", C.c_int), ("reliable_updates", C.c_int),
                 ("residual_abs_inf", C.c_double), ("residual_rel_l2", C.c_double), ("b_norm_l2", C.c_double),
-                ("solve_ms", C.c_double), ("num_particles", C.c_int64), ("num_fluid_cells", C.c_int64)]
+                ("solve_ms", C.c_double), ("num_particles", C.c_int64), ("num_fluid_cells", C.c_int64),
+                ("learned_iterations", C.c_int), ("learned_stop", C.c_int)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "timer_ms"}
@@ -97,6 +98,13 @@ PROTOTYPES = {
     "fsim_get_grid": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "fsim_set_grid": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "fsim_solver_apply": (C.c_int, [_P, C.c_int, C.c_double, _P, _P]),
+    "fsim_learned_num_params": (C.c_int64, []),
+    "fsim_learned_init": (C.c_int, [_P, C.c_uint64]),
+    "fsim_learned_set_params": (C.c_int, [_P, _P]),
+    "fsim_learned_get_params": (C.c_int, [_P, _P]),
+    "fsim_learned_forward": (C.c_int, [_P, _P, _P, _P]),
+    "fsim_learned_grad": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(C.c_double)]),
+    "fsim_learned_train_batch": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_float, C.POINTER(C.c_double)]),
     "fsim_scene_dam_break": (C.c_int, [C.POINTER(GridDesc), C.c_int, _P, C.c_int, C.POINTER(Obstacle), C.c_uint64,
                                        C.c_int64, _P, C.POINTER(C.c_int64)]),
     "fsim_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
@@ -199,6 +207,10 @@ def scene_dam_break(grid: Grid, blocks, obstacles=(), seed: int = 0) -> np.ndarr
     if rc:
         raise FsimError(rc, "scene generation failed")
     return pos
+
+
+def learned_num_params() -> int:
+    return int(load().fsim_learned_num_params())
 
 
 class Simulation:
@@ -317,3 +329,49 @@ class Simulation:
         v = None if vec is None else np.ascontiguousarray(vec, dtype=np.float64).reshape(-1)
         self._check(self.lib.fsim_solver_apply(self._ctx, op, dt, _ptr(v), _ptr(out)))
         return out
+
+    # ---- learned preconditioner ----
+    def learned_init(self, seed: int = 0):
+        self._check(self.lib.fsim_learned_init(self._ctx, seed))
+
+    def learned_set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        if p.size != learned_num_params():
+            raise ValueError(f"expected {learned_num_params()} parameters, got {p.size}")
+        self._check(self.lib.fsim_learned_set_params(self._ctx, _ptr(p)))
+
+    def learned_get_params(self) -> np.ndarray:
+        p = np.empty(learned_num_params(), np.float32)
+        self._check(self.lib.fsim_learned_get_params(self._ctx, _ptr(p)))
+        return p
+
+    def _sample(self, b, label):
+        b = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+        lab = np.ascontiguousarray(label, dtype=np.uint8).reshape(-1)
+        if b.size != self.grid.nc or lab.size != self.grid.nc:
+            raise ValueError("b and label need one value per cell")
+        return b, lab
+
+    def learned_forward(self, b: np.ndarray, label: np.ndarray) -> np.ndarray:
+        b, lab = self._sample(b, label)
+        out = np.empty(self.grid.nc, np.float32)
+        self._check(self.lib.fsim_learned_forward(self._ctx, _ptr(b), _ptr(lab), _ptr(out)))
+        return out
+
+    def learned_grad(self, b: np.ndarray, label: np.ndarray, p: np.ndarray):
+        b, lab = self._sample(b, label)
+        pv = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        g = np.empty(learned_num_params(), np.float32)
+        loss = C.c_double(0.0)
+        self._check(self.lib.fsim_learned_grad(self._ctx, _ptr(b), _ptr(lab), _ptr(pv), _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def learned_train_batch(self, b: np.ndarray, label: np.ndarray, p: np.ndarray, lr: float) -> float:
+        n = len(b)
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(n, -1)
+        lab = np.ascontiguousarray(label, dtype=np.uint8).reshape(n, -1)
+        pv = np.ascontiguousarray(p, dtype=np.float64).reshape(n, -1)
+        loss = C.c_double(0.0)
+        self._check(self.lib.fsim_learned_train_batch(self._ctx, n, _ptr(bb), _ptr(lab), _ptr(pv), C.c_float(lr),
+                                                      C.byref(loss)))
+        return loss.value

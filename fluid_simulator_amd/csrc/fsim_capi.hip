@@ -30,7 +30,7 @@ int stage_p2g_fused(fsim_ctx* ctx);
 static const char* const kKernelNames[KID_COUNT] = {
     "spawn", "move", "bin_keys", "bin_scan", "bin_scatter", "bin_segsort", "bin_gather", "p2g", "mark",
     "obstacles", "borders", "post_p2g", "rhs", "mic_factor", "sweep_fwd", "sweep_bwd", "spmv", "update", "dot",
-    "xpay", "true_residual", "finalize", "pressure", "extrapolate", "g2p"};
+    "true_residual", "pressure", "extrapolate", "g2p", "learned"};
 
 int set_profiling(fsim_ctx* ctx, bool on) {
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -101,6 +101,7 @@ static int dalloc(fsim_ctx* ctx, T** p, int64_t n) {
 }
 
 static void free_all(fsim_ctx* ctx) {
+    learned_free(ctx);
     void* ptrs[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed, ctx->u, ctx->v, ctx->w,
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
                     ctx->scan_tmp, ctx->nbinfo, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
@@ -513,6 +514,42 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
 int fsim_solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out) {
     if (!ctx || !out || (op != FSIM_OP_FACTOR_MIC0 && !in) || !(dt > 0.0)) return FSIM_ERR_INVALID;
     return solver_apply(ctx, op, dt, in, out);
+}
+
+int64_t fsim_learned_num_params(void) { return learned_num_params(); }
+
+int fsim_learned_init(fsim_ctx* ctx, uint64_t seed) {
+    if (!ctx) return FSIM_ERR_INVALID;
+    return learned_init(ctx, seed);
+}
+
+int fsim_learned_set_params(fsim_ctx* ctx, const float* params) {
+    if (!ctx || !params) return FSIM_ERR_INVALID;
+    for (int64_t i = 0; i < learned_num_params(); i++)
+        if (!std::isfinite(params[i])) return set_err(ctx, FSIM_ERR_INVALID, "non-finite network parameter");
+    return learned_set_params(ctx, params);
+}
+
+int fsim_learned_get_params(fsim_ctx* ctx, float* params) {
+    if (!ctx || !params) return FSIM_ERR_INVALID;
+    return learned_get_params(ctx, params);
+}
+
+int fsim_learned_forward(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out) {
+    if (!ctx || !b || !label || !out) return FSIM_ERR_INVALID;
+    return learned_forward_host(ctx, b, label, out);
+}
+
+int fsim_learned_grad(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad,
+                      double* loss) {
+    if (!ctx || !b || !label || !p || !grad || !loss) return FSIM_ERR_INVALID;
+    return learned_grad_host(ctx, b, label, p, grad, loss);
+}
+
+int fsim_learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
+                             double* mean_loss) {
+    if (!ctx || !b || !label || !p || !mean_loss || n <= 0 || !(lr > 0.f)) return FSIM_ERR_INVALID;
+    return learned_train_batch(ctx, n, b, label, p, lr, mean_loss);
 }
 
 }  // extern "C"

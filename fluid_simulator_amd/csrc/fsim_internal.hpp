@@ -55,7 +55,10 @@ constexpr int WF_NW = FSIM_WF_NW;
 constexpr int WF_TY = WF_R * WF_NW;
 constexpr int WF_TZ = 64;
 constexpr int WF_THREADS = (WF_NW + 2) * WF_TZ;
-constexpr int WF_PREFETCH = 8;   // steps of per-cell inputs loaded ahead (and the loop unroll)
+#ifndef FSIM_WF_PREFETCH
+#define FSIM_WF_PREFETCH 8
+#endif
+constexpr int WF_PREFETCH = FSIM_WF_PREFETCH;   // steps of per-cell inputs loaded ahead (and the loop unroll)
 // Steps of one tile: all rows of compute wave w are at x-index t - w - tz at
 // step t, so the last cell (wave WF_NW-1, lane 63, x-index nxi-1) is at step
 // nxi + WF_NW + WF_TZ - 3; rounded up to whole blocks of WF_PREFETCH steps
@@ -121,6 +124,7 @@ struct SolveScalars {
     double rr, rmax, bb;             // recursive residual ||r||^2, max|r|, ||b||^2
     double tr_ss, tr_max;            // true residual ||b-Ax||^2, max|b-Ax|
     double zr;                       // z.r
+    double zq;                       // z.q (flexible PCG)
     double nfluid;                   // number of WATER cells of the solve
     int nan_flag;
     int pad;
@@ -130,8 +134,8 @@ struct SolveScalars {
 enum KernelId {
     KID_SPAWN = 0, KID_MOVE, KID_BIN_KEYS, KID_BIN_SCAN, KID_BIN_SCATTER, KID_BIN_SEGSORT, KID_BIN_GATHER,
     KID_P2G, KID_MARK, KID_OBSTACLES, KID_BORDERS, KID_POST_P2G, KID_RHS, KID_MIC_FACTOR, KID_SWEEP_FWD,
-    KID_SWEEP_BWD, KID_SPMV, KID_UPDATE, KID_DOT, KID_XPAY, KID_TRUE_RES, KID_FINALIZE, KID_PRESSURE,
-    KID_EXTRAPOLATE, KID_G2P, KID_COUNT
+    KID_SWEEP_BWD, KID_SPMV, KID_UPDATE, KID_DOT, KID_TRUE_RES, KID_PRESSURE, KID_EXTRAPOLATE, KID_G2P,
+    KID_LEARNED, KID_COUNT
 };
 
 struct Particles {
@@ -144,6 +148,10 @@ struct Particles {
 };
 
 }  // namespace fsim
+
+namespace fsim {
+struct LearnedNet;
+}
 
 struct fsim_ctx {
     int device = 0;
@@ -182,6 +190,8 @@ struct fsim_ctx {
     double* partials = nullptr;       // block partial sums (FP64)
     unsigned* red_count = nullptr;    // blocks finished in the running reduction (0 between launches)
     hipEvent_t res_ev = nullptr;      // PCG: the iteration's scalars are on the host
+    int last_mic_iterations = 0;      // MIC(0)-PCG iterations of the last solve that used it
+    fsim::LearnedNet* learned = nullptr;   // learned preconditioner (network parameters and buffers)
     int partials_cap = 0;
     fsim::SolveScalars* scal = nullptr;   // device
     fsim::SolveScalars* scal_host = nullptr;  // pinned host mirror
@@ -279,5 +289,18 @@ int stage_solve(fsim_ctx* ctx, double dt);
 int wave_launch(fsim_ctx* ctx, int kind, const float* in, float* out, double scale);
 int wave_init(fsim_ctx* ctx);
 int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out);
+
+// ---- learned preconditioner (fsim_learned.hip) ------------------------------
+int64_t learned_num_params();
+int learned_init(fsim_ctx* ctx, uint64_t seed);
+int learned_set_params(fsim_ctx* ctx, const float* params);
+int learned_get_params(fsim_ctx* ctx, float* params);
+bool learned_ready(const fsim_ctx* ctx);
+int learned_apply(fsim_ctx* ctx, const float* r, float* z);   // z = M(r), solver layout
+int learned_forward_host(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out);
+int learned_grad_host(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad, double* loss);
+int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
+                        double* mean_loss);
+void learned_free(fsim_ctx* ctx);
 
 }  // namespace fsim

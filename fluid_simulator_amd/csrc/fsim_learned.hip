@@ -1,0 +1,870 @@
+// fsim_learned.hip — learned preconditioner: a 4-level 3D conv encoder-decoder
+// (no counterpart in the reference) used as M^-1 in flexible PCG.
+//
+// Network (fully convolutional, so weights trained on small grids apply to
+// any grid whose padded size is a multiple of 8):
+//   in  = (r / s, WATER mask, 0 x 6)                         level 0, 8 ch
+//   e0  = relu(conv3(in, 8 -> 16))                           level 0
+//   e1  = relu(conv3(pool(e0), 16 -> 32))                    level 1
+//   e2  = relu(conv3(pool(e1), 32 -> 64))                    level 2
+//   e3  = relu(conv3(pool(e2), 64 -> 128))                   level 3
+//   d2  = relu(conv3(cat(up(e3), e2), 192 -> 64))            level 2
+//   d1  = relu(conv3(cat(up(d2), e1), 96 -> 32))             level 1
+//   d0  = relu(conv3(cat(up(d1), e0), 48 -> 16))             level 0
+//   out = conv1(d0, 16 -> 1);  z = WATER ? out * s : 0
+// with s = max|r|, pool = 2x2x2 average, up = nearest, conv3 = 3x3x3, stride 1,
+// zero padding.  Activations and weights are BF16, every sum is accumulated in
+// FP32 (MFMA), biases and the output are FP32.
+//
+// Layout: channels last, voxel v = (x * H + y) * W + z at each level.  Conv
+// weights are stored per output channel, Wt[co][t * Cin + ci] with tap
+// t = ((dx+1) * 3 + (dy+1)) * 3 + (dz+1), so a K run of 8 is 8 channels of one tap.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fsim_internal.hpp"
+
+namespace fsim {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---- architecture --------------------------------------------------------------------
+struct LayerDef {
+    int cin, cout, level;
+};
+constexpr int kNetLayers = 7;   // 3x3x3 convolutions; the 1x1 output conv follows
+constexpr LayerDef kLayers[kNetLayers] = {
+    {8, 16, 0}, {16, 32, 1}, {32, 64, 2}, {64, 128, 3}, {192, 64, 2}, {96, 32, 1}, {48, 16, 0}};
+constexpr int kInCh = 8, kOutIn = 16;
+
+__host__ __device__ inline int64_t conv_weights(int l) { return (int64_t)kLayers[l].cout * 27 * kLayers[l].cin; }
+// offsets of each layer's weights and biases in the flat parameter vector
+static void param_offsets(int64_t* woff, int64_t* boff, int64_t* out_w, int64_t* out_b, int64_t* total) {
+    int64_t o = 0;
+    for (int l = 0; l < kNetLayers; l++) {
+        woff[l] = o;
+        o += conv_weights(l);
+        boff[l] = o;
+        o += kLayers[l].cout;
+    }
+    *out_w = o;
+    o += kOutIn;
+    *out_b = o;
+    o += 1;
+    *total = o;
+}
+int64_t learned_num_params() {
+    int64_t w[kNetLayers], b[kNetLayers], ow, ob, t;
+    param_offsets(w, b, &ow, &ob, &t);
+    return t;
+}
+
+// ---- implicit-GEMM 3x3x3 convolution on MFMA ------------------------------------------
+// Y[v][co] = bias[co] + sum_{t, ci} X[v + off(t)][ci] * Wt[co][t * Cin + ci]
+// One wave: 16 consecutive voxels x all Cout channels (NT tiles of 16), K in
+// chunks of 32 with v_mfma_f32_16x16x32_bf16.  Operand lanes: A (16 x 32):
+// lane l holds row l % 16, k = 8 (l / 16) .. + 7; B (32 x 16): lane l holds
+// column l % 16, the same k run; D (16 x 16): lane l holds rows
+// 4 (l / 16) .. + 3 of column l % 16.  Out-of-grid taps read zeros.
+struct ConvArgs {
+    const __bf16* x;      // [V][cin]
+    const __bf16* w;      // [cout][27 * cin]
+    const float* bias;    // [cout] or null
+    void* y;              // [V][cout], bf16 or float
+    int D, H, W;          // grid of this level
+    int cin, cout;
+    int relu;
+};
+
+template <int NT, bool OUT_F32>
+__global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t V = (int64_t)a.D * a.H * a.W;
+    const int64_t m0 = ((int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 16;
+    if (m0 >= V) return;   // wave-uniform
+    const int64_t vr = m0 + (lane & 15);   // this lane's A row (voxel)
+    const bool rin = vr < V;
+    const int z = rin ? (int)(vr % a.W) : 0;
+    const int y = rin ? (int)((vr / a.W) % a.H) : 0;
+    const int x = rin ? (int)(vr / ((int64_t)a.W * a.H)) : 0;
+    const int kq = lane >> 4;   // which 8-run of the 32-wide K chunk
+    const int ktot = 27 * a.cin;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < ktot; kc += 32) {
+        const int k0 = kc + 8 * kq;
+        bf16x8 av;
+        bool kin = k0 < ktot;
+        if (kin && rin) {
+            const int t = k0 / a.cin, ci = k0 - t * a.cin;
+            const int xx = x + t / 9 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t % 3 - 1;
+            if (xx >= 0 && xx < a.D && yy >= 0 && yy < a.H && zz >= 0 && zz < a.W) {
+                const int64_t nv = ((int64_t)xx * a.H + yy) * a.W + zz;
+                av = *reinterpret_cast<const bf16x8*>(a.x + nv * a.cin + ci);
+            } else {
+                av = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            }
+        } else {
+            av = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const int co = 16 * j + (lane & 15);
+            bf16x8 bv = kin ? *reinterpret_cast<const bf16x8*>(a.w + (int64_t)co * ktot + k0)
+                            : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const int co = 16 * j + (lane & 15);
+        const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t v = m0 + 4 * kq + i;
+            if (v >= V) continue;
+            float o = acc[j][i] + b;
+            if (a.relu) o = fmaxf(o, 0.f);
+            if (OUT_F32) static_cast<float*>(a.y)[v * a.cout + co] = o;
+            else static_cast<__bf16*>(a.y)[v * a.cout + co] = (__bf16)o;
+        }
+    }
+}
+
+template <bool OUT_F32>
+static void launch_conv3(hipStream_t st, const ConvArgs& a) {
+    const int64_t V = (int64_t)a.D * a.H * a.W;
+    const dim3 grid((unsigned)((V + 63) / 64)), block(256);
+    switch (a.cout / 16) {
+    case 1: hipLaunchKernelGGL((k_conv3<1, OUT_F32>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_conv3<2, OUT_F32>), grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_conv3<3, OUT_F32>), grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_conv3<4, OUT_F32>), grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_conv3<6, OUT_F32>), grid, block, 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_conv3<8, OUT_F32>), grid, block, 0, st, a); break;
+    case 12: hipLaunchKernelGGL((k_conv3<12, OUT_F32>), grid, block, 0, st, a); break;
+    default: break;   // channel counts are fixed by kLayers
+    }
+}
+
+// ---- elementwise pieces -------------------------------------------------------------
+struct Dims {
+    int D, H, W;
+    __host__ __device__ int64_t V() const { return (int64_t)D * H * W; }
+};
+
+// 2x2x2 average pool (FP32 sum of the 8 BF16 values, / 8, rounded to BF16)
+__global__ void k_pool2(const __bf16* __restrict__ in, __bf16* __restrict__ out, Dims fine, int C) {
+    const Dims cs{fine.D / 2, fine.H / 2, fine.W / 2};
+    const int64_t n = cs.V() * C;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    const int z = (int)(v % cs.W), y = (int)((v / cs.W) % cs.H), x = (int)(v / ((int64_t)cs.W * cs.H));
+    float s = 0.f;
+    for (int d = 0; d < 8; d++) {
+        const int64_t fv = ((int64_t)(2 * x + (d >> 2)) * fine.H + 2 * y + ((d >> 1) & 1)) * fine.W + 2 * z + (d & 1);
+        s += (float)in[fv * C + c];
+    }
+    out[i] = (__bf16)(s * 0.125f);
+}
+
+// cat(up(coarse), skip) at the fine level: channels [0, Cu) from the coarse parent, [Cu, Cu+Cs) from skip
+__global__ void k_upcat(const __bf16* __restrict__ coarse, const __bf16* __restrict__ skip, __bf16* __restrict__ out,
+                        Dims fine, int Cu, int Cs) {
+    const int C = Cu + Cs;
+    const int64_t n = fine.V() * C;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    if (c >= Cu) {
+        out[i] = skip[v * Cs + (c - Cu)];
+        return;
+    }
+    const int z = (int)(v % fine.W), y = (int)((v / fine.W) % fine.H), x = (int)(v / ((int64_t)fine.W * fine.H));
+    const int64_t pv = ((int64_t)(x / 2) * (fine.H / 2) + y / 2) * (fine.W / 2) + z / 2;
+    out[i] = coarse[pv * Cu + c];
+}
+
+// network input from the solver's r (rotated layout) or from a flat host-order vector
+__global__ void k_net_input(const float* __restrict__ r, const uint16_t* __restrict__ nbinfo, int rotated,
+                            const uint8_t* __restrict__ label, const float* __restrict__ scale, __bf16* __restrict__ x0,
+                            Dims nd, int nx, int ny, int nz) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nd.V()) return;
+    const int z = (int)(v % nd.W), y = (int)((v / nd.W) % nd.H), x = (int)(v / ((int64_t)nd.W * nd.H));
+    float val = 0.f, fl = 0.f;
+    if (x < nx && y < ny && z < nz) {
+        if (rotated) {
+            const int64_t i = ridx(x, y, z, nx, ny, nz);
+            const bool f = (nbinfo[i] & NB_FLUID) != 0;
+            fl = f ? 1.f : 0.f;
+            val = f ? r[i] : 0.f;
+        } else {
+            const int64_t i = ((int64_t)x * ny + y) * nz + z;
+            const bool f = label[i] == FSIM_WATER;
+            fl = f ? 1.f : 0.f;
+            val = f ? r[i] : 0.f;
+        }
+    }
+    const float s = *scale;
+    const float inv = s > 0.f ? 1.f / s : 0.f;
+    bf16x8 o = {(__bf16)(val * inv), (__bf16)fl, 0, 0, 0, 0, 0, 0};
+    *reinterpret_cast<bf16x8*>(x0 + v * kInCh) = o;
+}
+
+// s = max |r| over WATER cells (non-negative floats order like their bit patterns)
+__global__ void k_absmax(const float* __restrict__ r, const uint16_t* __restrict__ nbinfo, int64_t n,
+                         unsigned* __restrict__ out) {
+    unsigned m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (nbinfo[i] & NB_FLUID) m = max(m, __float_as_uint(fabsf(r[i])));
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (unsigned)__shfl_down((int)m, d, 64));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+// out[v] = b + sum_c d0[v][c] * w[c]  (FP32)
+__global__ void k_conv1_fwd(const __bf16* __restrict__ d0, const float* __restrict__ w, const float* __restrict__ b,
+                            float* __restrict__ out, int64_t V) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    float s = *b;
+    for (int c = 0; c < kOutIn; c++) s += (float)d0[v * kOutIn + c] * (float)(__bf16)w[c];
+    out[v] = s;
+}
+
+// z (solver layout) = WATER ? out * s : 0
+__global__ void k_net_output(const float* __restrict__ out, const uint16_t* __restrict__ nbinfo,
+                             const float* __restrict__ scale, float* __restrict__ z, Dims nd, int nx, int ny, int nz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nc = (int64_t)nx * ny * nz;
+    if (i >= nc) return;
+    // i is a flat cell index; write its rotated slot
+    const int zc = (int)(i % nz), yc = (int)((i / nz) % ny), xc = (int)(i / ((int64_t)nz * ny));
+    const int64_t ri = ridx(xc, yc, zc, nx, ny, nz);
+    const int64_t v = ((int64_t)xc * nd.H + yc) * nd.W + zc;
+    z[ri] = (nbinfo[ri] & NB_FLUID) ? out[v] * *scale : 0.f;
+}
+
+// ---- backward pieces (training) ----------------------------------------------------------
+// g = dy * (y > 0), as FP32 and BF16 (the BF16 copy feeds the MFMA data-gradient convolution)
+__global__ void k_relu_bwd(const float* __restrict__ dy, const __bf16* __restrict__ y, float* __restrict__ g,
+                           __bf16* __restrict__ g16, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = (float)y[i] > 0.f ? dy[i] : 0.f;
+    g[i] = v;
+    g16[i] = (__bf16)v;
+}
+
+// encoder gradient: g = (dskip + dpool[parent] / 8) * (y > 0)
+__global__ void k_pool_bwd_relu(const float* __restrict__ dskip, const float* __restrict__ dpool,
+                                const __bf16* __restrict__ y, float* __restrict__ g, __bf16* __restrict__ g16, Dims fine,
+                                int C) {
+    const int64_t n = fine.V() * C;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    const int z = (int)(v % fine.W), yy = (int)((v / fine.W) % fine.H), x = (int)(v / ((int64_t)fine.W * fine.H));
+    const int64_t pv = ((int64_t)(x / 2) * (fine.H / 2) + yy / 2) * (fine.W / 2) + z / 2;
+    float d = dpool ? dpool[pv * C + c] * 0.125f : 0.f;
+    if (dskip) d += dskip[i];
+    const float o = (float)y[i] > 0.f ? d : 0.f;
+    g[i] = o;
+    g16[i] = (__bf16)o;
+}
+
+// gradient of cat(up(coarse), skip): dcoarse = sum over the 8 children, dskip = slice
+__global__ void k_upcat_bwd_up(const float* __restrict__ dcat, float* __restrict__ dcoarse, Dims fine, int Cu, int Cs) {
+    const Dims cs{fine.D / 2, fine.H / 2, fine.W / 2};
+    const int C = Cu + Cs;
+    const int64_t n = cs.V() * Cu;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % Cu);
+    const int64_t v = i / Cu;
+    const int z = (int)(v % cs.W), y = (int)((v / cs.W) % cs.H), x = (int)(v / ((int64_t)cs.W * cs.H));
+    float s = 0.f;
+    for (int d = 0; d < 8; d++) {
+        const int64_t fv = ((int64_t)(2 * x + (d >> 2)) * fine.H + 2 * y + ((d >> 1) & 1)) * fine.W + 2 * z + (d & 1);
+        s += dcat[fv * C + c];
+    }
+    dcoarse[i] = s;
+}
+__global__ void k_upcat_bwd_skip(const float* __restrict__ dcat, float* __restrict__ dskip, Dims fine, int Cu, int Cs) {
+    const int64_t n = fine.V() * Cs;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)(i % Cs);
+    const int64_t v = i / Cs;
+    dskip[i] = dcat[v * (Cu + Cs) + Cu + c];
+}
+
+// data-gradient weights: Wb[ci][t' * Cout + co] = W[co][(26 - t') * Cin + ci]
+__global__ void k_flip_weights(const __bf16* __restrict__ w, __bf16* __restrict__ wb, int cin, int cout) {
+    const int64_t n = (int64_t)cout * 27 * cin;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int co = (int)(i % cout);
+    const int64_t r = i / cout;
+    const int tp = (int)(r % 27), ci = (int)(r / 27);
+    wb[i] = w[(int64_t)co * 27 * cin + (int64_t)(26 - tp) * cin + ci];
+}
+
+// Weight gradient of one 3x3x3 convolution, FP32: dW[co][t * Cin + ci] = sum_v g[v][co] x[v + off(t)][ci].
+// Block (tap t, voxel chunk): stages kChunk voxels of g and of the shifted x in LDS, each thread
+// accumulates its (co, ci) outputs; partials[chunk][co][t * Cin + ci] are summed in chunk order later.
+constexpr int kWgChunk = 32;
+__global__ void __launch_bounds__(256) k_conv3_wgrad(const float* __restrict__ g, const __bf16* __restrict__ x,
+                                                     float* __restrict__ partials, Dims d, int cin, int cout,
+                                                     int64_t vox_per_block) {
+    extern __shared__ float lds[];
+    float* sg = lds;                        // [kWgChunk][cout]
+    float* sx = lds + kWgChunk * cout;      // [kWgChunk][cin]
+    const int t = blockIdx.y;
+    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    const int64_t V = d.V();
+    const int64_t v_begin = (int64_t)blockIdx.x * vox_per_block;
+    const int64_t v_end = min(V, v_begin + vox_per_block);
+    const int nout = cout * cin;
+    constexpr int kMaxPer = (64 * 192 + 255) / 256;   // largest cout * cin of the network (layer 4)
+    float acc[kMaxPer];
+#pragma unroll
+    for (int k = 0; k < kMaxPer; k++) acc[k] = 0.f;
+    for (int64_t v0 = v_begin; v0 < v_end; v0 += kWgChunk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kWgChunk * cout; e += blockDim.x) {
+            const int j = e / cout, co = e - j * cout;
+            const int64_t v = v0 + j;
+            sg[e] = v < v_end ? g[v * cout + co] : 0.f;
+        }
+        for (int e = threadIdx.x; e < kWgChunk * cin; e += blockDim.x) {
+            const int j = e / cin, ci = e - j * cin;
+            const int64_t v = v0 + j;
+            float val = 0.f;
+            if (v < v_end) {
+                const int z = (int)(v % d.W), y = (int)((v / d.W) % d.H), xx = (int)(v / ((int64_t)d.W * d.H));
+                const int X = xx + dx, Y = y + dy, Z = z + dz;
+                if (X >= 0 && X < d.D && Y >= 0 && Y < d.H && Z >= 0 && Z < d.W)
+                    val = (float)x[(((int64_t)X * d.H + Y) * d.W + Z) * cin + ci];
+            }
+            sx[e] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kMaxPer; k++) {
+            const int o = threadIdx.x + k * 256;
+            if (o < nout) {
+                const int co = o / cin, ci = o - co * cin;
+                float s = acc[k];
+                for (int j = 0; j < kWgChunk; j++) s += sg[j * cout + co] * sx[j * cin + ci];
+                acc[k] = s;
+            }
+        }
+    }
+    float* out = partials + (int64_t)blockIdx.x * cout * 27 * cin;
+#pragma unroll
+    for (int k = 0; k < kMaxPer; k++) {
+        const int o = threadIdx.x + k * 256;
+        if (o < nout) {
+            const int co = o / cin, ci = o - co * cin;
+            out[(int64_t)co * 27 * cin + (int64_t)t * cin + ci] = acc[k];
+        }
+    }
+}
+
+// sum of nparts partial vectors of length n, in partial order
+__global__ void k_sum_parts(const float* __restrict__ parts, int nparts, int64_t n, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int p = 0; p < nparts; p++) s += parts[(int64_t)p * n + i];
+    out[i] = s;
+}
+
+// bias gradient: db[c] = sum_v g[v][c] (one block per channel, fixed order)
+__global__ void __launch_bounds__(256) k_bias_grad(const float* __restrict__ g, int64_t V, int C, float* __restrict__ db) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int64_t v = threadIdx.x; v < V; v += blockDim.x) s += g[v * C + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
+}
+
+// output layer backward: dd0[v][c] = dout[v] * w[c];  dw[c] = sum_v dout[v] d0[v][c];  db = sum dout
+__global__ void k_conv1_bwd_data(const float* __restrict__ dout, const float* __restrict__ w, float* __restrict__ dd0,
+                                 int64_t V) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V * kOutIn) return;
+    dd0[i] = dout[i / kOutIn] * (float)(__bf16)w[i % kOutIn];
+}
+__global__ void __launch_bounds__(256) k_conv1_bwd_weight(const float* __restrict__ dout, const __bf16* __restrict__ d0,
+                                                          int64_t V, float* __restrict__ dw) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;   // 0..15 weights, 16 = bias
+    float s = 0.f;
+    for (int64_t v = threadIdx.x; v < V; v += blockDim.x) s += dout[v] * (c < kOutIn ? (float)d0[v * kOutIn + c] : 1.f);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dw[c] = red[0];
+}
+
+// masked MSE over WATER cells: loss = sum (out - t)^2 / n_water, dout = 2 (out - t) / n_water
+__global__ void k_mse(const float* __restrict__ out, const float* __restrict__ target, const uint8_t* __restrict__ fluid,
+                      int64_t V, float inv_n, float* __restrict__ dout, float* __restrict__ loss_parts) {
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const float e = fluid[v] ? out[v] - target[v] : 0.f;
+        dout[v] = 2.f * e * inv_n;
+        s += e * e;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0] * inv_n;
+}
+
+// Adam on FP32 master weights; BF16 copy of the conv weights for the MFMA kernels
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
+                       int64_t n, float lr, float b1, float b2, float eps, float c1, float c2, float gscale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+__global__ void k_to_bf16(const float* __restrict__ a, __bf16* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (__bf16)a[i];
+}
+
+// ---- host side ----------------------------------------------------------------------------
+struct LearnedNet {
+    int64_t woff[kNetLayers], boff[kNetLayers], out_w = 0, out_b = 0, nparams = 0;
+    bool has_weights = false;
+    int adam_t = 0;
+    Dims lv[4];
+    float *w32 = nullptr, *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr, *grad_acc = nullptr;
+    __bf16* w16 = nullptr;
+    // forward
+    __bf16 *x0 = nullptr, *e[4] = {}, *pl[3] = {}, *cat[3] = {}, *dd[3] = {};   // cat/dd index = level
+    float *out = nullptr, *scale = nullptr;
+    unsigned* scale_bits = nullptr;
+    // training
+    bool train_ready = false;
+    float *in_flat = nullptr, *target = nullptr, *dout = nullptr, *dd0 = nullptr, *loss_parts = nullptr;
+    uint8_t *label_flat = nullptr, *fluid = nullptr;
+    float* g32[kNetLayers] = {};
+    __bf16* g16[kNetLayers] = {};
+    float *dcat[3] = {}, *dcoarse[3] = {}, *dskip[3] = {}, *dpool[3] = {};
+    __bf16* wflip = nullptr;
+    float* wparts = nullptr;
+    int64_t wparts_cap = 0;
+    std::vector<void*> allocs;
+};
+
+template <typename T>
+static int nalloc(fsim_ctx* ctx, LearnedNet* N, T** p, int64_t n) {
+    if (hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess)
+        return set_err(ctx, FSIM_ERR_OOM, "learned preconditioner: device allocation failed");
+    N->allocs.push_back(*p);
+    return FSIM_OK;
+}
+
+void learned_free(fsim_ctx* ctx) {
+    LearnedNet* N = ctx->learned;
+    if (!N) return;
+    for (void* p : N->allocs) (void)hipFree(p);
+    delete N;
+    ctx->learned = nullptr;
+}
+
+static unsigned gb(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+static int learned_alloc(fsim_ctx* ctx) {
+    if (ctx->learned) return FSIM_OK;
+    LearnedNet* N = new LearnedNet();
+    ctx->learned = N;
+    param_offsets(N->woff, N->boff, &N->out_w, &N->out_b, &N->nparams);
+    auto r8 = [](int n) { return (n + 7) / 8 * 8; };
+    N->lv[0] = Dims{r8(ctx->g.nx), r8(ctx->g.ny), r8(ctx->g.nz)};
+    for (int l = 1; l < 4; l++) N->lv[l] = Dims{N->lv[l - 1].D / 2, N->lv[l - 1].H / 2, N->lv[l - 1].W / 2};
+    const int64_t V0 = N->lv[0].V(), V1 = N->lv[1].V(), V2 = N->lv[2].V(), V3 = N->lv[3].V();
+    int rc;
+    if ((rc = nalloc(ctx, N, &N->w32, N->nparams)) || (rc = nalloc(ctx, N, &N->w16, N->nparams)) ||
+        (rc = nalloc(ctx, N, &N->x0, V0 * kInCh)) || (rc = nalloc(ctx, N, &N->e[0], V0 * 16)) ||
+        (rc = nalloc(ctx, N, &N->e[1], V1 * 32)) || (rc = nalloc(ctx, N, &N->e[2], V2 * 64)) ||
+        (rc = nalloc(ctx, N, &N->e[3], V3 * 128)) || (rc = nalloc(ctx, N, &N->pl[0], V1 * 16)) ||
+        (rc = nalloc(ctx, N, &N->pl[1], V2 * 32)) || (rc = nalloc(ctx, N, &N->pl[2], V3 * 64)) ||
+        (rc = nalloc(ctx, N, &N->cat[2], V2 * 192)) || (rc = nalloc(ctx, N, &N->cat[1], V1 * 96)) ||
+        (rc = nalloc(ctx, N, &N->cat[0], V0 * 48)) || (rc = nalloc(ctx, N, &N->dd[2], V2 * 64)) ||
+        (rc = nalloc(ctx, N, &N->dd[1], V1 * 32)) || (rc = nalloc(ctx, N, &N->dd[0], V0 * 16)) ||
+        (rc = nalloc(ctx, N, &N->out, V0)) || (rc = nalloc(ctx, N, &N->scale, 1)) ||
+        (rc = nalloc(ctx, N, &N->scale_bits, 1)))
+        return rc;
+    return FSIM_OK;
+}
+
+static int learned_alloc_train(fsim_ctx* ctx) {
+    LearnedNet* N = ctx->learned;
+    if (N->train_ready) return FSIM_OK;
+    const int64_t V0 = N->lv[0].V(), V1 = N->lv[1].V(), V2 = N->lv[2].V(), V3 = N->lv[3].V();
+    const int64_t Vl[4] = {V0, V1, V2, V3};
+    int rc;
+    if ((rc = nalloc(ctx, N, &N->adam_m, N->nparams)) || (rc = nalloc(ctx, N, &N->adam_v, N->nparams)) ||
+        (rc = nalloc(ctx, N, &N->grad, N->nparams)) || (rc = nalloc(ctx, N, &N->grad_acc, N->nparams)) ||
+        (rc = nalloc(ctx, N, &N->in_flat, ctx->g.nc)) || (rc = nalloc(ctx, N, &N->label_flat, ctx->g.nc)) ||
+        (rc = nalloc(ctx, N, &N->target, V0)) || (rc = nalloc(ctx, N, &N->fluid, V0)) ||
+        (rc = nalloc(ctx, N, &N->dout, V0)) || (rc = nalloc(ctx, N, &N->dd0, V0 * 16)) ||
+        (rc = nalloc(ctx, N, &N->loss_parts, 1024)) || (rc = nalloc(ctx, N, &N->wflip, 64 * 27 * 192)))
+        return rc;
+    for (int l = 0; l < kNetLayers; l++) {
+        const int64_t n = Vl[kLayers[l].level] * kLayers[l].cout;
+        if ((rc = nalloc(ctx, N, &N->g32[l], n)) || (rc = nalloc(ctx, N, &N->g16[l], n))) return rc;
+    }
+    // concat data gradients at levels 2, 1, 0 and their splits
+    const int cu[3] = {32, 64, 128}, cs[3] = {16, 32, 64};   // index = level
+    for (int l = 0; l < 3; l++) {
+        if ((rc = nalloc(ctx, N, &N->dcat[l], Vl[l] * (cu[l] + cs[l]))) ||
+            (rc = nalloc(ctx, N, &N->dcoarse[l], Vl[l + 1] * cu[l])) || (rc = nalloc(ctx, N, &N->dskip[l], Vl[l] * cs[l])))
+            return rc;
+    }
+    // data gradients of the pooled inputs: dpool[l] at level l + 1 with the channels of e[l]
+    const int pc[3] = {16, 32, 64};
+    for (int l = 0; l < 3; l++)
+        if ((rc = nalloc(ctx, N, &N->dpool[l], Vl[l + 1] * pc[l]))) return rc;
+    N->wparts_cap = 64LL * 64 * 27 * 192;
+    if ((rc = nalloc(ctx, N, &N->wparts, N->wparts_cap))) return rc;
+    FSIM_HIP(ctx, hipMemsetAsync(N->adam_m, 0, sizeof(float) * N->nparams, ctx->stream));
+    FSIM_HIP(ctx, hipMemsetAsync(N->adam_v, 0, sizeof(float) * N->nparams, ctx->stream));
+    N->adam_t = 0;
+    N->train_ready = true;
+    return FSIM_OK;
+}
+
+static ConvArgs conv_args(const LearnedNet* N, int l, const __bf16* x, void* y) {
+    const Dims d = N->lv[kLayers[l].level];
+    return ConvArgs{x, N->w16 + N->woff[l], N->w32 + N->boff[l], y, d.D, d.H, d.W, kLayers[l].cin, kLayers[l].cout, 1};
+}
+
+// forward from x0 (already packed) to out
+static int net_forward(fsim_ctx* ctx) {
+    LearnedNet* N = ctx->learned;
+    hipStream_t st = ctx->stream;
+    const Dims* L = N->lv;
+    launch_conv3<false>(st, conv_args(N, 0, N->x0, N->e[0]));
+    for (int l = 1; l < 4; l++) {
+        hipLaunchKernelGGL(k_pool2, dim3(gb(L[l].V() * kLayers[l].cin)), dim3(256), 0, st, N->e[l - 1], N->pl[l - 1],
+                           L[l - 1], kLayers[l].cin);
+        launch_conv3<false>(st, conv_args(N, l, N->pl[l - 1], N->e[l]));
+    }
+    // decoder: layer 4 at level 2 (up e3 + e2), 5 at level 1 (up d2 + e1), 6 at level 0 (up d1 + e0)
+    const __bf16* coarse = N->e[3];
+    for (int l = 4; l < kNetLayers; l++) {
+        const int lev = kLayers[l].level;
+        const int Cs = kLayers[l].cin / 3, Cu = kLayers[l].cin - Cs;
+        hipLaunchKernelGGL(k_upcat, dim3(gb(L[lev].V() * kLayers[l].cin)), dim3(256), 0, st, coarse, N->e[lev], N->cat[lev],
+                           L[lev], Cu, Cs);
+        launch_conv3<false>(st, conv_args(N, l, N->cat[lev], N->dd[lev]));
+        coarse = N->dd[lev];
+    }
+    hipLaunchKernelGGL(k_conv1_fwd, dim3(gb(L[0].V())), dim3(256), 0, st, N->dd[0], N->w32 + N->out_w, N->w32 + N->out_b,
+                       N->out, L[0].V());
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+static int sync_bf16(fsim_ctx* ctx) {
+    LearnedNet* N = ctx->learned;
+    hipLaunchKernelGGL(k_to_bf16, dim3(gb(N->nparams)), dim3(256), 0, ctx->stream, N->w32, N->w16, N->nparams);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+bool learned_ready(const fsim_ctx* ctx) { return ctx->learned && ctx->learned->has_weights; }
+
+int learned_set_params(fsim_ctx* ctx, const float* params) {
+    FSIM_TRY(learned_alloc(ctx));
+    LearnedNet* N = ctx->learned;
+    FSIM_HIP(ctx, hipMemcpyAsync(N->w32, params, sizeof(float) * N->nparams, hipMemcpyHostToDevice, ctx->stream));
+    FSIM_TRY(sync_bf16(ctx));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    N->has_weights = true;
+    N->adam_t = 0;
+    if (N->train_ready) {
+        FSIM_HIP(ctx, hipMemsetAsync(N->adam_m, 0, sizeof(float) * N->nparams, ctx->stream));
+        FSIM_HIP(ctx, hipMemsetAsync(N->adam_v, 0, sizeof(float) * N->nparams, ctx->stream));
+    }
+    return FSIM_OK;
+}
+
+int learned_get_params(fsim_ctx* ctx, float* params) {
+    if (!learned_ready(ctx)) return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner has no parameters");
+    LearnedNet* N = ctx->learned;
+    FSIM_HIP(ctx, hipMemcpyAsync(params, N->w32, sizeof(float) * N->nparams, hipMemcpyDeviceToHost, ctx->stream));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FSIM_OK;
+}
+
+// He-normal initial parameters from a counter RNG (splitmix64 + Box-Muller), biases 0
+int learned_init(fsim_ctx* ctx, uint64_t seed) {
+    int64_t woff[kNetLayers], boff[kNetLayers], ow, ob, total;
+    param_offsets(woff, boff, &ow, &ob, &total);
+    std::vector<float> p((size_t)total, 0.f);
+    uint64_t ctr = 0;
+    auto next = [&]() {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (++ctr);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);   // (0, 1)
+    };
+    auto normal = [&]() { return std::sqrt(-2.0 * std::log(next())) * std::cos(6.283185307179586 * next()); };
+    for (int l = 0; l < kNetLayers; l++) {
+        const double sd = std::sqrt(2.0 / (27.0 * kLayers[l].cin));
+        for (int64_t i = 0; i < conv_weights(l); i++) p[(size_t)(woff[l] + i)] = (float)(sd * normal());
+    }
+    for (int c = 0; c < kOutIn; c++) p[(size_t)(ow + c)] = (float)(std::sqrt(1.0 / kOutIn) * normal());
+    return learned_set_params(ctx, p.data());
+}
+
+// z = M(r) in the solver layout (r, z: rotated FP32 vectors of the context)
+int learned_apply(fsim_ctx* ctx, const float* r, float* z) {
+    LearnedNet* N = ctx->learned;
+    hipStream_t st = ctx->stream;
+    const int64_t nc = ctx->g.nc;
+    FSIM_HIP(ctx, hipMemsetAsync(N->scale_bits, 0, sizeof(unsigned), st));
+    hipLaunchKernelGGL(k_absmax, dim3(std::min<unsigned>(gb(nc), 2048u)), dim3(256), 0, st, r, ctx->nbinfo, nc,
+                       N->scale_bits);
+    static_assert(sizeof(unsigned) == sizeof(float), "scale bits");
+    const float* sc = reinterpret_cast<const float*>(N->scale_bits);   // the max as a float
+    hipLaunchKernelGGL(k_net_input, dim3(gb(N->lv[0].V())), dim3(256), 0, st, r, ctx->nbinfo, 1, nullptr, sc, N->x0,
+                       N->lv[0], ctx->g.nx, ctx->g.ny, ctx->g.nz);
+    FSIM_TRY(net_forward(ctx));
+    hipLaunchKernelGGL(k_net_output, dim3(gb(nc)), dim3(256), 0, st, N->out, ctx->nbinfo, sc, z, N->lv[0], ctx->g.nx,
+                       ctx->g.ny, ctx->g.nz);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// Uploads one sample given in flat grid order: input (b / s, WATER mask), and for training the
+// target p / s and the WATER mask in network order.  s = max |b| over WATER cells.
+static int upload_sample(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* s_out) {
+    LearnedNet* N = ctx->learned;
+    const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
+    const int64_t nc = ctx->g.nc;
+    float s = 0.f;
+    for (int64_t i = 0; i < nc; i++)
+        if (label[i] == FSIM_WATER) s = std::max(s, std::fabs(b[i]));
+    *s_out = s;
+    FSIM_HIP(ctx, hipMemcpyAsync(N->in_flat, b, sizeof(float) * nc, hipMemcpyHostToDevice, ctx->stream));
+    FSIM_HIP(ctx, hipMemcpyAsync(N->label_flat, label, nc, hipMemcpyHostToDevice, ctx->stream));
+    FSIM_HIP(ctx, hipMemcpyAsync(N->scale, &s, sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_net_input, dim3(gb(N->lv[0].V())), dim3(256), 0, ctx->stream, N->in_flat, nullptr, 0,
+                       N->label_flat, N->scale, N->x0, N->lv[0], nx, ny, nz);
+    FSIM_HIP(ctx, hipGetLastError());
+    if (p) {
+        const Dims d = N->lv[0];
+        std::vector<float> t((size_t)d.V(), 0.f);
+        std::vector<uint8_t> f((size_t)d.V(), 0);
+        const float inv = s > 0.f ? 1.f / s : 0.f;
+        for (int x = 0; x < nx; x++)
+            for (int y = 0; y < ny; y++)
+                for (int z = 0; z < nz; z++) {
+                    const int64_t i = ((int64_t)x * ny + y) * nz + z, v = ((int64_t)x * d.H + y) * d.W + z;
+                    if (label[i] != FSIM_WATER) continue;
+                    f[(size_t)v] = 1;
+                    t[(size_t)v] = (float)p[i] * inv;
+                }
+        FSIM_HIP(ctx, hipMemcpyAsync(N->target, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, ctx->stream));
+        FSIM_HIP(ctx, hipMemcpyAsync(N->fluid, f.data(), f.size(), hipMemcpyHostToDevice, ctx->stream));
+        // the host vectors go out of scope: wait for the copies
+        FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return FSIM_OK;
+}
+
+// network output for one flat-order sample (raw network output, flat grid order)
+int learned_forward_host(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out) {
+    if (!learned_ready(ctx)) return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner has no parameters");
+    LearnedNet* N = ctx->learned;
+    FSIM_TRY(learned_alloc_train(ctx));
+    float s;
+    FSIM_TRY(upload_sample(ctx, b, label, nullptr, &s));
+    FSIM_TRY(net_forward(ctx));
+    const Dims d = N->lv[0];
+    std::vector<float> o((size_t)d.V());
+    FSIM_HIP(ctx, hipMemcpyAsync(o.data(), N->out, sizeof(float) * o.size(), hipMemcpyDeviceToHost, ctx->stream));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++)
+                out[((int64_t)x * ny + y) * nz + z] = o[(size_t)(((int64_t)x * d.H + y) * d.W + z)];
+    return FSIM_OK;
+}
+
+static int wgrad(fsim_ctx* ctx, int l, const float* g, const __bf16* x) {
+    LearnedNet* N = ctx->learned;
+    const Dims d = N->lv[kLayers[l].level];
+    const int cin = kLayers[l].cin, cout = kLayers[l].cout;
+    const int64_t V = d.V();
+    const int64_t n = (int64_t)cout * 27 * cin;
+    int nblk = (int)std::min<int64_t>(64, (V + 1023) / 1024);
+    while ((int64_t)nblk * n > N->wparts_cap) nblk--;
+    const int64_t per = (V + nblk - 1) / nblk;
+    const size_t lds = sizeof(float) * kWgChunk * (cin + cout);
+    hipLaunchKernelGGL(k_conv3_wgrad, dim3(nblk, 27), dim3(256), lds, ctx->stream, g, x, N->wparts, d, cin, cout, per);
+    hipLaunchKernelGGL(k_sum_parts, dim3(gb(n)), dim3(256), 0, ctx->stream, N->wparts, nblk, n, N->grad + N->woff[l]);
+    hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, ctx->stream, g, V, cout, N->grad + N->boff[l]);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// data gradient of layer l's input: dx = conv3(g16, flipped weights)
+static int dgrad(fsim_ctx* ctx, int l, const __bf16* g16, float* dx) {
+    LearnedNet* N = ctx->learned;
+    const Dims d = N->lv[kLayers[l].level];
+    const int cin = kLayers[l].cin, cout = kLayers[l].cout;
+    hipLaunchKernelGGL(k_flip_weights, dim3(gb((int64_t)cout * 27 * cin)), dim3(256), 0, ctx->stream, N->w16 + N->woff[l],
+                       N->wflip, cin, cout);
+    ConvArgs a{g16, N->wflip, nullptr, dx, d.D, d.H, d.W, cout, cin, 0};
+    launch_conv3<true>(ctx->stream, a);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// loss and gradient of one uploaded sample into N->grad; returns the loss
+static int net_backward(fsim_ctx* ctx, double* loss) {
+    LearnedNet* N = ctx->learned;
+    hipStream_t st = ctx->stream;
+    const Dims* L = N->lv;
+    const int64_t V0 = L[0].V();
+    int64_t nfl = 0;
+    {
+        std::vector<uint8_t> f((size_t)V0);
+        FSIM_HIP(ctx, hipMemcpyAsync(f.data(), N->fluid, f.size(), hipMemcpyDeviceToHost, st));
+        FSIM_HIP(ctx, hipStreamSynchronize(st));
+        for (uint8_t c : f) nfl += c;
+    }
+    const float inv_n = nfl > 0 ? 1.f / (float)nfl : 0.f;
+    const int nlb = 256;
+    hipLaunchKernelGGL(k_mse, dim3(nlb), dim3(256), 0, st, N->out, N->target, N->fluid, V0, inv_n, N->dout, N->loss_parts);
+    // output layer
+    hipLaunchKernelGGL(k_conv1_bwd_data, dim3(gb(V0 * kOutIn)), dim3(256), 0, st, N->dout, N->w32 + N->out_w, N->dd0, V0);
+    hipLaunchKernelGGL(k_conv1_bwd_weight, dim3(kOutIn + 1), dim3(256), 0, st, N->dout, N->dd[0], V0, N->grad + N->out_w);
+    // decoder: layer 6 (level 0), 5 (level 1), 4 (level 2)
+    const float* dy = N->dd0;
+    for (int l = 6; l >= 4; l--) {
+        const int lev = kLayers[l].level;
+        const int64_t n = L[lev].V() * kLayers[l].cout;
+        hipLaunchKernelGGL(k_relu_bwd, dim3(gb(n)), dim3(256), 0, st, dy, N->dd[lev], N->g32[l], N->g16[l], n);
+        FSIM_TRY(wgrad(ctx, l, N->g32[l], N->cat[lev]));
+        FSIM_TRY(dgrad(ctx, l, N->g16[l], N->dcat[lev]));
+        const int Cs = kLayers[l].cin / 3, Cu = kLayers[l].cin - Cs;
+        hipLaunchKernelGGL(k_upcat_bwd_up, dim3(gb(L[lev + 1].V() * Cu)), dim3(256), 0, st, N->dcat[lev], N->dcoarse[lev],
+                           L[lev], Cu, Cs);
+        hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(gb(L[lev].V() * Cs)), dim3(256), 0, st, N->dcat[lev], N->dskip[lev],
+                           L[lev], Cu, Cs);
+        dy = N->dcoarse[lev];   // gradient of the coarse decoder output (d1, d2) or of e3
+    }
+    // bottleneck, layer 3 at level 3: its output e3's gradient is dcoarse[2]
+    {
+        const int64_t n = L[3].V() * 128;
+        hipLaunchKernelGGL(k_relu_bwd, dim3(gb(n)), dim3(256), 0, st, N->dcoarse[2], N->e[3], N->g32[3], N->g16[3], n);
+        FSIM_TRY(wgrad(ctx, 3, N->g32[3], N->pl[2]));
+        FSIM_TRY(dgrad(ctx, 3, N->g16[3], N->dpool[2]));
+    }
+    // encoder layers 2, 1, 0: gradient of e[l] = skip gradient + pooled-input gradient of layer l + 1
+    for (int l = 2; l >= 0; l--) {
+        const int C = kLayers[l].cout;
+        hipLaunchKernelGGL(k_pool_bwd_relu, dim3(gb(L[l].V() * C)), dim3(256), 0, st, N->dskip[l], N->dpool[l], N->e[l],
+                           N->g32[l], N->g16[l], L[l], C);
+        FSIM_TRY(wgrad(ctx, l, N->g32[l], l == 0 ? N->x0 : N->pl[l - 1]));
+        if (l > 0) FSIM_TRY(dgrad(ctx, l, N->g16[l], N->dpool[l - 1]));
+    }
+    FSIM_HIP(ctx, hipGetLastError());
+    std::vector<float> lp(nlb);
+    FSIM_HIP(ctx, hipMemcpyAsync(lp.data(), N->loss_parts, sizeof(float) * nlb, hipMemcpyDeviceToHost, st));
+    FSIM_HIP(ctx, hipStreamSynchronize(st));
+    double s = 0;
+    for (float v : lp) s += v;
+    *loss = s;
+    return FSIM_OK;
+}
+
+int learned_grad_host(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad, double* loss) {
+    if (!learned_ready(ctx)) return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner has no parameters");
+    FSIM_TRY(learned_alloc_train(ctx));
+    LearnedNet* N = ctx->learned;
+    float s;
+    FSIM_TRY(upload_sample(ctx, b, label, p, &s));
+    FSIM_TRY(net_forward(ctx));
+    FSIM_TRY(net_backward(ctx, loss));
+    FSIM_HIP(ctx, hipMemcpyAsync(grad, N->grad, sizeof(float) * N->nparams, hipMemcpyDeviceToHost, ctx->stream));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FSIM_OK;
+}
+
+__global__ void k_axpy_f(float* __restrict__ y, const float* __restrict__ x, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+// One Adam step on the mean gradient of n samples (flat grid order, stride nc each).
+int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
+                        double* mean_loss) {
+    if (!learned_ready(ctx)) return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner has no parameters");
+    if (n <= 0) return set_err(ctx, FSIM_ERR_INVALID, "empty training batch");
+    FSIM_TRY(learned_alloc_train(ctx));
+    LearnedNet* N = ctx->learned;
+    const int64_t nc = ctx->g.nc;
+    FSIM_HIP(ctx, hipMemsetAsync(N->grad_acc, 0, sizeof(float) * N->nparams, ctx->stream));
+    double lsum = 0;
+    for (int k = 0; k < n; k++) {
+        float s;
+        double l;
+        FSIM_TRY(upload_sample(ctx, b + k * nc, label + k * nc, p + k * nc, &s));
+        FSIM_TRY(net_forward(ctx));
+        FSIM_TRY(net_backward(ctx, &l));
+        lsum += l;
+        hipLaunchKernelGGL(k_axpy_f, dim3(gb(N->nparams)), dim3(256), 0, ctx->stream, N->grad_acc, N->grad, N->nparams);
+    }
+    N->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)N->adam_t), c2 = 1.f - std::pow(b2, (float)N->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(gb(N->nparams)), dim3(256), 0, ctx->stream, N->w32, N->adam_m, N->adam_v, N->grad_acc,
+                       N->nparams, lr, b1, b2, 1e-8f, c1, c2, 1.f / (float)n);
+    FSIM_TRY(sync_bf16(ctx));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *mean_loss = lsum / n;
+    return FSIM_OK;
+}
+
+}  // namespace fsim

@@ -90,7 +90,10 @@ __device__ inline double wave_max(double v) {
 // result is deterministic, and no separate finalize launch is needed.
 constexpr int AUX_NONE = 0, AUX_MAX = 1, AUX_SUM = 2;
 constexpr unsigned kMaxParts = 2048;   // reduction grids have at most this many blocks, a multiple of 8
-enum FinOp { FIN_NONE = -1, FIN_BB = 0, FIN_SIGMA0 = 1, FIN_ALPHA = 2, FIN_RES = 3, FIN_BETA = 4, FIN_TRUE = 5 };
+enum FinOp {
+    FIN_NONE = -1, FIN_BB = 0, FIN_SIGMA0 = 1, FIN_ALPHA = 2, FIN_RES = 3, FIN_BETA = 4, FIN_TRUE = 5, FIN_ZQ = 6,
+    FIN_FBETA = 7
+};
 
 struct Reduce {
     double* partials;   // 2 x gridDim.x
@@ -119,6 +122,13 @@ __device__ inline void apply_fin(int op, double s, double m, SolveScalars* sc) {
         break;
     }
     case FIN_TRUE: sc->tr_ss = s; sc->tr_max = m; break;
+    case FIN_ZQ: sc->zq = s; break;
+    case FIN_FBETA: {   // flexible PCG: beta = z'.(r' - r) / z.r = -alpha z'.q / z.r
+        if (sc->sigma != sc->sigma) sc->nan_flag |= 2;
+        sc->zr = s;
+        sc->beta = -sc->alpha * sc->zq / sc->sigma;
+        break;
+    }
     default: break;
     }
 }
@@ -426,6 +436,20 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
     };
     const RowItems it = row_items(S);
     const int lane = threadIdx.x & 63;
+    // the nbinfo words of the next item are loaded while this item's gathers are in flight
+    auto load_info = [&](int k, uint16_t (&inf)[4]) {
+        if (k >= it.end) return;
+        int p, y, z0;
+        item_cells(it, k, p, y, z0);
+        const int64_t rowbase = ((int64_t)p * S.ny + y) * S.nz;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int zc = z0 + lane + 64 * c;
+            inf[c] = zc < S.nz - 1 ? nbinfo[rowbase + zc] : (uint16_t)0;
+        }
+    };
+    uint16_t info_next[4] = {0, 0, 0, 0};
+    load_info(it.first, info_next);
     for (int k = it.first; k < it.end; k += it.step) {
         int p, y, z0;
         item_cells(it, k, p, y, z0);
@@ -435,11 +459,11 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
         bool any = false;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            zc[c] = z0 + lane + 64 * c;
-            info[c] = zc[c] < S.nz - 1 ? nbinfo[rowbase + zc[c]] : (uint16_t)0;
-            zc[c] = min(zc[c], S.nz - 2);   // lanes past the row gather an interior cell, unused
+            info[c] = info_next[c];
+            zc[c] = min(z0 + lane + 64 * c, S.nz - 2);   // lanes past the row gather an interior cell, unused
             any |= (info[c] & NB_FLUID) != 0;
         }
+        load_info(k + it.step, info_next);
         if (__builtin_amdgcn_ballot_w64(any) == 0) continue;   // no WATER cell in the wave's chunk
         // all gathers of the 4 cells first (interior cells: every neighbour index is in range),
         // then the sums with the reference's term order, by mask
@@ -582,6 +606,42 @@ __global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const uint16_t* __res
         if (nbinfo[i] & NB_FLUID) d += (double)a[i] * (double)b[i];
     }
     block_finish<AUX_NONE>(d, 0.0, red);
+}
+
+// z = y + omega (r - A y) / diag(A) on WATER cells (0 elsewhere), same stencil and item split as k_spmv
+constexpr float kLearnedOmega = 2.0f / 3.0f;
+__global__ void __launch_bounds__(kBlock) k_jacobi_correct(SolveArgs S, const uint16_t* __restrict__ nbinfo,
+                                                           const float* __restrict__ r, const float* __restrict__ y,
+                                                           float* __restrict__ z, float omega) {
+    const float ms = -S.scale_f;
+    const RowItems it = row_items(S);
+    const int lane = threadIdx.x & 63;
+    for (int k = it.first; k < it.end; k += it.step) {
+        int p, yy, z0;
+        item_cells(it, k, p, yy, z0);
+        const int64_t rowbase = ((int64_t)p * S.ny + yy) * S.nz;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int zc = z0 + lane + 64 * c;
+            if (zc >= S.nz - 1) continue;
+            const int64_t i = rowbase + zc;
+            const uint16_t f = nbinfo[i];
+            if (!(f & NB_FLUID)) {
+                z[i] = 0.f;
+                continue;
+            }
+            const RNeighbours n = rneighbours(S, i, p, yy, zc);
+            const float diag = S.scale_f * (float)((f >> NB_NONSOLID_SHIFT) & 7);
+            float ay = diag * y[i];
+            if (f & NB_PX) ay += ms * y[n.px];
+            if (f & NB_PY) ay += ms * y[n.py];
+            if (f & NB_PZ) ay += ms * y[n.pz];
+            if (f & NB_MX) ay += ms * y[n.mx];
+            if (f & NB_MY) ay += ms * y[n.my];
+            if (f & NB_MZ) ay += ms * y[n.mz];
+            z[i] = diag > 0.f ? y[i] + omega * (r[i] - ay) / diag : y[i];   // no non-SOLID neighbour: no correction
+        }
+    }
 }
 
 // FP64 true residual t = b - A x; optionally r = (float)t.
@@ -768,13 +828,14 @@ static int true_residual(fsim_ctx* ctx, const SolveArgs& S, bool replace) {
     return FSIM_OK;
 }
 
-static int dot_zr(fsim_ctx* ctx, int op) {
+static int dot_with(fsim_ctx* ctx, const float* a, const float* b, int op) {
     FSIM_PROF(ctx, KID_DOT);
-    hipLaunchKernelGGL(k_dot, dim3(red_blocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, ctx->g.nc, ctx->nbinfo, ctx->z,
-                       ctx->r, reduce_to(ctx, op));
+    hipLaunchKernelGGL(k_dot, dim3(red_blocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, ctx->g.nc, ctx->nbinfo, a, b,
+                       reduce_to(ctx, op));
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
+static int dot_zr(fsim_ctx* ctx, int op) { return dot_with(ctx, ctx->z, ctx->r, op); }
 
 // New search direction s (z, or z + beta s) into the second buffer, q = A s,
 // s.q -> alpha; with pending_x also the previous iteration's x += alpha s.
@@ -808,6 +869,121 @@ static bool stop_met(const fsim_ctx* ctx, double ss, double mx, double bnorm) {
     return mx < ctx->prm.residual_tolerance;
 }
 
+// z = M^-1 r with the solve's preconditioner: MIC(0) sweeps, or the learned one:
+// y = net(r), then one damped-Jacobi correction z = y + omega D^-1 (r - A y).
+// The network carries the smooth part of A^-1; the correction removes the
+// high-frequency residual it leaves, which a convolutional map trained on
+// right-hand sides handles poorly.
+static int precondition(fsim_ctx* ctx, const SolveArgs& S, bool flexible) {
+    if (!flexible) return mic_apply(ctx, S, ctx->r, ctx->y, ctx->z);
+    FSIM_PROF(ctx, KID_LEARNED);
+    FSIM_TRY(learned_apply(ctx, ctx->r, ctx->y));
+    hipLaunchKernelGGL(k_jacobi_correct, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->r,
+                       ctx->y, ctx->z, kLearnedOmega);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// After a new z: z.r (and, flexible, z.q first), giving beta.  Plain PCG:
+// beta = z'.r' / z.r (Fletcher-Reeves form of the reference).  Flexible PCG
+// (the preconditioner is not a fixed linear map): beta = z'.(r' - r) / z.r
+// = -alpha z'.q / z.r, since r' - r = -alpha q.
+static int next_beta(fsim_ctx* ctx, bool flexible) {
+    if (flexible) FSIM_TRY(dot_with(ctx, ctx->z, ctx->q, FIN_ZQ));
+    return dot_zr(ctx, flexible ? FIN_FBETA : FIN_BETA);
+}
+
+struct PcgResult {
+    int iterations = 0;
+    bool converged = false;   // stopped on the FP64 true residual
+    bool nan = false;         // stopped on a NaN alpha or sigma
+};
+
+// PCG from x = 0, r = b.  Reliable updates (restarted mixed-precision PCG):
+// when the FP32 recursive residual meets the tolerance, or has dropped by
+// kDelta since the last FP64 true residual, the true residual b - A x is
+// recomputed in FP64.  The solve stops if it meets the tolerance; otherwise r
+// takes its value and the search direction restarts from z = M^-1 r.
+//
+// One host round trip per iteration: the iteration's scalars are copied after
+// the r update, and the next preconditioner application and its dots run
+// while the host waits for them.  If the solve stops or restarts, those
+// results are discarded.  x += alpha s of an iteration that goes on is applied
+// by the next direction step.
+static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexible, int max_it, PcgResult* res) {
+    const int64_t nc = S.nc;
+    fsim_step_stats& st = ctx->stats;
+    FSIM_TRY(fill_f64(ctx, ctx->x, 0.0, nc));
+    hipLaunchKernelGGL(k_copy_f32, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->b, ctx->r);
+    FSIM_TRY(fill_f32(ctx, ctx->q, 0.0f, nc));
+    FSIM_TRY(fill_f32(ctx, ctx->z, 0.0f, nc));
+    FSIM_TRY(fill_f32(ctx, ctx->s, 0.0f, nc));
+    FSIM_TRY(fill_f32(ctx, ctx->s2, 0.0f, nc));
+    FSIM_TRY(precondition(ctx, S, flexible));
+    FSIM_TRY(dot_zr(ctx, FIN_SIGMA0));
+    const int max_reliable = 64;
+    const double kDelta = 1e-3;
+    double last_true = bnorm;
+    bool copy = true, pending_x = false;
+    int it = 0;
+    res->converged = false;
+    res->nan = false;
+    for (; it < max_it; it++) {
+        FSIM_TRY(direction_step(ctx, S, copy, pending_x));
+        pending_x = false;
+        {
+            FSIM_PROF(ctx, KID_UPDATE);
+            hipLaunchKernelGGL(k_update_r, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->q,
+                               ctx->r, reduce_to(ctx, FIN_RES));
+        }
+        FSIM_HIP(ctx, hipGetLastError());
+        FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
+        FSIM_HIP(ctx, hipEventRecord(ctx->res_ev, ctx->stream));
+        FSIM_TRY(precondition(ctx, S, flexible));
+        FSIM_TRY(next_beta(ctx, flexible));
+        FSIM_HIP(ctx, hipEventSynchronize(ctx->res_ev));
+        const SolveScalars h = *ctx->scal_host;
+        // the previous iteration's old-sigma test, then this iteration's alpha test
+        if (h.nan_flag & 2) {
+            it--;
+            res->nan = true;
+            break;
+        }
+        if (h.nan_flag & 1) {
+            res->nan = true;
+            break;
+        }
+        bool restart = false;
+        if (stop_met(ctx, h.rr, h.rmax, bnorm) || std::sqrt(h.rr) < kDelta * last_true) {
+            FSIM_TRY(update_x(ctx));
+            const bool replace = st.reliable_updates < max_reliable;
+            FSIM_TRY(true_residual(ctx, S, replace));
+            FSIM_TRY(read_scalars(ctx));
+            if (stop_met(ctx, ctx->scal_host->tr_ss, ctx->scal_host->tr_max, bnorm)) {
+                res->converged = true;
+                break;
+            }
+            if (!replace) {
+                if (stop_met(ctx, h.rr, h.rmax, bnorm)) break;
+            } else {
+                st.reliable_updates++;
+                last_true = std::sqrt(ctx->scal_host->tr_ss);
+                restart = true;
+            }
+        } else {
+            pending_x = true;
+        }
+        if (restart) {
+            FSIM_TRY(precondition(ctx, S, flexible));
+            FSIM_TRY(dot_zr(ctx, FIN_SIGMA0));
+        }
+        copy = restart;
+    }
+    if (pending_x) FSIM_TRY(update_x(ctx));
+    res->iterations = it;
+    return FSIM_OK;
+}
+
 int stage_solve(fsim_ctx* ctx, double dt) {
     const SolveArgs S = solve_args(ctx, dt);
     const int64_t nc = S.nc;
@@ -818,6 +994,12 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     st.fallback = 0;
     st.reliable_updates = 0;
     st.pcg_iterations = 0;
+    st.learned_iterations = 0;
+    st.learned_stop = 0;
+    const bool learned = ctx->prm.precond == FSIM_PRECOND_LEARNED;
+    if (learned && !learned_ready(ctx))
+        return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner selected but it has no parameters "
+                                              "(fsim_learned_init or fsim_learned_set_params)");
     FSIM_HIP(ctx, hipMemsetAsync(ctx->scal, 0, sizeof(SolveScalars), ctx->stream));
     FSIM_TRY(fill_f64(ctx, ctx->x, 0.0, nc));
     FSIM_TRY(build_nbinfo(ctx, S, true));
@@ -834,77 +1016,28 @@ int stage_solve(fsim_ctx* ctx, double dt) {
         st.residual_rel_l2 = 1.0;
         return FSIM_OK;
     }
-    FSIM_TRY(mic_factor(ctx, S));
-    hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(kBlock), 0, ctx->stream, nc, ctx->b, ctx->r);
-    FSIM_TRY(fill_f32(ctx, ctx->q, 0.0f, nc));
-    FSIM_TRY(fill_f32(ctx, ctx->z, 0.0f, nc));
-    FSIM_TRY(fill_f32(ctx, ctx->s, 0.0f, nc));
-    FSIM_TRY(fill_f32(ctx, ctx->s2, 0.0f, nc));
-    FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
-    FSIM_TRY(dot_zr(ctx, FIN_SIGMA0));
     const int max_it = ctx->prm.max_iterations;
-    const int max_reliable = 64;
-    // Reliable updates (restarted mixed-precision PCG): when the FP32
-    // recursive residual meets the tolerance, or has dropped by kDelta since
-    // the last FP64 true residual, the true residual b - A x is recomputed in
-    // FP64.  The solve stops if it meets the tolerance; otherwise r takes its
-    // value and the search direction restarts from z = M^-1 r.
-    //
-    // One host round trip per iteration: the iteration's scalars are copied
-    // after the r update, and the next preconditioner application and z.r
-    // run while the host waits for them.  If the solve stops or restarts,
-    // those results are discarded.  x += alpha s of an iteration that goes
-    // on is applied by the next direction step.
-    const double kDelta = 1e-3;
-    double last_true = bnorm;
-    bool copy = true, pending_x = false;
-    int it = 0;
-    for (; it < max_it; it++) {
-        FSIM_TRY(direction_step(ctx, S, copy, pending_x));
-        pending_x = false;
-        {
-            FSIM_PROF(ctx, KID_UPDATE);
-            hipLaunchKernelGGL(k_update_r, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->q,
-                               ctx->r, reduce_to(ctx, FIN_RES));
+    PcgResult res;
+    bool use_mic = !learned;
+    if (learned) {
+        // flexible PCG with the network; MIC(0)-PCG from scratch when it does not
+        // reach the tolerance within twice the last MIC(0) iteration count
+        const int budget = ctx->last_mic_iterations > 0 ? std::min(max_it, 2 * ctx->last_mic_iterations) : max_it;
+        FSIM_TRY(pcg_run(ctx, S, bnorm, true, budget, &res));
+        st.learned_iterations = res.iterations;
+        st.learned_stop = res.converged ? 0 : res.nan ? 2 : 1;
+        if (!res.converged) {
+            st.fallback = 1;
+            use_mic = true;
+            FSIM_HIP(ctx, hipMemsetAsync(ctx->scal, 0, sizeof(SolveScalars), ctx->stream));
         }
-        FSIM_HIP(ctx, hipGetLastError());
-        FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-        FSIM_HIP(ctx, hipEventRecord(ctx->res_ev, ctx->stream));
-        FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
-        FSIM_TRY(dot_zr(ctx, FIN_BETA));
-        FSIM_HIP(ctx, hipEventSynchronize(ctx->res_ev));
-        const SolveScalars h = *ctx->scal_host;
-        // the previous iteration's old-sigma test, then this iteration's alpha test
-        if (h.nan_flag & 2) {
-            it--;
-            break;
-        }
-        if (h.nan_flag & 1) break;
-        bool restart = false;
-        if (stop_met(ctx, h.rr, h.rmax, bnorm) || std::sqrt(h.rr) < kDelta * last_true) {
-            FSIM_TRY(update_x(ctx));
-            const bool replace = st.reliable_updates < max_reliable;
-            FSIM_TRY(true_residual(ctx, S, replace));
-            FSIM_TRY(read_scalars(ctx));
-            if (stop_met(ctx, ctx->scal_host->tr_ss, ctx->scal_host->tr_max, bnorm)) break;
-            if (!replace) {
-                if (stop_met(ctx, h.rr, h.rmax, bnorm)) break;
-            } else {
-                st.reliable_updates++;
-                last_true = std::sqrt(ctx->scal_host->tr_ss);
-                restart = true;
-            }
-        } else {
-            pending_x = true;
-        }
-        if (restart) {
-            FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
-            FSIM_TRY(dot_zr(ctx, FIN_SIGMA0));
-        }
-        copy = restart;
     }
-    if (pending_x) FSIM_TRY(update_x(ctx));
-    st.pcg_iterations = it;
+    if (use_mic) {
+        FSIM_TRY(mic_factor(ctx, S));
+        FSIM_TRY(pcg_run(ctx, S, bnorm, false, max_it, &res));
+        if (res.converged) ctx->last_mic_iterations = res.iterations;
+    }
+    st.pcg_iterations = res.iterations;
     const double sc = dt / (ctx->prm.fluid_density * ctx->g.cd);
     {
         FSIM_PROF(ctx, KID_PRESSURE);

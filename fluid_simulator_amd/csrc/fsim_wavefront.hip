@@ -59,7 +59,7 @@ namespace fsim {
 constexpr int kSpinLimit = 1 << 21;
 constexpr int kBusyPolls = 32;   // polls before a waiting wave starts to sleep
 constexpr int kPrefetch = WF_PREFETCH;   // compute-wave register prefetch (steps)
-constexpr int kRingV = 8;        // LDS ring of row values between compute waves (steps)
+constexpr int kRingV = kPrefetch;   // LDS ring of row values between compute waves (steps)
 constexpr int kBpEvery = 4;      // ring-reuse check period (steps); allowed lag kRingV - kBpEvery
 constexpr uint32_t kOob = 0x80000000u;        // buffer offset beyond every array: reads 0, drops stores
 constexpr int kRawBufferFlags = 0x00020000;   // buffer resource word 3 (raw 32-bit access, gfx9)
@@ -350,16 +350,38 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     const bool cy = w == 0 && J > 0;
     T prev[R];
     unroll<R>([&](auto rc) { prev[decltype(rc)::value] = 0; });
+    // Upstream values of one step: the previous wave's last row (ring slot),
+    // the -y and -z packets and, every kBpEvery steps, the next wave's ring
+    // slot (reuse check).  The reads of step t+1 are issued right after step
+    // t's ring write, so their latency overlaps the rest of step t; the tag
+    // test at the top of step t+1 re-reads only if a value was not there yet.
+    uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
+    auto rd_at = [&](auto dc2, int tt) {
+        constexpr int d2 = decltype(dc2)::value;   // tt mod kPrefetch (= tt mod kRingV)
+        constexpr int s_up = (d2 + kRingV - 1) % kRingV, s_bp = (d2 + kBpEvery) % kRingV;
+        const int ps = tt % RINGP;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            a[k] = lpeek(up_base + s_up * kSlotV + k * WF_TZ);
+            c[k] = lpeek(yp_base + ps * NP * WF_TZ + k * WF_TZ);
+        }
+        unroll<R>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+#pragma unroll
+            for (int k = 0; k < NP; k++) zq[r][k] = lpeek(zp_base + ps * NP * TY + k * TY + r);
+        });
+        if constexpr (d2 % kBpEvery == 0) bp = lpeek(dn_base + s_bp * kSlotV);
+    };
+    static_assert(kRingV == kPrefetch, "ring slots are indexed by the unrolled step");
+    rd_at(std::integral_constant<int, 0>{}, 0);
     for (int t0 = 0; t0 < nsteps; t0 += kPrefetch) {
         // compile-time step index, so each prefetch slot stays in one register
         // and the value-ring slots are constants
         unroll<kPrefetch>([&](auto dc) {
             constexpr int d = decltype(dc)::value;
-            constexpr int s_up = (d + kRingV - 1) % kRingV, s_bp = (d + kBpEvery) % kRingV;
             constexpr bool bp_step = d % kBpEvery == 0;
             const int t = t0 + d;
-            const int ps = t % RINGP;
-            // ---- receive: unconditional LDS reads of valid slots, then tag checks
+            // ---- receive: tag checks of the values read ahead (re-read while not ready)
             const bool need_up = w > 0 && t > 0;
             const bool need_bp = bp_step && w < NW - 1 && t >= kRingV - kBpEvery;
             const int t_bp = t - (kRingV - kBpEvery);
@@ -374,35 +396,21 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                 need_z[r] = colz[r] && xin0;
             });
             const bool need_y = cy && act[0];
-            uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
-            auto rd = [&]() {
-#pragma unroll
-                for (int k = 0; k < NP; k++) {
-                    a[k] = lpeek(up_base + s_up * kSlotV + k * WF_TZ);
-                    c[k] = lpeek(yp_base + ps * NP * WF_TZ + k * WF_TZ);
-                }
-                unroll<R>([&](auto rc) {
-                    constexpr int r = decltype(rc)::value;
-#pragma unroll
-                    for (int k = 0; k < NP; k++) zq[r][k] = lpeek(zp_base + ps * NP * TY + k * TY + r);
-                });
-                if constexpr (bp_step) bp = lpeek(dn_base + s_bp * kSlotV);
-            };
             // branch-free: per-lane masks, one ballot (the -z packet slots are
             // read by every lane from one address, so their test is uniform)
             auto ready = [&]() {
-                bool lane = (tag_of(bp) >= t_bp) | !need_bp;
+                unsigned lane = (unsigned)(tag_of(bp) >= t_bp) | (unsigned)!need_bp;
 #pragma unroll
                 for (int k = 0; k < NP; k++) {
-                    lane = lane & ((tag_of(a[k]) == t - 1) | !need_up) & ((tag_of(c[k]) == t) | !need_y);
+                    lane &= ((unsigned)(tag_of(a[k]) == t - 1) | (unsigned)!need_up) &
+                            ((unsigned)(tag_of(c[k]) == t) | (unsigned)!need_y);
                     unroll<R>([&](auto rc) {
                         constexpr int r = decltype(rc)::value;
-                        lane = lane & ((tag_of(zq[r][k]) == t) | !need_z[r]);
+                        lane &= (unsigned)(tag_of(zq[r][k]) == t) | (unsigned)!need_z[r];
                     });
                 }
-                return __builtin_amdgcn_ballot_w64(!lane) == 0;
+                return __builtin_amdgcn_ballot_w64(lane == 0u) == 0;
             };
-            rd();
             if (!ready()) {
                 int spins = 0;
                 do {
@@ -412,7 +420,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     }
                     if ((spins & 1023) == 0 && gave_up(A.err)) break;
                     if (spins > kBusyPolls) __builtin_amdgcn_s_sleep(1);
-                    rd();
+                    rd_at(dc, t);
                 } while (!ready());
             }
             // ---- compute the R rows (all upstream values are from step t-1)
@@ -495,6 +503,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             // ---- publish the last row for wave w+1, advance, prefetch step t + kPrefetch
 #pragma unroll
             for (int k = 0; k < NP; k++) lput(my_base + d * kSlotV + k * WF_TZ, pack(word(vnew[R - 1], k), (uint32_t)t));
+            rd_at(std::integral_constant<int, (d + 1) % kPrefetch>{}, t + 1);
             // the packet reads of steps <= t have completed (their values were used above)
             if (d == kPrefetch - 1 && tz == 0) lput32(&S.prog[w], t);
             unroll<R>([&](auto rc) {

@@ -156,6 +156,8 @@ typedef struct fsim_step_stats {
     double solve_ms;                    /* RHS ready -> pressure converged (HIP events) */
     int64_t num_particles;
     int64_t num_fluid_cells;
+    int learned_iterations;             /* flexible-PCG iterations run with the learned preconditioner */
+    int learned_stop;                   /* why FPCG stopped: 0 converged, 1 iteration budget, 2 NaN */
 } fsim_step_stats;
 
 /* Grid fields for fsim_get_grid / fsim_set_grid (host layout above). */
@@ -239,6 +241,36 @@ int fsim_num_kernels(void);
 const char* fsim_kernel_name(int kernel);
 int fsim_get_kernel_time(fsim_ctx* ctx, int kernel, double* total_ms, int64_t* launches);
 int fsim_reset_kernel_times(fsim_ctx* ctx);
+
+/* ---- learned preconditioner (no counterpart in the reference) -------------
+ * A 4-level 3D convolutional encoder-decoder (3x3x3 kernels, 16 base
+ * channels, ReLU, BF16 weights and activations, FP32 accumulation) that maps
+ * (r / max|r|, WATER mask) to an approximation of A^-1 r.  With
+ * fsim_params.precond = FSIM_PRECOND_LEARNED the pressure solve runs flexible
+ * PCG with it and falls back to MIC(0)-PCG (fsim_step_stats.fallback = 1) when
+ * the tolerance is not reached within twice the last MIC(0) iteration count
+ * (max_iterations if MIC(0) has not run yet).  The network is fully
+ * convolutional: parameters trained on a small grid apply to any grid.
+ * Parameter vector order: for the 7 conv layers (8->16, 16->32, 32->64,
+ * 64->128, 192->64, 96->32, 48->16) the weights [cout][27 taps][cin] (tap =
+ * ((dx+1)*3 + dy+1)*3 + dz+1) then the biases [cout]; then the 1x1 output
+ * layer's 16 weights and its bias. */
+int64_t fsim_learned_num_params(void);
+/* He-normal weights from a counter RNG keyed by seed, zero biases */
+int fsim_learned_init(fsim_ctx* ctx, uint64_t seed);
+int fsim_learned_set_params(fsim_ctx* ctx, const float* params);
+int fsim_learned_get_params(fsim_ctx* ctx, float* params);
+/* Network output (before the max|b| scale and the WATER mask) for one
+ * right-hand side b and label field, both in flat grid order. */
+int fsim_learned_forward(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out);
+/* Loss (mean squared error on WATER cells against p / max|b|) and its gradient
+ * with respect to every parameter, for one sample (flat grid order). */
+int fsim_learned_grad(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad,
+                      double* loss);
+/* One Adam step (lr, beta 0.9 / 0.999, eps 1e-8) on the mean gradient of n
+ * samples stored back to back (n * Nc values each). */
+int fsim_learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
+                             double* mean_loss);
 
 /* ---- synthetic scenes (host-only, no GPU needed) -------------------------- */
 /* Dam-break generator: a cell (not on the domain border, not inside any

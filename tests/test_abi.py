@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "fsim.h")
 
 def declared_functions():
     src = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fsim_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(fsim_\w+)\s*\(", src, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
