@@ -54,7 +54,7 @@ constexpr int WF_R = FSIM_WF_R;
 constexpr int WF_NW = FSIM_WF_NW;
 constexpr int WF_TY = WF_R * WF_NW;
 constexpr int WF_TZ = 64;
-constexpr int WF_THREADS = (WF_NW + 2) * WF_TZ;
+constexpr int WF_THREADS = (WF_NW + 4) * WF_TZ;   // compute waves, 2 receiving and 2 sending edge waves
 #ifndef FSIM_WF_PREFETCH
 #define FSIM_WF_PREFETCH 8
 #endif

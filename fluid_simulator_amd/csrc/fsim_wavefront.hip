@@ -36,12 +36,15 @@
 // with device-scope stores, one packet row per step of the consuming tile
 // (ybnd[tile][s][64], zbnd[tile][s][WF_TY] of the producing tile), so a
 // step's packets are contiguous.
-// Two communication waves per workgroup read the upstream tile's packets in
-// batches of steps and republish them into LDS rings tagged with the step.
-// This keeps every global read whose result is needed "now" out of the
-// compute waves: on gfx9 the memory counter completes in order, so such a
-// read would drain the compute waves' prefetch stream (per-cell inputs are
-// loaded kPrefetch steps ahead, each prefetch slot in a fixed register).
+// Four communication waves per workgroup keep all of this traffic out of
+// the compute waves: two read the upstream tiles' packets, a few steps at a
+// time, and republish them into LDS rings tagged with the step; two take this
+// tile's edge values from LDS out-rings (the last compute wave's row, lane 63
+// of every row) and store them as packets.  On gfx9 the memory counter
+// completes in order, so a global read whose result is needed "now", or a
+// slowly acknowledged device-scope store, would stall the compute waves'
+// prefetch stream (per-cell inputs are loaded kPrefetch steps ahead, each
+// prefetch slot in a fixed register).
 //
 // Every wait is bounded: past kSpinLimit polls, or as soon as any wave in the
 // grid has given up, a wave raises the error flag and carries on, so the
@@ -50,6 +53,7 @@
 // workgroups already running.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 #include "fsim_internal.hpp"
@@ -61,9 +65,9 @@ constexpr int kBusyPolls = 32;   // polls before a waiting wave starts to sleep
 constexpr int kPrefetch = WF_PREFETCH;   // compute-wave register prefetch (steps)
 constexpr int kRingV = kPrefetch;   // LDS ring of row values between compute waves (steps)
 constexpr int kBpEvery = 4;      // ring-reuse check period (steps); allowed lag kRingV - kBpEvery
+constexpr int kRingO = 32;       // LDS rings of this tile's +y / +z edge values (steps), a power of two
 constexpr uint32_t kOob = 0x80000000u;        // buffer offset beyond every array: reads 0, drops stores
 constexpr int kRawBufferFlags = 0x00020000;   // buffer resource word 3 (raw 32-bit access, gfx9)
-constexpr int kDeviceScope = 16;              // buffer cache-policy bit sc1: device-scope coherence
 
 template <int KIND>
 struct Kind {   // 1 = forward sweep, 2 = backward sweep (FP32)
@@ -169,6 +173,9 @@ struct WaveLds {
     uint64_t yp[RINGP][NP][WF_TZ];          // tagged -y packets for row 0
     uint64_t zp[RINGP][NP][WF_TY];          // tagged -z packets for lane 0 of each row
     int prog[WF_NW];                        // a step whose packets the compute wave has consumed
+    uint64_t yo[kRingO][NP][WF_TZ];         // tagged last row of the last compute wave: the +y edge
+    uint64_t zo[kRingO][NP][WF_TY];         // tagged lane-63 value of every row: the +z edge
+    int sprog[2];                           // compute step up to which the +y / +z edge has been sent
 };
 
 // ---- communication wave: upstream tile edge -> LDS packet ring ------------
@@ -237,6 +244,61 @@ __device__ void comm_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
     }
 }
 
+// ---- sending wave: this tile's edge (LDS out-ring) -> HBM packets ----------
+// The packet of downstream step s is the value the compute waves produced at
+// step s + lag (lag NW-1 for the +y edge: the last row of wave NW-1; 63 for
+// the +z edge: lane 63 of every row).  The wave waits until every value of
+// that step carries its step tag, stores the step's packets with
+// device-scope stores and reports its progress every 4 steps, which the
+// compute waves check before reusing an out-ring slot.  The device-scope
+// stores stay out of the compute waves, whose memory counter also orders
+// their prefetch loads.
+template <int KIND, bool YEDGE>
+__device__ void send_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
+    constexpr int NP = Kind<KIND>::NP;
+    constexpr int NL = YEDGE ? WF_TZ : WF_TY;
+    int* prog = &S.sprog[YEDGE ? 0 : 1];
+    const int l = threadIdx.x % 64;
+    if (YEDGE ? J == A.TJ - 1 : K == A.TK - 1) return;   // no downstream tile: progress preset to "all sent"
+    const bool on = l < NL;
+    const int li = on ? l : 0;
+    const int lag = YEDGE ? WF_NW - 1 : WF_TZ - 1;
+    const int end = (YEDGE ? WF_TZ - 1 : WF_NW - 1) + (A.nx - 2);   // downstream steps consumed
+    const int64_t tile = (int64_t)J * A.TK + K;
+    uint64_t* out = YEDGE ? A.ybnd + tile * A.SP * WF_TZ * NP : A.zbnd + tile * A.SP * WF_TY * NP;
+    for (int s = 0; s < end; s++) {
+        const int t = s + lag;
+        const uint64_t* src = YEDGE ? &S.yo[t & (kRingO - 1)][0][li] : &S.zo[t & (kRingO - 1)][0][li];
+        uint64_t v[NP];
+        int spins = 0;
+        for (;;) {
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < NP; k++) {
+                v[k] = lpeek(src + k * NL);
+                ok = ok && (!on || tag_of(v[k]) == t);
+            }
+            if (__all(ok)) break;
+            if (++spins > kSpinLimit || ((spins & 1023) == 0 && gave_up(A.err))) {
+                if (l == 0) {
+                    atomicOr(A.err, 1);
+                    lput32(prog, INT_MAX);   // never block the compute waves
+                }
+                return;
+            }
+            if (spins > kBusyPolls) __builtin_amdgcn_s_sleep(1);
+        }
+        if (on) {
+#pragma unroll
+            for (int k = 0; k < NP; k++)
+                __hip_atomic_store(out + ((int64_t)s * NL + li) * NP + k, pack((uint32_t)v[k], A.epoch), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if ((s & 3) == 3 && l == 0) lput32(prog, t);
+    }
+    if (l == 0) lput32(prog, INT_MAX);
+}
+
 // ---- compute waves -------------------------------------------------------------
 // Factor (KIND 0, FP64):   e = diag - sum_a (A_a p_a)^2 - tau sum_a A_a (o1 + o2) p_a^2,
 //                          guarded by sigma, p = 1/sqrt(e)  (calculatePreconditioner)
@@ -259,7 +321,6 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     const uint32_t plane = (uint32_t)(A.ny * A.nz);
     const uint32_t nc = (uint32_t)nx * plane;   // below 2^28 (kWaveMaxCells)
     const uint32_t plane8 = plane * 8u, nc8 = nc * 8u;
-    const uint32_t epoch = A.epoch;
     // buffer resources.  factor: rA nbinfo, rB W, rC P64, rD P32;  sweeps: rA W, rB in, rC out
     __amdgpu_buffer_rsrc_t rA, rB, rC, rD;
     if constexpr (KIND == 0) {
@@ -273,9 +334,6 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
         rC = rsrc(A.out, (int64_t)nc * 4);
         rD = rC;
     }
-    const int64_t tile = (int64_t)J * A.TK + K;
-    const __amdgpu_buffer_rsrc_t ryb = rsrc(A.ybnd + tile * A.SP * WF_TZ * NP, (int64_t)A.SP * WF_TZ * NP * 8);
-    const __amdgpu_buffer_rsrc_t rzb = rsrc(A.zbnd + tile * A.SP * TY * NP, (int64_t)A.SP * TY * NP * 8);
     const float ms = A.ms;
     const double scale = A.scale;
 
@@ -287,7 +345,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     // elements (kOob, shifted, stays out of range for every array).  Per
     // wave: the byte offsets (8-byte elements) of the rotated plane of the
     // current step and of the prefetch step, in scalar registers.
-    uint32_t lo8[R], lo4[R], lo2[R], zo[R];
+    uint32_t lo8[R], lo4[R], lo2[R];
     bool colr[R], colz[R];
     const int z = z_of<FWD>(A, K, tz);
     const int base = w + tz;
@@ -302,8 +360,6 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
         lo8[r] = colr[r] ? (uint32_t)(y * A.nz + z) * 8u : kOob;
         lo4[r] = lo8[r] >> 1;
         lo2[r] = lo8[r] >> 2;
-        // lane 63 writes the row's -z edge packet (every row, so each packet step is complete)
-        zo[r] = tz == WF_TZ - 1 ? (uint32_t)(ty * NP * 8) : kOob;
         // lane 0 of the row reads -z packets when there is an upstream z tile
         colz[r] = K > 0 && rowin;
     });
@@ -331,12 +387,13 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
         }
     };
     // issue order pinned, and each prologue step issues as many memory
-    // operations as the loop steps do at least (2R stores, here dropped as
+    // operations as the loop steps do (the stores of a step, here dropped as
     // out of range, then the loads): the wait for slot d then allows the same
     // number of younger operations on entry as at every later step
+    constexpr int kStores = (KIND == 0 ? 3 : 1) * R;
     unroll<kPrefetch>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
-        unroll<2 * R>([&](auto) { bstore32(0u, rC, kOob, 0); });
+        unroll<kStores>([&](auto) { bstore32(0u, rC, kOob, 0); });
         unroll<R>([&](auto rc) { fetch(d, decltype(rc)::value); });
         advance(op);
         __builtin_amdgcn_sched_barrier(0);
@@ -356,6 +413,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     // t's ring write, so their latency overlaps the rest of step t; the tag
     // test at the top of step t+1 re-reads only if a value was not there yet.
     uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
+    int spy = 0, spz = 0;   // progress of the sending waves (read on reuse-check steps)
     auto rd_at = [&](auto dc2, int tt) {
         constexpr int d2 = decltype(dc2)::value;   // tt mod kPrefetch (= tt mod kRingV)
         constexpr int s_up = (d2 + kRingV - 1) % kRingV, s_bp = (d2 + kBpEvery) % kRingV;
@@ -370,7 +428,11 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
 #pragma unroll
             for (int k = 0; k < NP; k++) zq[r][k] = lpeek(zp_base + ps * NP * TY + k * TY + r);
         });
-        if constexpr (d2 % kBpEvery == 0) bp = lpeek(dn_base + s_bp * kSlotV);
+        if constexpr (d2 % kBpEvery == 0) {
+            bp = lpeek(dn_base + s_bp * kSlotV);
+            spy = lpeek32(&S.sprog[0]);
+            spz = lpeek32(&S.sprog[1]);
+        }
     };
     static_assert(kRingV == kPrefetch, "ring slots are indexed by the unrolled step");
     rd_at(std::integral_constant<int, 0>{}, 0);
@@ -385,6 +447,10 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             const bool need_up = w > 0 && t > 0;
             const bool need_bp = bp_step && w < NW - 1 && t >= kRingV - kBpEvery;
             const int t_bp = t - (kRingV - kBpEvery);
+            // out-ring slots of steps t .. t + kBpEvery - 1 are free once the sending waves passed
+            // step t + kBpEvery - 1 - kRingO (the last wave feeds the +y ring, every wave the +z ring)
+            const int t_bo = t + kBpEvery - 1 - kRingO;
+            const bool bo_ok = !bp_step || (spz >= t_bo && (w != NW - 1 || spy >= t_bo));
             // the lane's x-index is in the grid (same for all rows of the wave),
             // and the same test for lane 0 (the -z packet consumer)
             const bool xin = (unsigned)(t - base) < (unsigned)nxi;
@@ -409,7 +475,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                         lane &= (unsigned)(tag_of(zq[r][k]) == t) | (unsigned)!need_z[r];
                     });
                 }
-                return __builtin_amdgcn_ballot_w64(lane == 0u) == 0;
+                return bo_ok && __builtin_amdgcn_ballot_w64(lane == 0u) == 0;
             };
             if (!ready()) {
                 int spins = 0;
@@ -485,24 +551,25 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     vnew[r] = fl ? v : 0.f;
                     bstore32(__float_as_uint(outv), rC, fl ? lo4[r] : kOob, oc >> 1);
                 }
-                // -z edge packet of this row (lane 63) for the tile K+1, step t - 63 of its lane 0
-                if (t >= WF_TZ - 1) {
-                    const uint32_t so = (uint32_t)(t - (WF_TZ - 1)) * (TY * NP * 8u);
-#pragma unroll
-                    for (int k = 0; k < NP; k++)
-                        bstore64<kDeviceScope>(pack(word(vnew[r], k), epoch), rzb, zo[r] + k * 8u, so);
-                }
             });
-            // -y edge packets of the tile's last row for the tile J+1, step t - (NW-1) of its row 0
-            if (w == NW - 1 && t >= NW - 1) {
-                const uint32_t so = (uint32_t)(t - (NW - 1)) * (WF_TZ * NP * 8u);
-#pragma unroll
-                for (int k = 0; k < NP; k++)
-                    bstore64<kDeviceScope>(pack(word(vnew[R - 1], k), epoch), ryb, (uint32_t)(tz * NP + k) * 8u, so);
-            }
-            // ---- publish the last row for wave w+1, advance, prefetch step t + kPrefetch
+            // ---- publish: the last row for wave w+1 (the last wave's into the +y out-ring), lane 63 of
+            // every row into the +z out-ring (both sent by the sending waves), then read ahead,
+            // advance and prefetch step t + kPrefetch
+            // (the last wave also keeps writing its ring slot: wave NW-2's reuse check reads its tags)
+            const int so = t & (kRingO - 1);
 #pragma unroll
             for (int k = 0; k < NP; k++) lput(my_base + d * kSlotV + k * WF_TZ, pack(word(vnew[R - 1], k), (uint32_t)t));
+            if (w == NW - 1) {
+#pragma unroll
+                for (int k = 0; k < NP; k++) lput(&S.yo[so][k][tz], pack(word(vnew[R - 1], k), (uint32_t)t));
+            }
+            if (tz == WF_TZ - 1) {
+                unroll<R>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+#pragma unroll
+                    for (int k = 0; k < NP; k++) lput(&S.zo[so][k][w * R + r], pack(word(vnew[r], k), (uint32_t)t));
+                });
+            }
             rd_at(std::integral_constant<int, (d + 1) % kPrefetch>{}, t + 1);
             // the packet reads of steps <= t have completed (their values were used above)
             if (d == kPrefetch - 1 && tz == 0) lput32(&S.prog[w], t);
@@ -529,12 +596,21 @@ __global__ void __launch_bounds__(WF_THREADS) k_wave(WaveArgs A) {
     for (int i = threadIdx.x; i < kRingV * WF_NW * NP * WF_TZ; i += WF_THREADS) (&S.v[0][0][0][0])[i] = ~0ull;
     for (int i = threadIdx.x; i < RINGP * NP * WF_TZ; i += WF_THREADS) (&S.yp[0][0][0])[i] = ~0ull;
     for (int i = threadIdx.x; i < RINGP * NP * WF_TY; i += WF_THREADS) (&S.zp[0][0][0])[i] = ~0ull;
+    for (int i = threadIdx.x; i < kRingO * NP * WF_TZ; i += WF_THREADS) (&S.yo[0][0][0])[i] = ~0ull;
+    for (int i = threadIdx.x; i < kRingO * NP * WF_TY; i += WF_THREADS) (&S.zo[0][0][0])[i] = ~0ull;
     if (threadIdx.x < WF_NW) S.prog[threadIdx.x] = -1;
+    // steps below a sending wave's lag are never sent; without a downstream tile nothing is
+    if (threadIdx.x == 0) {
+        S.sprog[0] = J == A.TJ - 1 ? INT_MAX : WF_NW - 2;
+        S.sprog[1] = K == A.TK - 1 ? INT_MAX : WF_TZ - 2;
+    }
     __syncthreads();
     const int wave = threadIdx.x / 64;
     if (wave < WF_NW) compute_wave<KIND>(A, S, J, K);
     else if (wave == WF_NW) comm_wave<KIND, true>(A, S, J, K);
-    else comm_wave<KIND, false>(A, S, J, K);
+    else if (wave == WF_NW + 1) comm_wave<KIND, false>(A, S, J, K);
+    else if (wave == WF_NW + 2) send_wave<KIND, true>(A, S, J, K);
+    else send_wave<KIND, false>(A, S, J, K);
 }
 
 int wave_launch(fsim_ctx* ctx, int kind, const float* in, float* out, double scale) {
