@@ -127,7 +127,7 @@ def main():
     ap.add_argument("--size", type=int, default=256, help="grid cells per axis (default 256)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-pcg-cap", type=int, default=12)
+    ap.add_argument("--cpu-pcg-cap", type=int, default=60)
     ap.add_argument("--profile-steps", type=int, default=None,
                     help="steps timed per kernel with HIP events after the timed region (default: --steps)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -450,22 +450,39 @@ __global__ void __launch_bounds__(kBlock) k_bin_keys(const float* __restrict__ p
                                                      uint32_t* __restrict__ key, int32_t* __restrict__ rank,
                                                      int32_t* __restrict__ count, int64_t n, double cdinv, int nx,
                                                      int ny, int nz, uint32_t nc) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t k;
-    if (removed[i]) {
-        k = nc;
-    } else {
-        // ivec3(pos * cellDInv), simulator.cpp:358-359, evaluated in FP64
-        // positions outside the grid (never produced by the step, whose
-        // advection clamps them) are clamped into it so no key leaves [0, Nc)
-        int cx = min(max((int)((double)px[i] * cdinv), 0), nx - 1);
-        int cy = min(max((int)((double)py[i] * cdinv), 0), ny - 1);
-        int cz = min(max((int)((double)pz[i] * cdinv), 0), nz - 1);
-        k = (uint32_t)(((int64_t)cx * ny + cy) * nz + cz);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < n;
+    uint32_t k = 0xffffffffu;   // lanes past n: a key of their own, no count
+    if (in) {
+        if (removed[i]) {
+            k = nc;
+        } else {
+            // ivec3(pos * cellDInv), simulator.cpp:358-359, evaluated in FP64
+            // positions outside the grid (never produced by the step, whose
+            // advection clamps them) are clamped into it so no key leaves [0, Nc)
+            int cx = min(max((int)((double)px[i] * cdinv), 0), nx - 1);
+            int cy = min(max((int)((double)py[i] * cdinv), 0), ny - 1);
+            int cz = min(max((int)((double)pz[i] * cdinv), 0), nz - 1);
+            k = (uint32_t)(((int64_t)cx * ny + cy) * nz + cz);
+        }
+        key[i] = k;
     }
-    key[i] = k;
-    rank[i] = atomicAdd(&count[k], 1);
+    // The particles arrive sorted by cell (the previous bin), so neighbouring
+    // lanes share keys: each run of equal keys takes one atomicAdd of its
+    // length and its lanes number themselves from the returned base.  The
+    // rank inside a cell is provisional (the segment sort fixes the order).
+    const int lane = threadIdx.x & 63;
+    const uint32_t kprev = (uint32_t)__shfl_up((int)k, 1, 64);
+    const bool head = lane == 0 || kprev != k;
+    const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const int s = 63 - __clzll(heads & upto);   // this lane's run head
+    const uint64_t later = heads & ~upto;
+    const int e = later ? __ffsll((long long)later) - 1 : 64;   // next run head
+    int base = 0;
+    if (head && in) base = atomicAdd(&count[k], e - s);
+    base = __shfl(base, s, 64);
+    if (in) rank[i] = base + (lane - s);
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint32_t* __restrict__ key, const int32_t* __restrict__ rank,

@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
             v[c][4] = dir(n.mx);
             v[c][5] = dir(n.my);
             v[c][6] = dir(n.mz);
-            xo[c] = pending_x ? x[i] : 0.0;
+            xo[c] = pending_x && (info[c] & NB_FLUID) ? x[i] : 0.0;
         }
 #pragma unroll
         for (int c = 0; c < 4; c++) {
