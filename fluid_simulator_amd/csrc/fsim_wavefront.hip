@@ -175,6 +175,7 @@ struct WaveLds {
     int prog[WF_NW];                        // a step whose packets the compute wave has consumed
     uint64_t yo[kRingO][NP][WF_TZ];         // tagged last row of the last compute wave: the +y edge
     uint64_t zo[kRingO][NP][WF_TY];         // tagged lane-63 value of every row: the +z edge
+    uint64_t zsink[KIND == 0 ? 1 : WF_NW][WF_TZ - 1][NP * WF_TY];   // sweeps: +z writes of lanes 0..62, never read
     int sprog[2];                           // compute step up to which the +y / +z edge has been sent
 };
 
@@ -331,7 +332,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     } else {
         rA = rsrc(A.W, (int64_t)nc * 8);
         rB = rsrc(A.in, (int64_t)nc * 4);
-        rC = rsrc(A.out, (int64_t)nc * 4);
+        // one word longer than the input's range (never addressed): distinct descriptor words
+        // keep the compiler from rebuilding one descriptor from the other at every store
+        rC = rsrc(A.out, (int64_t)nc * 4 + 4);
         rD = rC;
     }
     const float ms = A.ms;
@@ -402,6 +405,11 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     const uint64_t* up_base = &S.v[0][w > 0 ? w - 1 : 0][0][tz];
     const uint64_t* dn_base = &S.v[0][w < NW - 1 ? w + 1 : w][0][tz];
     uint64_t* my_base = &S.v[0][w][0][tz];
+    // +z out-ring: lane 63 writes its rows into zo[slot][k][w R + r]; the other lanes write the same
+    // pattern into a sink of their own, so the store needs no lane mask (slot stride 0 there)
+    const bool zl = tz == WF_TZ - 1;
+    uint64_t* const zo_lane = zl ? &S.zo[0][0][w * R] : &S.zsink[w][tz][0];
+    const uint32_t zo_stride = zl ? NP * TY : 0;   // words per ring slot
     const uint64_t* yp_base = &S.yp[0][0][tz];
     const uint64_t* zp_base = &S.zp[0][0][w * R];
     const bool cy = w == 0 && J > 0;
@@ -443,6 +451,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             constexpr int d = decltype(dc)::value;
             constexpr bool bp_step = d % kBpEvery == 0;
             const int t = t0 + d;
+            // the tail of the previous step (prefetch, plane advance) stays ahead of the
+            // tag checks, so the latency of the values read ahead is overlapped
+            __builtin_amdgcn_sched_barrier(0);
             // ---- receive: tag checks of the values read ahead (re-read while not ready)
             const bool need_up = w > 0 && t > 0;
             const bool need_bp = bp_step && w < NW - 1 && t >= kRingV - kBpEvery;
@@ -536,8 +547,8 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     const uint64_t wd = wring[d][r];
                     const float p = __uint_as_float((uint32_t)wd);
                     const bool fl = act[r] && ((uint32_t)(wd >> 32) & NB_FLUID) != 0;
-                    float acc = 0.f;
-                    acc += ms * prev[r];
+                    // 0 + ms * prev in one rounding: fma(ms, prev, +0) equals the product, and +0 for -0
+                    float acc = __builtin_fmaf(ms, prev[r], 0.f);
                     acc += ms * vy;
                     acc += ms * vz;
                     float outv, v;
@@ -563,11 +574,14 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
 #pragma unroll
                 for (int k = 0; k < NP; k++) lput(&S.yo[so][k][tz], pack(word(vnew[R - 1], k), (uint32_t)t));
             }
-            if (tz == WF_TZ - 1) {
+            // (the factor's FP64 values would make the sink writes of all lanes cost more LDS
+            // bandwidth than the lane mask costs instructions: it keeps the mask)
+            if (KIND != 0 || zl) {
+                uint64_t* zdst = zo_lane + (uint32_t)so * zo_stride;
                 unroll<R>([&](auto rc) {
                     constexpr int r = decltype(rc)::value;
 #pragma unroll
-                    for (int k = 0; k < NP; k++) lput(&S.zo[so][k][w * R + r], pack(word(vnew[r], k), (uint32_t)t));
+                    for (int k = 0; k < NP; k++) lput(zdst + k * TY + r, pack(word(vnew[r], k), (uint32_t)t));
                 });
             }
             rd_at(std::integral_constant<int, (d + 1) % kPrefetch>{}, t + 1);
