@@ -8,8 +8,10 @@ binning, P2G, labels, pressure solve, extrapolation, G2P, with the particle
 and grid state resident in HBM.
 
 Prints one JSON line: steps/s (value), plus the roofline of the dominant
-kernel measured with HIP events on the library's stream, and the FP64 CPU
-oracle timed on a bounded sample of the same workload.
+kernel measured with HIP events on the library's stream, the FP64 CPU
+oracle timed on a bounded sample of the same workload, and the second
+metric, pressure-solve ms to 1e-6 with MIC(0)-PCG vs the learned FPCG on the
+same right-hand side at 128^3 (network trained in this run, on the GPU).
 """
 from __future__ import annotations
 
@@ -24,7 +26,7 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-from fluid_simulator_amd import capi, scenes  # noqa: E402
+from fluid_simulator_amd import capi, learned, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
@@ -119,6 +121,91 @@ def pmc_traffic(kernel: str, size: int) -> float | None:
         return None
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16
+
+
+def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int) -> dict:
+    """Pressure-solve ms to 1e-6 relative residual, MIC(0)-PCG vs the learned FPCG, on the same right-hand side.
+
+    The network is trained here on the GPU (generate_samples + train: real MIC(0)-PCG solves of seeded
+    random dam breaks at train_n^3 and residual pairs), then set on an n^3 context.  The n^3 scene is the
+    APIC dam break with a static sphere; it is stepped 3 times with MIC(0)-PCG, its particle state is
+    saved, and the next step is solved from that same state with each preconditioner (the second of two
+    runs is kept, so nothing is allocated in the timed one).  `learned` is the production path: FPCG
+    with the MIC(0)-PCG fallback after twice the last MIC(0) iteration count.  `fpcg_alone` is the
+    same FPCG on a fresh context, whose budget is max_iterations = 10 x the MIC(0) count."""
+    t0 = time.perf_counter()
+    tgrid, B, L, P = learned.generate_samples(train_n, train_scenes, 4, seed=0, residuals_per_solve=6)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    params, hist = learned.train(tgrid, B, L, P, epochs=epochs, batch=4, lr=1e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    print(f"# learned: {len(B)} samples at {train_n}^3 in {t_gen:.1f}s, {epochs} epochs in {t_train:.1f}s, "
+          f"loss {hist[0]:.3e} -> {hist[-1]:.3e}", file=sys.stderr, flush=True)
+
+    sc = scenes.CONFIG_128_APIC_SPHERE() if n == 128 else scenes.dam_break(n, capi.APIC, seed=0, sphere=True)
+    sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
+    sim.set_obstacles(sc.obstacles)
+    sim.set_particles(sc.particles())
+    for _ in range(3):
+        sim.step(scenes.DT)
+    state = sim.get_particles()
+    res = {}
+
+    def solve(name, precond, prm=None, profile=False):
+        sim.set_params(scenes.params(transfer=sc.transfer, precond=precond))
+        if prm is not None:
+            sim.learned_set_params(prm)
+        for _ in range(2):
+            sim.set_particles(*state)
+            st = sim.step(scenes.DT)
+        res[name] = {"solve_ms": round(st.solve_ms, 3), "pcg_iterations": st.pcg_iterations,
+                     "learned_iterations": st.learned_iterations, "learned_stop": st.learned_stop,
+                     "fallback": st.fallback, "residual_rel_l2": st.residual_rel_l2}
+        print(f"# {n}^3 {name}: {res[name]}", file=sys.stderr, flush=True)
+        if not profile:
+            return None
+        # a third run with per-kernel HIP events, for the network's time per application
+        sim.set_particles(*state)
+        sim.set_option(capi.OPT_PROFILE, 1)
+        sim.reset_kernel_times()
+        sim.step(scenes.DT)
+        sim.set_option(capi.OPT_PROFILE, 0)
+        return sim.kernel_times()
+
+    solve("mic0_pcg", capi.PRECOND_MIC0)
+    kt = solve("learned", capi.PRECOND_LEARNED, params, profile=True)
+    k_mic = res["mic0_pcg"]["pcg_iterations"]
+    sim.close()
+    alone = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED,
+                                                     max_iterations=10 * k_mic))
+    alone.set_obstacles(sc.obstacles)
+    alone.learned_set_params(params)
+    alone.set_particles(*state)
+    st = alone.step(scenes.DT)
+    alone.close()
+    res["fpcg_alone"] = {"learned_iterations": st.learned_iterations, "learned_stop": st.learned_stop,
+                         "budget": 10 * k_mic}
+    print(f"# {n}^3 fpcg_alone: {res['fpcg_alone']}", file=sys.stderr, flush=True)
+    net_ms, net_calls = kt.get("learned", (0.0, 0))
+    network = None
+    if net_calls > 0 and net_ms > 0:
+        fl = learned.flops_per_apply(sc.grid.shape)
+        tf = fl / (net_ms / net_calls * 1e-3) / 1e12
+        network = {"bound": "mfma", "achieved": round(tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4), "flops_per_apply": fl,
+                   "ms_per_apply": round(net_ms / net_calls, 4), "applies": net_calls}
+    mic_ms, l_ms = res["mic0_pcg"]["solve_ms"], res["learned"]["solve_ms"]
+    return {"grid": [n] * 3, "workload": f"dam break {n}^3 APIC with a static sphere, 4th step, "
+                                          "same right-hand side for both solvers",
+            "mic0_pcg": res["mic0_pcg"], "learned": res["learned"], "fpcg_alone": res["fpcg_alone"],
+            "learned_over_mic0_time": round(l_ms / mic_ms, 3) if mic_ms > 0 else None,
+            "training": {"samples": len(B), "grid": [train_n] * 3, "epochs": epochs,
+                         "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+                         "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]},
+            "network": network}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +220,12 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device-to-device copy of known size (for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the pressure-solve comparison MIC(0)-PCG vs learned FPCG")
+    ap.add_argument("--learned-size", type=int, default=128)
+    ap.add_argument("--learned-train-size", type=int, default=32)
+    ap.add_argument("--learned-scenes", type=int, default=6)
+    ap.add_argument("--learned-epochs", type=int, default=20)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -194,6 +287,11 @@ def main():
     print(f"# per-step kernel ms: {json.dumps(breakdown)}", file=sys.stderr, flush=True)
     sim.close()
 
+    solve_cmp = None
+    if not args.no_learned:
+        solve_cmp = learned_solve_compare(args.learned_size, args.learned_train_size, args.learned_scenes,
+                                          args.learned_epochs)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, its, args.cpu_threads, args.cpu_pcg_cap)
@@ -217,6 +315,7 @@ def main():
                    "pcg_iterations": round(its, 1), "pressure_solve_ms": round(solve_ms, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "pressure_solve": solve_cmp,
         "kernel_ms_per_step": breakdown,
     }
     print(json.dumps(out), flush=True)

@@ -66,91 +66,138 @@ int64_t learned_num_params() {
 
 // ---- implicit-GEMM 3x3x3 convolution on MFMA ------------------------------------------
 // Y[v][co] = bias[co] + sum_{t, ci} X[v + off(t)][ci] * Wt[co][t * Cin + ci]
-// One wave: 16 consecutive voxels x all Cout channels (NT tiles of 16), K in
-// chunks of 32 with v_mfma_f32_16x16x32_bf16.  Operand lanes: A (16 x 32):
-// lane l holds row l % 16, k = 8 (l / 16) .. + 7; B (32 x 16): lane l holds
-// column l % 16, the same k run; D (16 x 16): lane l holds rows
-// 4 (l / 16) .. + 3 of column l % 16.  Out-of-grid taps read zeros.
+// One wave: MT tiles of 16 consecutive voxels x NT tiles of 16 output
+// channels (output channel group blockIdx.y), K = 27 * CIN in chunks of 32
+// with v_mfma_f32_16x16x32_bf16.  Operand lanes: A (16 x 32): lane l holds
+// row l % 16, k = 8 (l / 16) .. + 7; B (32 x 16): lane l holds column l % 16,
+// the same k run; D (16 x 16): lane l holds rows 4 (l / 16) .. + 3 of column
+// l % 16.  A K run of 8 is 8 channels of one tap (channel counts are
+// multiples of 8), so each operand is one 16-byte load; a chunk's tap
+// geometry is computed once for all MT tiles, each B fragment feeds MT MFMAs
+// and each A fragment NT.  Out-of-grid taps read zeros.
+// With CU > 0 the input is cat(up(coarse), skip) without materialising it:
+// channels [0, CU) are read from the coarse parent voxel (x/2, y/2, z/2) of
+// xc, channels [CU, CIN) from x (skip, CIN - CU channels) -- the same BF16
+// values a concatenated copy would hold.
 struct ConvArgs {
-    const __bf16* x;      // [V][cin]
+    const __bf16* x;      // [V][cin] (or the skip part [V][cin - cu])
     const __bf16* w;      // [cout][27 * cin]
     const float* bias;    // [cout] or null
     void* y;              // [V][cout], bf16 or float
     int D, H, W;          // grid of this level
     int cin, cout;
     int relu;
+    const __bf16* xc;     // coarse parent [V/8][cu] when cu > 0
+    int cu;
 };
 
-template <int NT, bool OUT_F32>
+template <int CIN, int CU, int NT, int MT, bool OUT_F32>
 __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
+    constexpr int KTOT = 27 * CIN;
+    constexpr int NCH = (KTOT + 31) / 32;
+    constexpr int CS = CIN - CU;   // channels read from x
     const int lane = threadIdx.x & 63;
     const int64_t V = (int64_t)a.D * a.H * a.W;
-    const int64_t m0 = ((int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 16;
+    const int64_t m0 = ((int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * (16 * MT);
     if (m0 >= V) return;   // wave-uniform
-    const int64_t vr = m0 + (lane & 15);   // this lane's A row (voxel)
-    const bool rin = vr < V;
-    const int z = rin ? (int)(vr % a.W) : 0;
-    const int y = rin ? (int)((vr / a.W) % a.H) : 0;
-    const int x = rin ? (int)(vr / ((int64_t)a.W * a.H)) : 0;
+    const int co0 = 16 * NT * (int)blockIdx.y;
     const int kq = lane >> 4;   // which 8-run of the 32-wide K chunk
-    const int ktot = 27 * a.cin;
-    f32x4 acc[NT];
+    const __bf16* xl[MT];       // this lane's A row (voxel) of each tile in x, at channel 0
+    int xs[MT], ys[MT], zs[MT];
+    bool rin[MT];
 #pragma unroll
-    for (int j = 0; j < NT; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < ktot; kc += 32) {
-        const int k0 = kc + 8 * kq;
-        bf16x8 av;
-        bool kin = k0 < ktot;
-        if (kin && rin) {
-            const int t = k0 / a.cin, ci = k0 - t * a.cin;
-            const int xx = x + t / 9 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t % 3 - 1;
-            if (xx >= 0 && xx < a.D && yy >= 0 && yy < a.H && zz >= 0 && zz < a.W) {
-                const int64_t nv = ((int64_t)xx * a.H + yy) * a.W + zz;
-                av = *reinterpret_cast<const bf16x8*>(a.x + nv * a.cin + ci);
-            } else {
-                av = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            }
-        } else {
-            av = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < MT; m++) {
+        const int64_t vr = m0 + 16 * m + (lane & 15);
+        rin[m] = vr < V;
+        const int64_t v = rin[m] ? vr : 0;
+        zs[m] = (int)(v % a.W);
+        ys[m] = (int)((v / a.W) % a.H);
+        xs[m] = (int)(v / ((int64_t)a.W * a.H));
+        xl[m] = a.x + v * CS;
+    }
+    const int Hc = a.H >> 1, Wc = a.W >> 1;
+    const __bf16* wl = a.w + (int64_t)(co0 + (lane & 15)) * KTOT + 8 * kq;
+    const bf16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int c = 0; c < NCH; c++) {
+        const int k0 = 32 * c + 8 * kq;
+        const bool kin = (KTOT % 32 == 0) || k0 < KTOT;
+        const int t = k0 / CIN, ci = k0 - t * CIN;
+        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        const bool up = CU > 0 && ci < CU;
+        const int64_t toff = (((int64_t)dx * a.H + dy) * a.W + dz) * CS + (ci - CU);
+        bf16x8 av[MT], bv[NT];
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const int xx = xs[m] + dx, yy = ys[m] + dy, zz = zs[m] + dz;
+            const bool ok = kin && rin[m] && (unsigned)xx < (unsigned)a.D && (unsigned)yy < (unsigned)a.H &&
+                            (unsigned)zz < (unsigned)a.W;
+            const __bf16* src = xl[m] + toff;
+            if (up) src = a.xc + (((int64_t)(xx >> 1) * Hc + (yy >> 1)) * Wc + (zz >> 1)) * (CU > 0 ? CU : 1) + ci;
+            av[m] = ok ? *reinterpret_cast<const bf16x8*>(src) : zero;
         }
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const int co = 16 * j + (lane & 15);
-            bf16x8 bv = kin ? *reinterpret_cast<const bf16x8*>(a.w + (int64_t)co * ktot + k0)
-                            : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
-        }
+        for (int j = 0; j < NT; j++)
+            bv[j] = kin ? *reinterpret_cast<const bf16x8*>(wl + (int64_t)16 * j * KTOT + 32 * c) : zero;
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int j = 0; j < NT; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[j], acc[m][j], 0, 0, 0);
     }
 #pragma unroll
     for (int j = 0; j < NT; j++) {
-        const int co = 16 * j + (lane & 15);
+        const int co = co0 + 16 * j + (lane & 15);
         const float b = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int64_t v = m0 + 4 * kq + i;
-            if (v >= V) continue;
-            float o = acc[j][i] + b;
-            if (a.relu) o = fmaxf(o, 0.f);
-            if (OUT_F32) static_cast<float*>(a.y)[v * a.cout + co] = o;
-            else static_cast<__bf16*>(a.y)[v * a.cout + co] = (__bf16)o;
-        }
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t v = m0 + 16 * m + 4 * kq + i;
+                if (v >= V) continue;
+                float o = acc[m][j][i] + b;
+                if (a.relu) o = fmaxf(o, 0.f);
+                if (OUT_F32) static_cast<float*>(a.y)[v * a.cout + co] = o;
+                else static_cast<__bf16*>(a.y)[v * a.cout + co] = (__bf16)o;
+            }
     }
 }
 
-template <bool OUT_F32>
-static void launch_conv3(hipStream_t st, const ConvArgs& a) {
+template <int CIN, int CU, int COUT, int NT, int MT, bool OUT_F32>
+static void conv3_launch(hipStream_t st, const ConvArgs& a) {
+    static_assert(COUT % (16 * NT) == 0 && CIN % 8 == 0 && CU % 8 == 0, "conv3 tile shape");
     const int64_t V = (int64_t)a.D * a.H * a.W;
-    const dim3 grid((unsigned)((V + 63) / 64)), block(256);
-    switch (a.cout / 16) {
-    case 1: hipLaunchKernelGGL((k_conv3<1, OUT_F32>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_conv3<2, OUT_F32>), grid, block, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_conv3<3, OUT_F32>), grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_conv3<4, OUT_F32>), grid, block, 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_conv3<6, OUT_F32>), grid, block, 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_conv3<8, OUT_F32>), grid, block, 0, st, a); break;
-    case 12: hipLaunchKernelGGL((k_conv3<12, OUT_F32>), grid, block, 0, st, a); break;
-    default: break;   // channel counts are fixed by kLayers
+    const int64_t waves = (V + 16 * MT - 1) / (16 * MT);
+    hipLaunchKernelGGL((k_conv3<CIN, CU, NT, MT, OUT_F32>), dim3((unsigned)((waves + 3) / 4), COUT / (16 * NT)),
+                       dim3(256), 0, st, a);
+}
+
+// The convolutions of the network, by (Cin, Cu, Cout) with their tile shapes
+// (NT output tiles x MT voxel tiles per wave): the 7 layers' forward
+// convolutions (BF16 out; the decoder reads cat(up, skip) directly), and the
+// data gradients of layers 1-6 (transposed: Cin = the layer's Cout; FP32
+// out).  Returns false for any other shape.
+template <bool OUT_F32>
+static bool launch_conv3(hipStream_t st, const ConvArgs& a) {
+#define FSIM_CONV3(CI_, CU_, CO_, NT_, MT_)                     \
+    if (a.cin == CI_ && a.cu == CU_ && a.cout == CO_) {         \
+        conv3_launch<CI_, CU_, CO_, NT_, MT_, OUT_F32>(st, a);  \
+        return true;                                            \
     }
+    if constexpr (!OUT_F32) {
+        FSIM_CONV3(8, 0, 16, 1, 4) FSIM_CONV3(16, 0, 32, 2, 4) FSIM_CONV3(32, 0, 64, 4, 2)
+        FSIM_CONV3(64, 0, 128, 2, 1) FSIM_CONV3(192, 128, 64, 4, 2) FSIM_CONV3(96, 64, 32, 2, 4)
+        FSIM_CONV3(48, 32, 16, 1, 4)
+    } else {
+        FSIM_CONV3(32, 0, 16, 1, 4) FSIM_CONV3(64, 0, 32, 2, 4) FSIM_CONV3(128, 0, 64, 4, 2)
+        FSIM_CONV3(64, 0, 192, 4, 2) FSIM_CONV3(32, 0, 96, 2, 4) FSIM_CONV3(16, 0, 48, 1, 4)
+    }
+#undef FSIM_CONV3
+    return false;
 }
 
 // ---- elementwise pieces -------------------------------------------------------------
@@ -522,8 +569,7 @@ static int learned_alloc(fsim_ctx* ctx) {
         (rc = nalloc(ctx, N, &N->e[1], V1 * 32)) || (rc = nalloc(ctx, N, &N->e[2], V2 * 64)) ||
         (rc = nalloc(ctx, N, &N->e[3], V3 * 128)) || (rc = nalloc(ctx, N, &N->pl[0], V1 * 16)) ||
         (rc = nalloc(ctx, N, &N->pl[1], V2 * 32)) || (rc = nalloc(ctx, N, &N->pl[2], V3 * 64)) ||
-        (rc = nalloc(ctx, N, &N->cat[2], V2 * 192)) || (rc = nalloc(ctx, N, &N->cat[1], V1 * 96)) ||
-        (rc = nalloc(ctx, N, &N->cat[0], V0 * 48)) || (rc = nalloc(ctx, N, &N->dd[2], V2 * 64)) ||
+        (rc = nalloc(ctx, N, &N->dd[2], V2 * 64)) ||
         (rc = nalloc(ctx, N, &N->dd[1], V1 * 32)) || (rc = nalloc(ctx, N, &N->dd[0], V0 * 16)) ||
         (rc = nalloc(ctx, N, &N->out, V0)) || (rc = nalloc(ctx, N, &N->scale, 1)) ||
         (rc = nalloc(ctx, N, &N->scale_bits, 1)))
@@ -542,7 +588,9 @@ static int learned_alloc_train(fsim_ctx* ctx) {
         (rc = nalloc(ctx, N, &N->in_flat, ctx->g.nc)) || (rc = nalloc(ctx, N, &N->label_flat, ctx->g.nc)) ||
         (rc = nalloc(ctx, N, &N->target, V0)) || (rc = nalloc(ctx, N, &N->fluid, V0)) ||
         (rc = nalloc(ctx, N, &N->dout, V0)) || (rc = nalloc(ctx, N, &N->dd0, V0 * 16)) ||
-        (rc = nalloc(ctx, N, &N->loss_parts, 1024)) || (rc = nalloc(ctx, N, &N->wflip, 64 * 27 * 192)))
+        (rc = nalloc(ctx, N, &N->loss_parts, 1024)) || (rc = nalloc(ctx, N, &N->wflip, 64 * 27 * 192)) ||
+        (rc = nalloc(ctx, N, &N->cat[2], V2 * 192)) || (rc = nalloc(ctx, N, &N->cat[1], V1 * 96)) ||
+        (rc = nalloc(ctx, N, &N->cat[0], V0 * 48)))
         return rc;
     for (int l = 0; l < kNetLayers; l++) {
         const int64_t n = Vl[kLayers[l].level] * kLayers[l].cout;
@@ -568,35 +616,41 @@ static int learned_alloc_train(fsim_ctx* ctx) {
     return FSIM_OK;
 }
 
-static ConvArgs conv_args(const LearnedNet* N, int l, const __bf16* x, void* y) {
+// layer l reading x (or, for a decoder layer, the skip x and the coarse parent xc)
+static ConvArgs conv_args(const LearnedNet* N, int l, const __bf16* x, void* y, const __bf16* xc = nullptr) {
     const Dims d = N->lv[kLayers[l].level];
-    return ConvArgs{x, N->w16 + N->woff[l], N->w32 + N->boff[l], y, d.D, d.H, d.W, kLayers[l].cin, kLayers[l].cout, 1};
+    const int cu = xc ? kLayers[l].cin - kLayers[l].cin / 3 : 0;
+    return ConvArgs{x, N->w16 + N->woff[l], N->w32 + N->boff[l], y, d.D, d.H, d.W, kLayers[l].cin, kLayers[l].cout, 1,
+                    xc, cu};
 }
 
-// forward from x0 (already packed) to out
-static int net_forward(fsim_ctx* ctx) {
+// forward from x0 (already packed) to out; for training the concatenated
+// decoder inputs are also stored (the weight gradients read them)
+static int net_forward(fsim_ctx* ctx, bool keep_cat) {
     LearnedNet* N = ctx->learned;
     hipStream_t st = ctx->stream;
     const Dims* L = N->lv;
-    launch_conv3<false>(st, conv_args(N, 0, N->x0, N->e[0]));
+    bool ok = launch_conv3<false>(st, conv_args(N, 0, N->x0, N->e[0]));
     for (int l = 1; l < 4; l++) {
         hipLaunchKernelGGL(k_pool2, dim3(gb(L[l].V() * kLayers[l].cin)), dim3(256), 0, st, N->e[l - 1], N->pl[l - 1],
                            L[l - 1], kLayers[l].cin);
-        launch_conv3<false>(st, conv_args(N, l, N->pl[l - 1], N->e[l]));
+        ok = ok && launch_conv3<false>(st, conv_args(N, l, N->pl[l - 1], N->e[l]));
     }
     // decoder: layer 4 at level 2 (up e3 + e2), 5 at level 1 (up d2 + e1), 6 at level 0 (up d1 + e0)
     const __bf16* coarse = N->e[3];
     for (int l = 4; l < kNetLayers; l++) {
         const int lev = kLayers[l].level;
         const int Cs = kLayers[l].cin / 3, Cu = kLayers[l].cin - Cs;
-        hipLaunchKernelGGL(k_upcat, dim3(gb(L[lev].V() * kLayers[l].cin)), dim3(256), 0, st, coarse, N->e[lev], N->cat[lev],
-                           L[lev], Cu, Cs);
-        launch_conv3<false>(st, conv_args(N, l, N->cat[lev], N->dd[lev]));
+        if (keep_cat)
+            hipLaunchKernelGGL(k_upcat, dim3(gb(L[lev].V() * kLayers[l].cin)), dim3(256), 0, st, coarse, N->e[lev],
+                               N->cat[lev], L[lev], Cu, Cs);
+        ok = ok && launch_conv3<false>(st, conv_args(N, l, N->e[lev], N->dd[lev], coarse));
         coarse = N->dd[lev];
     }
     hipLaunchKernelGGL(k_conv1_fwd, dim3(gb(L[0].V())), dim3(256), 0, st, N->dd[0], N->w32 + N->out_w, N->w32 + N->out_b,
                        N->out, L[0].V());
     FSIM_HIP(ctx, hipGetLastError());
+    if (!ok) return set_err(ctx, FSIM_ERR_INVALID, "learned network: convolution shape without a kernel");
     return FSIM_OK;
 }
 
@@ -666,7 +720,7 @@ int learned_apply(fsim_ctx* ctx, const float* r, float* z) {
     const float* sc = reinterpret_cast<const float*>(N->scale_bits);   // the max as a float
     hipLaunchKernelGGL(k_net_input, dim3(gb(N->lv[0].V())), dim3(256), 0, st, r, ctx->nbinfo, 1, nullptr, sc, N->x0,
                        N->lv[0], ctx->g.nx, ctx->g.ny, ctx->g.nz);
-    FSIM_TRY(net_forward(ctx));
+    FSIM_TRY(net_forward(ctx, false));
     hipLaunchKernelGGL(k_net_output, dim3(gb(nc)), dim3(256), 0, st, N->out, ctx->nbinfo, sc, z, N->lv[0], ctx->g.nx,
                        ctx->g.ny, ctx->g.nz);
     FSIM_HIP(ctx, hipGetLastError());
@@ -717,7 +771,7 @@ int learned_forward_host(fsim_ctx* ctx, const float* b, const uint8_t* label, fl
     FSIM_TRY(learned_alloc_train(ctx));
     float s;
     FSIM_TRY(upload_sample(ctx, b, label, nullptr, &s));
-    FSIM_TRY(net_forward(ctx));
+    FSIM_TRY(net_forward(ctx, false));
     const Dims d = N->lv[0];
     std::vector<float> o((size_t)d.V());
     FSIM_HIP(ctx, hipMemcpyAsync(o.data(), N->out, sizeof(float) * o.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -754,8 +808,9 @@ static int dgrad(fsim_ctx* ctx, int l, const __bf16* g16, float* dx) {
     const int cin = kLayers[l].cin, cout = kLayers[l].cout;
     hipLaunchKernelGGL(k_flip_weights, dim3(gb((int64_t)cout * 27 * cin)), dim3(256), 0, ctx->stream, N->w16 + N->woff[l],
                        N->wflip, cin, cout);
-    ConvArgs a{g16, N->wflip, nullptr, dx, d.D, d.H, d.W, cout, cin, 0};
-    launch_conv3<true>(ctx->stream, a);
+    ConvArgs a{g16, N->wflip, nullptr, dx, d.D, d.H, d.W, cout, cin, 0, nullptr, 0};
+    if (!launch_conv3<true>(ctx->stream, a))
+        return set_err(ctx, FSIM_ERR_INVALID, "learned network: convolution shape without a kernel");
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
@@ -825,7 +880,7 @@ int learned_grad_host(fsim_ctx* ctx, const float* b, const uint8_t* label, const
     LearnedNet* N = ctx->learned;
     float s;
     FSIM_TRY(upload_sample(ctx, b, label, p, &s));
-    FSIM_TRY(net_forward(ctx));
+    FSIM_TRY(net_forward(ctx, true));
     FSIM_TRY(net_backward(ctx, loss));
     FSIM_HIP(ctx, hipMemcpyAsync(grad, N->grad, sizeof(float) * N->nparams, hipMemcpyDeviceToHost, ctx->stream));
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -851,7 +906,7 @@ int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* lab
         float s;
         double l;
         FSIM_TRY(upload_sample(ctx, b + k * nc, label + k * nc, p + k * nc, &s));
-        FSIM_TRY(net_forward(ctx));
+        FSIM_TRY(net_forward(ctx, true));
         FSIM_TRY(net_backward(ctx, &l));
         lsum += l;
         hipLaunchKernelGGL(k_axpy_f, dim3(gb(N->nparams)), dim3(256), 0, ctx->stream, N->grad_acc, N->grad, N->nparams);

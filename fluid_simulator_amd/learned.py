@@ -15,6 +15,21 @@ from . import capi, scenes
 
 DT = 1.0 / 240.0
 
+# (input channels, output channels, level) of the 3x3x3 layers, in order
+# (fsim_learned.hip kLayers); the 1x1 output layer maps 16 channels to 1.
+LAYERS = [(8, 16, 0), (16, 32, 1), (32, 64, 2), (64, 128, 3), (192, 64, 2), (96, 32, 1), (48, 16, 0)]
+
+
+def flops_per_apply(shape) -> float:
+    """Multiply-add FLOPs (2 per MAC) of one network evaluation on a grid of `shape` cells, as executed:
+    every dimension padded to a multiple of 8, all 8 input channels counted."""
+    d0 = [(n + 7) // 8 * 8 for n in shape]
+    total = 0.0
+    for cin, cout, lev in LAYERS:
+        vox = float(np.prod([n >> lev for n in d0]))
+        total += 2.0 * 27 * cin * cout * vox
+    return total + 2.0 * 16 * float(np.prod(d0))
+
 
 def random_blocks(rng: np.random.Generator, n_max: int = 3):
     blocks = []
