@@ -127,21 +127,28 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16
 def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int) -> dict:
     """Pressure-solve ms to 1e-6 relative residual, MIC(0)-PCG vs the learned FPCG, on the same right-hand side.
 
-    The network is trained here on the GPU (generate_samples + train: real MIC(0)-PCG solves of seeded
-    random dam breaks at train_n^3 and residual pairs), then set on an n^3 context.  The n^3 scene is the
-    APIC dam break with a static sphere; it is stepped 3 times with MIC(0)-PCG, its particle state is
-    saved, and the next step is solved from that same state with each preconditioner (the second of two
-    runs is kept, so nothing is allocated in the timed one).  `learned` is the production path: FPCG
-    with the MIC(0)-PCG fallback after twice the last MIC(0) iteration count.  `fpcg_alone` is the
-    same FPCG on a fresh context, whose budget is max_iterations = 10 x the MIC(0) count."""
+    The network is trained here on the GPU with the energy-norm loss (generate_samples + train: real
+    MIC(0)-PCG solves of seeded random dam breaks at train_n^3 and residual pairs), then set on an n^3
+    context.  The n^3 scene is the APIC dam break with a static sphere; it is stepped 3 times with
+    MIC(0)-PCG, its particle state is saved, and the next step is solved from that same state with each
+    preconditioner (the second of two runs is timed, so nothing is allocated in it).  `learned` is the
+    production path: FPCG with the MIC(0)-PCG fallback after twice the last MIC(0) iteration count.
+    `fpcg_iterations` compares the FPCG iteration counts, on fresh contexts with a budget of 10x the MIC(0)
+    count, of the trained network, the untrained (initial) network and an all-zero network (the same
+    solver with the damped-Jacobi steps alone)."""
     t0 = time.perf_counter()
-    tgrid, B, L, P = learned.generate_samples(train_n, train_scenes, 4, seed=0, residuals_per_solve=6)
+    tgrid, B, L, P = learned.generate_samples(train_n, train_scenes, 4, seed=0, residuals_per_solve=4)
     t_gen = time.perf_counter() - t0
+    init = capi.Simulation(tgrid, scenes.params())
+    init.learned_init(1)
+    untrained = init.learned_get_params()
+    init.close()
     t0 = time.perf_counter()
-    params, hist = learned.train(tgrid, B, L, P, epochs=epochs, batch=4, lr=1e-3, seed=1)
+    params, hist = learned.train(tgrid, B, L, P, epochs=epochs, batch=4, lr=1e-3, seed=1, params=untrained,
+                                 loss=learned.LOSS_ENERGY)
     t_train = time.perf_counter() - t0
     print(f"# learned: {len(B)} samples at {train_n}^3 in {t_gen:.1f}s, {epochs} epochs in {t_train:.1f}s, "
-          f"loss {hist[0]:.3e} -> {hist[-1]:.3e}", file=sys.stderr, flush=True)
+          f"energy loss {hist[0]:.3e} -> {hist[-1]:.3e}", file=sys.stderr, flush=True)
 
     sc = scenes.CONFIG_128_APIC_SPHERE() if n == 128 else scenes.dam_break(n, capi.APIC, seed=0, sphere=True)
     sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
@@ -177,16 +184,19 @@ def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int) 
     kt = solve("learned", capi.PRECOND_LEARNED, params, profile=True)
     k_mic = res["mic0_pcg"]["pcg_iterations"]
     sim.close()
-    alone = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED,
-                                                     max_iterations=10 * k_mic))
-    alone.set_obstacles(sc.obstacles)
-    alone.learned_set_params(params)
-    alone.set_particles(*state)
-    st = alone.step(scenes.DT)
-    alone.close()
-    res["fpcg_alone"] = {"learned_iterations": st.learned_iterations, "learned_stop": st.learned_stop,
-                         "budget": 10 * k_mic}
-    print(f"# {n}^3 fpcg_alone: {res['fpcg_alone']}", file=sys.stderr, flush=True)
+    iters = {}
+    for name, prm in (("trained", params), ("untrained", untrained), ("zero_network", np.zeros_like(params))):
+        alone = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED,
+                                                         max_iterations=10 * k_mic))
+        alone.set_obstacles(sc.obstacles)
+        alone.learned_set_params(prm)
+        alone.set_particles(*state)
+        st = alone.step(scenes.DT)
+        alone.close()
+        iters[name] = st.learned_iterations if st.learned_stop == 0 else None
+    iters["budget"] = 10 * k_mic
+    iters["mic0_pcg"] = k_mic
+    print(f"# {n}^3 FPCG iterations (None: not converged within the budget): {iters}", file=sys.stderr, flush=True)
     net_ms, net_calls = kt.get("learned", (0.0, 0))
     network = None
     if net_calls > 0 and net_ms > 0:
@@ -194,13 +204,14 @@ def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int) 
         tf = fl / (net_ms / net_calls * 1e-3) / 1e12
         network = {"bound": "mfma", "achieved": round(tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4), "flops_per_apply": fl,
-                   "ms_per_apply": round(net_ms / net_calls, 4), "applies": net_calls}
+                   "ms_per_apply": round(net_ms / net_calls, 4), "applies": net_calls,
+                   "note": "ms_per_apply covers the network and its damped-Jacobi steps"}
     mic_ms, l_ms = res["mic0_pcg"]["solve_ms"], res["learned"]["solve_ms"]
     return {"grid": [n] * 3, "workload": f"dam break {n}^3 APIC with a static sphere, 4th step, "
                                           "same right-hand side for both solvers",
-            "mic0_pcg": res["mic0_pcg"], "learned": res["learned"], "fpcg_alone": res["fpcg_alone"],
+            "mic0_pcg": res["mic0_pcg"], "learned": res["learned"], "fpcg_iterations": iters,
             "learned_over_mic0_time": round(l_ms / mic_ms, 3) if mic_ms > 0 else None,
-            "training": {"samples": len(B), "grid": [train_n] * 3, "epochs": epochs,
+            "training": {"samples": len(B), "grid": [train_n] * 3, "epochs": epochs, "loss": "energy norm",
                          "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
                          "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]},
             "network": network}
@@ -224,8 +235,8 @@ def main():
                     help="skip the pressure-solve comparison MIC(0)-PCG vs learned FPCG")
     ap.add_argument("--learned-size", type=int, default=128)
     ap.add_argument("--learned-train-size", type=int, default=32)
-    ap.add_argument("--learned-scenes", type=int, default=6)
-    ap.add_argument("--learned-epochs", type=int, default=20)
+    ap.add_argument("--learned-scenes", type=int, default=30)
+    ap.add_argument("--learned-epochs", type=int, default=15)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()

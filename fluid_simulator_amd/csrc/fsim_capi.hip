@@ -507,6 +507,14 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
         return FSIM_OK;
     case FSIM_OPT_PROFILE:
         return set_profiling(ctx, value != 0);
+    case FSIM_OPT_LEARNED_SMOOTHING:
+        if (value < 0 || value > 16) return set_err(ctx, FSIM_ERR_INVALID, "learned smoothing steps are 0 .. 16");
+        ctx->learned_smoothing = value;
+        return FSIM_OK;
+    case FSIM_OPT_LEARNED_LOSS:
+        if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "learned loss is 0 (MSE) or 1 (energy)");
+        ctx->learned_loss = value;
+        return FSIM_OK;
     default: return set_err(ctx, FSIM_ERR_INVALID, "unknown option");
     }
 }

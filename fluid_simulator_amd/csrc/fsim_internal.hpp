@@ -206,6 +206,8 @@ struct fsim_ctx {
     uint32_t wave_epoch = 0;
     int* wave_err = nullptr;
     int sweep_mode = 0;                          // 0 wavefront, 1 one launch per plane
+    int learned_smoothing = 4;                   // damped-Jacobi steps after the network
+    int learned_loss = 0;                        // training loss: 0 masked MSE, 1 energy norm
 
     // obstacles
     std::vector<fsim_obstacle> obs;

@@ -494,6 +494,48 @@ __global__ void k_mse(const float* __restrict__ out, const float* __restrict__ t
     if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0] * inv_n;
 }
 
+// Energy-norm loss: loss = e^T A e / t^T A t with e = (out - t) on WATER cells,
+// dout = 2 A e / t^T A t.  A is the unscaled 7-point pressure matrix on WATER
+// cells (diagonal = cnt, the number of non-SOLID neighbours; -1 towards WATER
+// neighbours; AIR neighbours are Dirichlet zeros).  Step 1: e into a buffer.
+__global__ void k_energy_err(const float* __restrict__ out, const float* __restrict__ target,
+                             const uint8_t* __restrict__ fluid, int64_t V, float* __restrict__ e) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) e[v] = fluid[v] ? out[v] - target[v] : 0.f;
+}
+// Step 2: A e (e is zero off WATER, so all six neighbours may be summed).
+__global__ void k_energy(const float* __restrict__ e, const uint8_t* __restrict__ cnt, Dims d, float inv_et,
+                         float* __restrict__ dout, float* __restrict__ loss_parts) {
+    __shared__ float red[256];
+    const int64_t V = d.V();
+    float s = 0.f;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const int c = cnt[v];
+        float ae = 0.f;
+        if (c) {
+            const int z = (int)(v % d.W), y = (int)((v / d.W) % d.H), x = (int)(v / ((int64_t)d.W * d.H));
+            const int64_t sx = (int64_t)d.H * d.W;
+            float nb = 0.f;
+            if (x > 0) nb += e[v - sx];
+            if (x < d.D - 1) nb += e[v + sx];
+            if (y > 0) nb += e[v - d.W];
+            if (y < d.H - 1) nb += e[v + d.W];
+            if (z > 0) nb += e[v - 1];
+            if (z < d.W - 1) nb += e[v + 1];
+            ae = (float)c * e[v] - nb;
+        }
+        dout[v] = 2.f * ae * inv_et;
+        s += e[v] * ae;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0] * inv_et;
+}
+
 // Adam on FP32 master weights; BF16 copy of the conv weights for the MFMA kernels
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2, float gscale) {
@@ -527,6 +569,9 @@ struct LearnedNet {
     bool train_ready = false;
     float *in_flat = nullptr, *target = nullptr, *dout = nullptr, *dd0 = nullptr, *loss_parts = nullptr;
     uint8_t *label_flat = nullptr, *fluid = nullptr;
+    uint8_t* cnt = nullptr;          // energy loss: non-SOLID neighbour count of WATER voxels, 0 elsewhere
+    float* eres = nullptr;           // energy loss: out - target on WATER
+    double energy_t = 0.0;           // energy loss: t^T A t of the uploaded sample
     float* g32[kNetLayers] = {};
     __bf16* g16[kNetLayers] = {};
     float *dcat[3] = {}, *dcoarse[3] = {}, *dskip[3] = {}, *dpool[3] = {};
@@ -590,7 +635,8 @@ static int learned_alloc_train(fsim_ctx* ctx) {
         (rc = nalloc(ctx, N, &N->dout, V0)) || (rc = nalloc(ctx, N, &N->dd0, V0 * 16)) ||
         (rc = nalloc(ctx, N, &N->loss_parts, 1024)) || (rc = nalloc(ctx, N, &N->wflip, 64 * 27 * 192)) ||
         (rc = nalloc(ctx, N, &N->cat[2], V2 * 192)) || (rc = nalloc(ctx, N, &N->cat[1], V1 * 96)) ||
-        (rc = nalloc(ctx, N, &N->cat[0], V0 * 48)))
+        (rc = nalloc(ctx, N, &N->cat[0], V0 * 48)) || (rc = nalloc(ctx, N, &N->cnt, V0)) ||
+        (rc = nalloc(ctx, N, &N->eres, V0)))
         return rc;
     for (int l = 0; l < kNetLayers; l++) {
         const int64_t n = Vl[kLayers[l].level] * kLayers[l].cout;
@@ -746,8 +792,13 @@ static int upload_sample(fsim_ctx* ctx, const float* b, const uint8_t* label, co
     if (p) {
         const Dims d = N->lv[0];
         std::vector<float> t((size_t)d.V(), 0.f);
-        std::vector<uint8_t> f((size_t)d.V(), 0);
+        std::vector<uint8_t> f((size_t)d.V(), 0), cn((size_t)d.V(), 0);
         const float inv = s > 0.f ? 1.f / s : 0.f;
+        auto lab = [&](int x, int y, int z) -> int {   // cells outside the grid count as SOLID
+            if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return FSIM_SOLID;
+            return label[((int64_t)x * ny + y) * nz + z];
+        };
+        static const int nb6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
         for (int x = 0; x < nx; x++)
             for (int y = 0; y < ny; y++)
                 for (int z = 0; z < nz; z++) {
@@ -755,9 +806,29 @@ static int upload_sample(fsim_ctx* ctx, const float* b, const uint8_t* label, co
                     if (label[i] != FSIM_WATER) continue;
                     f[(size_t)v] = 1;
                     t[(size_t)v] = (float)p[i] * inv;
+                    int c = 0;
+                    for (const auto& o : nb6) c += lab(x + o[0], y + o[1], z + o[2]) != FSIM_SOLID;
+                    cn[(size_t)v] = (uint8_t)c;
                 }
+        // t^T A t in FP64 over the WATER cells
+        double et = 0.0;
+        for (int x = 0; x < nx; x++)
+            for (int y = 0; y < ny; y++)
+                for (int z = 0; z < nz; z++) {
+                    const int64_t v = ((int64_t)x * d.H + y) * d.W + z;
+                    if (!f[(size_t)v]) continue;
+                    double at = (double)cn[(size_t)v] * t[(size_t)v];
+                    for (const auto& o : nb6) {
+                        const int xx = x + o[0], yy = y + o[1], zz = z + o[2];
+                        if (lab(xx, yy, zz) == FSIM_WATER) at -= t[(size_t)(((int64_t)xx * d.H + yy) * d.W + zz)];
+                    }
+                    et += (double)t[(size_t)v] * at;
+                }
+        N->energy_t = et;
         FSIM_HIP(ctx, hipMemcpyAsync(N->target, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, ctx->stream));
         FSIM_HIP(ctx, hipMemcpyAsync(N->fluid, f.data(), f.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (N->cnt)
+            FSIM_HIP(ctx, hipMemcpyAsync(N->cnt, cn.data(), cn.size(), hipMemcpyHostToDevice, ctx->stream));
         // the host vectors go out of scope: wait for the copies
         FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
@@ -830,7 +901,14 @@ static int net_backward(fsim_ctx* ctx, double* loss) {
     }
     const float inv_n = nfl > 0 ? 1.f / (float)nfl : 0.f;
     const int nlb = 256;
-    hipLaunchKernelGGL(k_mse, dim3(nlb), dim3(256), 0, st, N->out, N->target, N->fluid, V0, inv_n, N->dout, N->loss_parts);
+    if (ctx->learned_loss == 1) {
+        const float inv_et = N->energy_t > 0.0 ? (float)(1.0 / N->energy_t) : 0.f;
+        hipLaunchKernelGGL(k_energy_err, dim3(gb(V0)), dim3(256), 0, st, N->out, N->target, N->fluid, V0, N->eres);
+        hipLaunchKernelGGL(k_energy, dim3(nlb), dim3(256), 0, st, N->eres, N->cnt, L[0], inv_et, N->dout, N->loss_parts);
+    } else {
+        hipLaunchKernelGGL(k_mse, dim3(nlb), dim3(256), 0, st, N->out, N->target, N->fluid, V0, inv_n, N->dout,
+                           N->loss_parts);
+    }
     // output layer
     hipLaunchKernelGGL(k_conv1_bwd_data, dim3(gb(V0 * kOutIn)), dim3(256), 0, st, N->dout, N->w32 + N->out_w, N->dd0, V0);
     hipLaunchKernelGGL(k_conv1_bwd_weight, dim3(kOutIn + 1), dim3(256), 0, st, N->dout, N->dd[0], V0, N->grad + N->out_w);

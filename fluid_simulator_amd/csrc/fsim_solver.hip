@@ -869,17 +869,24 @@ static bool stop_met(const fsim_ctx* ctx, double ss, double mx, double bnorm) {
     return mx < ctx->prm.residual_tolerance;
 }
 
-// z = M^-1 r with the solve's preconditioner: MIC(0) sweeps, or the learned one:
-// y = net(r), then one damped-Jacobi correction z = y + omega D^-1 (r - A y).
-// The network carries the smooth part of A^-1; the correction removes the
-// high-frequency residual it leaves, which a convolutional map trained on
-// right-hand sides handles poorly.
+// z = M^-1 r with the solve's preconditioner: MIC(0) sweeps, or the learned
+// one: z0 = net(r), then nu damped-Jacobi steps z_{k+1} = z_k + omega D^-1
+// (r - A z_k) (nu = learned_smoothing).  The network carries the smooth part
+// of A^-1; the steps remove the high-frequency residual it leaves, which a
+// convolutional map trained on right-hand sides handles poorly.  The steps
+// alternate between y and z, starting in whichever makes the last land in z.
 static int precondition(fsim_ctx* ctx, const SolveArgs& S, bool flexible) {
     if (!flexible) return mic_apply(ctx, S, ctx->r, ctx->y, ctx->z);
     FSIM_PROF(ctx, KID_LEARNED);
-    FSIM_TRY(learned_apply(ctx, ctx->r, ctx->y));
-    hipLaunchKernelGGL(k_jacobi_correct, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, ctx->r,
-                       ctx->y, ctx->z, kLearnedOmega);
+    const int nu = ctx->learned_smoothing;
+    float* a = (nu & 1) ? ctx->y : ctx->z;
+    float* b = (nu & 1) ? ctx->z : ctx->y;
+    FSIM_TRY(learned_apply(ctx, ctx->r, a));
+    for (int k = 0; k < nu; k++) {
+        hipLaunchKernelGGL(k_jacobi_correct, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo,
+                           ctx->r, a, b, kLearnedOmega);
+        std::swap(a, b);
+    }
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }

@@ -14,6 +14,7 @@ import numpy as np
 from . import capi, scenes
 
 DT = 1.0 / 240.0
+LOSS_MSE, LOSS_ENERGY = 0, 1
 
 # (input channels, output channels, level) of the 3x3x3 layers, in order
 # (fsim_learned.hip kLayers); the 1x1 output layer maps 16 channels to 1.
@@ -97,10 +98,12 @@ def residual_samples(sim: capi.Simulation, label: np.ndarray, p: np.ndarray, cou
 
 
 def train(grid: capi.Grid, B, L, P, epochs: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
-          params: np.ndarray | None = None, log=None) -> tuple[np.ndarray, list[float]]:
-    """Adam over shuffled mini-batches; returns the parameters and the per-epoch mean loss."""
+          params: np.ndarray | None = None, log=None, loss: int = LOSS_MSE) -> tuple[np.ndarray, list[float]]:
+    """Adam over shuffled mini-batches; returns the parameters and the per-epoch mean loss.
+    loss: LOSS_MSE (masked MSE against p/s) or LOSS_ENERGY (e^T A e / t^T A t, the norm CG minimises)."""
     rng = np.random.default_rng(seed)
     sim = capi.Simulation(grid, scenes.params())
+    sim.set_option(capi.OPT_LEARNED_LOSS, loss)
     if params is None:
         sim.learned_init(seed)
     else:

@@ -231,7 +231,15 @@ typedef enum fsim_option {
      * used to cross-check the wavefront kernels). */
     FSIM_OPT_SWEEP_MODE = 0,
     /* 1: time every kernel with HIP events (fsim_get_kernel_time) */
-    FSIM_OPT_PROFILE = 1
+    FSIM_OPT_PROFILE = 1,
+    /* learned preconditioner: damped-Jacobi steps (omega 2/3) applied after
+     * the network, 0 .. 16 (default 4) */
+    FSIM_OPT_LEARNED_SMOOTHING = 2,
+    /* learned preconditioner training loss (fsim_learned_grad,
+     * fsim_learned_train_batch): 0 (default) masked MSE of out against p/s
+     * over WATER cells; 1 energy norm e^T A e / t^T A t of e = out - p/s, with
+     * A the unscaled pressure matrix on WATER cells */
+    FSIM_OPT_LEARNED_LOSS = 3
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
 

@@ -19,10 +19,17 @@ emulate_bf16 the weights, the input and every stored activation are rounded
 to bfloat16 (round to nearest even) where the GPU rounds them; sums stay in
 FP64 here (FP32 on the GPU).
 
+Training losses: masked MSE of out against p/s over WATER cells, or the
+energy norm e^T A e / t^T A t of e = out - p/s (A the unscaled 7-point
+pressure matrix on WATER cells: diagonal = number of non-SOLID neighbours,
+-1 towards WATER neighbours, AIR neighbours Dirichlet zeros).
+
 Parity pins (tests/test_learned_oracle.py): central finite differences of
-the FP64 loss against backward() on every layer, conv3 of a unit impulse
+both FP64 losses against backward() on every layer, conv3 of a unit impulse
 equals the mirrored kernel, pool(up(x)) == x, the 1x1 layer is a dot
-product.
+product; the energy form equals its edge sum
+sum_(WATER-WATER edges) (e_i - e_j)^2 + sum_i (AIR neighbours of i) e_i^2
+(loops over cells) and A is symmetric.
 """
 from __future__ import annotations
 
@@ -207,4 +214,50 @@ def loss_and_grad(P: dict, x: np.ndarray, target: np.ndarray, fluid: np.ndarray,
     err = np.where(fluid, out - target, 0.0)
     loss = float((err ** 2).sum() / n)
     g = backward(c, 2.0 * err / n)
+    return loss, g, out
+
+
+def neighbour_counts(label: np.ndarray, shape, padded, water: int = 0, solid: int = 2) -> np.ndarray:
+    """Non-SOLID neighbour count of each WATER cell (0 elsewhere) on the padded grid; cells outside the
+    grid count as SOLID."""
+    lab = label.reshape(shape)
+    pad = np.full(tuple(n + 2 for n in shape), solid, lab.dtype)
+    pad[1:-1, 1:-1, 1:-1] = lab
+    cnt = np.zeros(shape, np.int64)
+    for ax in range(3):
+        for d in (-1, 1):
+            cnt += np.roll(pad, -d, axis=ax)[1:-1, 1:-1, 1:-1] != solid
+    cnt = np.where(lab == water, cnt, 0)
+    out = np.zeros(padded, np.int64)
+    out[:shape[0], :shape[1], :shape[2]] = cnt
+    return out
+
+
+def apply_a(e: np.ndarray, fluid: np.ndarray, cnt: np.ndarray) -> np.ndarray:
+    """A e on WATER cells (e is taken as zero off WATER), zero elsewhere."""
+    ez = np.where(fluid, e, 0.0)
+    nb = np.zeros_like(ez)
+    for ax in range(3):
+        for d in (-1, 1):
+            sh = np.zeros_like(ez)
+            src = [slice(None)] * 3
+            dst = [slice(None)] * 3
+            if d > 0:
+                src[ax], dst[ax] = slice(1, None), slice(None, -1)
+            else:
+                src[ax], dst[ax] = slice(None, -1), slice(1, None)
+            sh[tuple(dst)] = ez[tuple(src)]
+            nb += sh
+    return np.where(fluid, cnt * ez - nb, 0.0)
+
+
+def energy_loss_and_grad(P: dict, x: np.ndarray, target: np.ndarray, fluid: np.ndarray, cnt: np.ndarray,
+                         emulate_bf16: bool = False):
+    """Energy norm e^T A e / t^T A t of e = out - target on WATER cells, and its parameter gradient."""
+    out, c = forward(P, x, emulate_bf16)
+    e = np.where(fluid, out - target, 0.0)
+    ae = apply_a(e, fluid, cnt)
+    et = float(np.sum(np.where(fluid, target, 0.0) * apply_a(target, fluid, cnt)))
+    loss = float(np.sum(e * ae) / et)
+    g = backward(c, 2.0 * ae / et)
     return loss, g, out

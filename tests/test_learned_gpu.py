@@ -77,6 +77,32 @@ def test_gradient_matches_oracle(gpu):
         assert c > 0.99 and rel < 0.1, name
 
 
+def test_energy_gradient_matches_oracle(gpu):
+    say("case: learned energy-norm loss gradient 16^3, seed 5")
+    grid, _, b, lab, p, _ = solved_sample(16, [0.0, 0.0, 0.0, 0.5, 0.6, 1.0])
+    sim = capi.Simulation(grid, scenes.params())
+    sim.set_option(capi.OPT_LEARNED_LOSS, 1)
+    sim.learned_init(5)
+    params = sim.learned_get_params()
+    loss, g = sim.learned_grad(b, lab, p)
+    sim.close()
+    x, s, fl = net.make_input(b, lab, grid.shape)
+    tgt = np.zeros(x.shape[:3])
+    tgt[:16, :16, :16] = np.where(fl, p.reshape(grid.shape) / s, 0.0)
+    flp = np.zeros(x.shape[:3], bool)
+    flp[:16, :16, :16] = fl
+    cnt = net.neighbour_counts(lab, grid.shape, x.shape[:3])
+    rloss, rg, _ = net.energy_loss_and_grad(net.unpack(params), x, tgt, flp, cnt, emulate_bf16=True)
+    say(f"  loss gpu {loss:.6e} oracle {rloss:.6e}")
+    assert loss == pytest.approx(rloss, rel=2e-2)
+    gp = net.unpack(g)
+    for name, _, _ in net.param_layout()[0]:
+        c = cosine(gp[name], rg[name])
+        rel = np.linalg.norm(gp[name] - rg[name]) / max(np.linalg.norm(rg[name]), 1e-30)
+        say(f"  {name}: cosine {c:.5f} rel {rel:.2e}")
+        assert c > 0.99 and rel < 0.1, name
+
+
 def test_training_lowers_the_loss(gpu):
     say("case: learned training 16^3, 2 samples, 40 Adam steps")
     samples = [solved_sample(16, blk) for blk in ([0.0, 0.0, 0.0, 0.5, 0.6, 1.0], [0.3, 0.0, 0.2, 1.0, 0.5, 0.8])]
@@ -106,8 +132,9 @@ def fpcg_step(grid, pos, params):
 
 
 def test_trained_network_beats_the_solver_without_it(gpu):
-    """FPCG with the trained network reaches the tolerance without fallback, agrees with MIC(0)-PCG,
-    and needs fewer iterations than the same solver with an all-zero network (Jacobi correction only)."""
+    """FPCG with the trained network (energy-norm loss) reaches the tolerance without fallback, agrees
+    with MIC(0)-PCG, and needs fewer iterations than the same solver with an all-zero network (its
+    damped-Jacobi steps alone)."""
     say("case: FPCG, network trained on 1 solve + 24 residual pairs, 16^3 dam break")
     grid = scenes.grid(16)
     pos = capi.scene_dam_break(grid, [[0.0, 0.0, 0.0, 0.5, 0.6, 1.0]], seed=0)
@@ -118,7 +145,7 @@ def test_trained_network_beats_the_solver_without_it(gpu):
     rb, rl, rp = learned.residual_samples(sim, lab, p_mic, 24, seed=0)
     sim.close()
     B, L, P = np.concatenate([b[None], rb]), np.concatenate([lab[None], rl]), np.concatenate([p_mic[None], rp])
-    params, hist = learned.train(grid, B, L, P, epochs=60, batch=4, lr=1e-3, seed=1)
+    params, hist = learned.train(grid, B, L, P, epochs=60, batch=4, lr=1e-3, seed=1, loss=learned.LOSS_ENERGY)
     st, p = fpcg_step(grid, pos, params)
     st0, _ = fpcg_step(grid, pos, np.zeros(capi.learned_num_params(), np.float32))
     say(f"  loss {hist[0]:.3e} -> {hist[-1]:.3e}; MIC(0)-PCG {st_mic.pcg_iterations} it; trained FPCG "

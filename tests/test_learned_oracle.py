@@ -94,3 +94,68 @@ def test_backward_matches_finite_differences():
             assert fd == pytest.approx(g[name][idx], rel=1e-4, abs=1e-8), (name, idx)
             checked += 1
     assert checked == 2 * len(net.param_layout()[0])
+
+
+def small_labels(rng, shape=(7, 6, 5)):
+    """SOLID border, random WATER / AIR inside."""
+    lab = np.full(shape, 2, np.uint8)
+    inner = rng.random(tuple(n - 2 for n in shape)) < 0.7
+    lab[1:-1, 1:-1, 1:-1] = np.where(inner, 0, 1)
+    return lab
+
+
+def test_energy_form_equals_edge_sum():
+    """e^T A e = sum over WATER-WATER edges (e_i - e_j)^2 + sum_i (AIR neighbours of i) e_i^2, by loops."""
+    rng = np.random.default_rng(11)
+    lab = small_labels(rng)
+    shape = lab.shape
+    fl = lab == 0
+    cnt = net.neighbour_counts(lab.reshape(-1), shape, shape)
+    e = np.where(fl, rng.standard_normal(shape), 0.0)
+    lhs = float(np.sum(e * net.apply_a(e, fl, cnt)))
+    rhs = 0.0
+    for x in range(shape[0]):
+        for y in range(shape[1]):
+            for z in range(shape[2]):
+                if not fl[x, y, z]:
+                    continue
+                for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                    xx, yy, zz = x + d[0], y + d[1], z + d[2]
+                    nl = lab[xx, yy, zz] if 0 <= xx < shape[0] and 0 <= yy < shape[1] and 0 <= zz < shape[2] else 2
+                    if nl == 1:                                   # AIR: Dirichlet zero
+                        rhs += e[x, y, z] ** 2
+                    elif nl == 0 and (xx, yy, zz) > (x, y, z):    # each WATER-WATER edge once
+                        rhs += (e[x, y, z] - e[xx, yy, zz]) ** 2
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+    u, v = rng.standard_normal(shape), rng.standard_normal(shape)
+    assert float(np.sum(np.where(fl, u, 0) * net.apply_a(v, fl, cnt))) == pytest.approx(
+        float(np.sum(np.where(fl, v, 0) * net.apply_a(u, fl, cnt))), rel=1e-12)
+
+
+def test_energy_loss_matches_finite_differences():
+    rng = np.random.default_rng(12)
+    shape = (8, 8, 8)
+    lab = np.full(shape, 2, np.uint8)
+    lab[1:7, 1:7, 1:7] = 1
+    lab[1:7, 1:5, 1:7] = 0
+    b = rng.standard_normal(shape).astype(np.float32)
+    x, s, fl = net.make_input(b.reshape(-1), lab.reshape(-1), shape)
+    cnt = net.neighbour_counts(lab.reshape(-1), shape, x.shape[:3])
+    target = np.where(fl, rng.standard_normal(shape), 0.0)
+    P = {}
+    for name, _, shp in net.param_layout()[0]:
+        fan = shp[1] * shp[2] if len(shp) == 3 else net.OUT_IN
+        P[name] = rnd(rng, *shp, scale=np.sqrt(2.0 / fan)) + (0.05 if name.startswith("b") else 0.0)
+    loss, g, _ = net.energy_loss_and_grad(P, x, target, fl, cnt)
+    assert loss > 0
+    h = 1e-6
+    for name in ("wo", "bo", "w6", "b5", "w3", "w0"):
+        shp = P[name].shape
+        idx = tuple(int(rng.integers(0, n)) for n in shp)
+        old = P[name][idx]
+        P[name][idx] = old + h
+        lp = net.energy_loss_and_grad(P, x, target, fl, cnt)[0]
+        P[name][idx] = old - h
+        lm = net.energy_loss_and_grad(P, x, target, fl, cnt)[0]
+        P[name][idx] = old
+        assert (lp - lm) / (2 * h) == pytest.approx(g[name][idx], rel=1e-4, abs=1e-8), (name, idx)
