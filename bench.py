@@ -60,28 +60,63 @@ def algo_bytes(name: str, nf: float, nc: int, np_: int) -> float | None:
     return b[0] * np_ + b[1] * nc
 
 
-def cpu_baseline(scene, gpu_iterations: float, threads: int, pcg_cap: int) -> dict:
-    """The FP64 oracle on the same scene (initial state), one step with the
-    PCG capped at `pcg_cap` iterations; its per-iteration solve time is
-    extrapolated to the GPU's iteration count for the step."""
+def divergence_l2(sim, grid) -> float:
+    """L2 norm over WATER cells of the MAC divergence of the projected velocity (u2, v2, w2); a cell's
+    faces[k] is its +k face, so div(i) = (u2(i) - u2(i-x) + v2(i) - v2(i-y) + w2(i) - w2(i-z)) / dx."""
+    shape = grid.shape
+    lab = sim.get_grid(capi.FIELD_LABEL).reshape(shape)
+    d = np.zeros(shape, np.float64)
+    for ax, f in enumerate((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2)):
+        v = sim.get_grid(f).reshape(shape).astype(np.float64)
+        d += v - np.roll(v, 1, axis=ax)
+    d /= grid.cell_d
+    return float(np.sqrt(np.sum(np.where(lab == capi.WATER, d, 0.0) ** 2)))
+
+
+def cpu_baseline(scene, threads: int) -> tuple[dict, dict]:
+    """The FP64 oracle (the reference's cost structure) runs one full step of the scene from its initial
+    state, PCG to the same 1e-6 relative residual, timed on `threads` host threads.  The GPU runs the
+    same step on a fresh context, and the two are compared: labels, pressure, post-projection divergence,
+    particle positions and velocities."""
     import oracle
 
-    pos = scene.particles().astype(np.float64)
-    prm = scenes.params(transfer=scene.transfer, max_iterations=pcg_cap)
+    pos = scene.particles()
+    prm = scenes.params(transfer=scene.transfer)
+    sim = capi.Simulation(scene.grid, prm)
+    sim.set_particles(pos)
+    st_g = sim.step(scenes.DT)
+    g = {"label": sim.get_grid(capi.FIELD_LABEL), "p": sim.get_grid(capi.FIELD_PRESSURE),
+         "div": divergence_l2(sim, scene.grid)}
+    g["pos"], g["vel"], _ = sim.get_particles()
+    sim.close()
+
     o = oracle.Oracle(scene.grid, prm, threads=threads)
-    o.set_particles(pos)
+    o.set_particles(pos.astype(np.float64))
     del pos
     t0 = time.perf_counter()
-    st = o.step(scenes.DT)
+    st_o = o.step(scenes.DT)
     wall = time.perf_counter() - t0
+    r = {"label": o.get_grid(capi.FIELD_LABEL), "p": o.get_grid(capi.FIELD_PRESSURE), "div": divergence_l2(o, scene.grid)}
+    r["pos"], r["vel"], _ = o.get_particles()
     o.close()
-    solve_s = st.timer_ms[capi.TIMER_KEYS.index("Incompressibility")] / 1e3
-    its = max(st.pcg_iterations, 1)
-    est_step = (wall - solve_s) + solve_s * (gpu_iterations / its)
-    return {"value": 1.0 / est_step, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle, 1 step of the {scene.n}^3 scene from its initial state with PCG capped at "
-                       f"{its} iterations ({wall:.1f} s wall, {solve_s:.1f} s solve), solve time extrapolated "
-                       f"per iteration to the GPU's {gpu_iterations:.0f} iterations")}
+
+    pn = float(np.linalg.norm(r["p"]))
+    acc = {"scope": f"one step of the {scene.n}^3 scene from its initial state, GPU vs the FP64 oracle, "
+                    "both with PCG to 1e-6 relative residual",
+           "labels_equal": bool(np.array_equal(g["label"], r["label"])),
+           "pressure_rel_l2": float(np.linalg.norm(g["p"] - r["p"]) / pn) if pn > 0 else None,
+           "divergence_l2_over_b": {"gpu": g["div"] / st_g.b_norm_l2 if st_g.b_norm_l2 > 0 else None,
+                                    "oracle": r["div"] / st_o.b_norm_l2 if st_o.b_norm_l2 > 0 else None},
+           "residual_rel_l2": {"gpu": st_g.residual_rel_l2, "oracle": st_o.residual_rel_l2},
+           "pcg_iterations": {"gpu": st_g.pcg_iterations, "oracle": st_o.pcg_iterations},
+           "position_drift_max": float(np.abs(g["pos"] - r["pos"]).max()),
+           "position_drift_max_cells": float(np.abs(g["pos"] - r["pos"]).max() / scene.grid.cell_d),
+           "velocity_rel_l2": float(np.linalg.norm(g["vel"] - r["vel"]) / max(np.linalg.norm(r["vel"]), 1e-300))}
+    base = {"value": 1.0 / wall, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": (f"FP64 oracle (clean-room restatement with the reference's cost structure), one full step "
+                       f"of the {scene.n}^3 scene from its initial state: {wall:.1f} s wall with "
+                       f"{st_o.pcg_iterations} PCG iterations to 1e-6 on {threads} threads")}
+    return base, acc
 
 
 COPY_BYTES = 1 << 30   # bytes read (and written) by each calibration copy
@@ -225,7 +260,6 @@ def main():
     ap.add_argument("--size", type=int, default=256, help="grid cells per axis (default 256)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-pcg-cap", type=int, default=60)
     ap.add_argument("--profile-steps", type=int, default=None,
                     help="steps timed per kernel with HIP events after the timed region (default: --steps)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -303,9 +337,10 @@ def main():
         solve_cmp = learned_solve_compare(args.learned_size, args.learned_train_size, args.learned_scenes,
                                           args.learned_epochs)
 
-    cpu = None
+    cpu, accuracy = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, its, args.cpu_threads, args.cpu_pcg_cap)
+        cpu, accuracy = cpu_baseline(scene, args.cpu_threads)
+        print(f"# accuracy vs the FP64 oracle: {json.dumps(accuracy)}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "steps/s @256^3 FLIP",
@@ -326,6 +361,7 @@ def main():
                    "pcg_iterations": round(its, 1), "pressure_solve_ms": round(solve_ms, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
         "pressure_solve": solve_cmp,
         "kernel_ms_per_step": breakdown,
     }
