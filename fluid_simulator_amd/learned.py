@@ -133,3 +133,33 @@ def load(path: str) -> np.ndarray:
     if p.size != capi.learned_num_params():
         raise ValueError(f"{path}: {p.size} parameters, expected {capi.learned_num_params()}")
     return p.astype(np.float32)
+
+
+def forward_timing(n: int, reps: int = 10, seed: int = 1) -> dict:
+    """Wall time of fsim_learned_forward on an n^3 grid (random right-hand side, every cell WATER,
+    initial parameters), host copies included; for profiling the network's kernels."""
+    import time
+    grid = scenes.grid(n)
+    sim = capi.Simulation(grid, scenes.params())
+    sim.learned_init(seed)
+    b = np.random.default_rng(seed).standard_normal(grid.nc).astype(np.float32)
+    lab = np.full(grid.nc, capi.WATER, np.uint8)
+    sim.learned_forward(b, lab)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = sim.learned_forward(b, lab)
+    dt = (time.perf_counter() - t0) / reps
+    sim.close()
+    return {"grid": [n] * 3, "reps": reps, "ms_per_forward_with_copies": dt * 1e3,
+            "gflop_per_forward": flops_per_apply(grid.shape) / 1e9, "checksum": float(np.abs(out).sum())}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="Time the learned network's forward pass (fsim_learned_forward)")
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--reps", type=int, default=10)
+    a = ap.parse_args()
+    print(json.dumps(forward_timing(a.size, a.reps)), flush=True)
