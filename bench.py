@@ -37,7 +37,7 @@ ALGO_BYTES = {
     "sweep_fwd": ("f", 13.0),      # R r 4, R P 4, R label 1, W q 4
     "sweep_bwd": ("f", 13.0),      # R q 4, R P 4, R label 1, W z 4
     "mic_factor": ("f", 5.0),      # R label 1, W P 4
-    "spmv": ("f", 25.0),           # direction step: R label 1, R z 4, R s 4, W s' 4, W q 4, R/W x 8
+    "spmv": ("f", 17.0),           # direction step: R label 1, R z 4, R s 4, W s' 4, W q 4
     "update": ("f", 13.0),         # R label 1, R q 4, R/W r 8
     "dot": ("f", 8.0),             # R z 4, R r 4
     "p2g": ("p+c", (24.0, 12.0)),  # R particle pos+vel 24 B/p, W u,v,w 12 B/cell

@@ -111,9 +111,12 @@ static void free_all(fsim_ctx* ctx) {
         if (p) (void)hipFree(p);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
     if (ctx->res_ev) (void)hipEventDestroy(ctx->res_ev);
+    if (ctx->dir_ev) (void)hipEventDestroy(ctx->dir_ev);
+    if (ctx->x_ev) (void)hipEventDestroy(ctx->x_ev);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->prof_pool) (void)hipEventDestroy(e);
+    if (ctx->xstream) (void)hipStreamDestroy(ctx->xstream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
@@ -151,7 +154,8 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
         delete ctx;
         return rc;
     };
-    if (hipGetDevice(&ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipGetDevice(&ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->xstream, hipStreamNonBlocking) != hipSuccess)
         return fail(FSIM_ERR_HIP);
     GridDims& g = ctx->g;
     g.nx = grid->nx;
@@ -186,7 +190,10 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
         return fail(FSIM_ERR_OOM);
     for (auto& e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(FSIM_ERR_HIP);
-    if (hipEventCreateWithFlags(&ctx->res_ev, hipEventDisableTiming) != hipSuccess) return fail(FSIM_ERR_HIP);
+    if (hipEventCreateWithFlags(&ctx->res_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->dir_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->x_ev, hipEventDisableTiming) != hipSuccess)
+        return fail(FSIM_ERR_HIP);
     // initial grid state: zero faces, AIR labels (MacGrid ctor, macGrid.cpp:9-17)
     float* f32s[] = {ctx->u, ctx->v, ctx->w, ctx->u2, ctx->v2, ctx->w2, ctx->density, ctx->b, ctx->r, ctx->z, ctx->s, ctx->s2,
                      ctx->q, ctx->y, ctx->P32};

@@ -156,6 +156,7 @@ struct LearnedNet;
 struct fsim_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t xstream = nullptr;    // PCG: x += alpha s beside the r update and the preconditioner
     fsim::GridDims g{};
     fsim_params prm{};
     std::string err;
@@ -190,6 +191,8 @@ struct fsim_ctx {
     double* partials = nullptr;       // block partial sums (FP64)
     unsigned* red_count = nullptr;    // blocks finished in the running reduction (0 between launches)
     hipEvent_t res_ev = nullptr;      // PCG: the iteration's scalars are on the host
+    hipEvent_t dir_ev = nullptr;      // PCG: the direction step (s, alpha) is done
+    hipEvent_t x_ev = nullptr;        // PCG: the x update on xstream is done
     int last_mic_iterations = 0;      // MIC(0)-PCG iterations of the last solve that used it
     fsim::LearnedNet* learned = nullptr;   // learned preconditioner (network parameters and buffers)
     int partials_cap = 0;

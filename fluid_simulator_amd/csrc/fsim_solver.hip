@@ -416,19 +416,15 @@ __device__ inline void item_cells(const RowItems& it, int k, int& p, int& y, int
 // SPMV_PLAIN: q = A s.  SPMV_COPY: s' = z, q = A s'.  SPMV_XPAY: s' = s * beta + z
 // (multSelfAndAdd :229-242), q = A s'.  The new search direction s' goes to a
 // second buffer and is recomputed for the neighbours from z and the old s, so
-// no block reads a value another block has already replaced.  With
-// pending_x the previous iteration's x += alpha s (FP64) is applied here,
-// before FIN_ALPHA replaces alpha.  Only WATER cells are written: q is zeroed
-// before a solve and never read elsewhere.
+// no block reads a value another block has already replaced.  Only WATER
+// cells are written: q is zeroed before a solve and never read elsewhere.
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __restrict__ nbinfo,
                                                  const float* __restrict__ z, const float* __restrict__ s,
-                                                 float* __restrict__ s_new, float* __restrict__ q, double* __restrict__ x,
-                                                 int pending_x, Reduce red) {
+                                                 float* __restrict__ s_new, float* __restrict__ q, Reduce red) {
     double d = 0.0;
     const float ms = -S.scale_f;
     const float beta = MODE == SPMV_XPAY ? (float)red.sc->beta : 0.f;
-    const double alpha = red.sc->alpha;
     auto dir = [&](int64_t j) {
         if constexpr (MODE == SPMV_PLAIN) return s[j];
         else if constexpr (MODE == SPMV_COPY) return z[j];
@@ -468,7 +464,6 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
         // all gathers of the 4 cells first (interior cells: every neighbour index is in range),
         // then the sums with the reference's term order, by mask
         float v[4][7];
-        double xo[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const int64_t i = rowbase + zc[c];
@@ -480,7 +475,6 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
             v[c][4] = dir(n.mx);
             v[c][5] = dir(n.my);
             v[c][6] = dir(n.mz);
-            xo[c] = pending_x && (info[c] & NB_FLUID) ? x[i] : 0.0;
         }
 #pragma unroll
         for (int c = 0; c < 4; c++) {
@@ -499,7 +493,6 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
             d += (double)si * (double)val;
             q[i] = val;
             if (MODE != SPMV_PLAIN) s_new[i] = si;
-            if (pending_x) x[i] = xo[c] + (double)s[i] * alpha;
         }
     }
     block_finish<AUX_NONE>(d, 0.0, red);
@@ -565,7 +558,9 @@ __global__ void __launch_bounds__(kBlock) k_update_r(int64_t n, const uint16_t* 
     block_finish<AUX_MAX>(ss, mx, red);
 }
 
-// x += alpha s (FP64) for WATER cells
+// x += alpha s (FP64) for WATER cells.  One cell per thread: it runs on a
+// second stream beside the r update, and a lighter HBM load there costs the
+// r update less than a vectorised kernel does (measured at 256^3).
 __global__ void __launch_bounds__(kBlock) k_update_x(int64_t n, const uint16_t* __restrict__ nbinfo,
                                                      const float* __restrict__ s, double* __restrict__ x,
                                                      const SolveScalars* __restrict__ sc) {
@@ -838,29 +833,42 @@ static int dot_with(fsim_ctx* ctx, const float* a, const float* b, int op) {
 static int dot_zr(fsim_ctx* ctx, int op) { return dot_with(ctx, ctx->z, ctx->r, op); }
 
 // New search direction s (z, or z + beta s) into the second buffer, q = A s,
-// s.q -> alpha; with pending_x also the previous iteration's x += alpha s.
-static int direction_step(fsim_ctx* ctx, const SolveArgs& S, bool copy, bool pending_x) {
+// s.q -> alpha.
+static int direction_step(fsim_ctx* ctx, const SolveArgs& S, bool copy) {
     {
         FSIM_PROF(ctx, KID_SPMV);
         const dim3 grid(red_blocks(S.nc)), block(kBlock);
         const Reduce red = reduce_to(ctx, FIN_ALPHA);
         if (copy)
             hipLaunchKernelGGL(k_spmv<SPMV_COPY>, grid, block, 0, ctx->stream, S, ctx->nbinfo, ctx->z, ctx->s, ctx->s2,
-                               ctx->q, ctx->x, pending_x ? 1 : 0, red);
+                               ctx->q, red);
         else
             hipLaunchKernelGGL(k_spmv<SPMV_XPAY>, grid, block, 0, ctx->stream, S, ctx->nbinfo, ctx->z, ctx->s, ctx->s2,
-                               ctx->q, ctx->x, pending_x ? 1 : 0, red);
+                               ctx->q, red);
     }
     FSIM_HIP(ctx, hipGetLastError());
     std::swap(ctx->s, ctx->s2);
     return FSIM_OK;
 }
 
-static int update_x(fsim_ctx* ctx) {
-    FSIM_PROF(ctx, KID_UPDATE);
-    hipLaunchKernelGGL(k_update_x, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, ctx->g.nc, ctx->nbinfo, ctx->s,
-                       ctx->x, ctx->scal);
+// x += alpha s of the direction step just queued, on the second stream: it
+// runs beside the r update instead of inside the next direction step
+// (measured at 256^3: started after the r update instead, it slows the
+// latency-bound forward sweep by more than it slows the r update here).  The
+// next direction step replaces alpha and, one step later, this s buffer, so
+// the main stream waits for it (join_x) before the next direction step and
+// before anything else reads x.
+static int update_x_beside(fsim_ctx* ctx) {
+    FSIM_HIP(ctx, hipEventRecord(ctx->dir_ev, ctx->stream));
+    FSIM_HIP(ctx, hipStreamWaitEvent(ctx->xstream, ctx->dir_ev, 0));
+    hipLaunchKernelGGL(k_update_x, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->xstream, ctx->g.nc, ctx->nbinfo,
+                       ctx->s, ctx->x, ctx->scal);
     FSIM_HIP(ctx, hipGetLastError());
+    FSIM_HIP(ctx, hipEventRecord(ctx->x_ev, ctx->xstream));
+    return FSIM_OK;
+}
+static int join_x(fsim_ctx* ctx) {
+    FSIM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->x_ev, 0));
     return FSIM_OK;
 }
 
@@ -915,8 +923,8 @@ struct PcgResult {
 // One host round trip per iteration: the iteration's scalars are copied after
 // the r update, and the next preconditioner application and its dots run
 // while the host waits for them.  If the solve stops or restarts, those
-// results are discarded.  x += alpha s of an iteration that goes on is applied
-// by the next direction step.
+// results are discarded.  x += alpha s runs on the second stream
+// (update_x_beside) and is joined before the next direction step.
 static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexible, int max_it, PcgResult* res) {
     const int64_t nc = S.nc;
     fsim_step_stats& st = ctx->stats;
@@ -931,13 +939,14 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
     const int max_reliable = 64;
     const double kDelta = 1e-3;
     double last_true = bnorm;
-    bool copy = true, pending_x = false;
+    bool copy = true;
     int it = 0;
     res->converged = false;
     res->nan = false;
     for (; it < max_it; it++) {
-        FSIM_TRY(direction_step(ctx, S, copy, pending_x));
-        pending_x = false;
+        FSIM_TRY(join_x(ctx));
+        FSIM_TRY(direction_step(ctx, S, copy));
+        FSIM_TRY(update_x_beside(ctx));
         {
             FSIM_PROF(ctx, KID_UPDATE);
             hipLaunchKernelGGL(k_update_r, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->q,
@@ -962,7 +971,7 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
         }
         bool restart = false;
         if (stop_met(ctx, h.rr, h.rmax, bnorm) || std::sqrt(h.rr) < kDelta * last_true) {
-            FSIM_TRY(update_x(ctx));
+            FSIM_TRY(join_x(ctx));
             const bool replace = st.reliable_updates < max_reliable;
             FSIM_TRY(true_residual(ctx, S, replace));
             FSIM_TRY(read_scalars(ctx));
@@ -977,8 +986,6 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
                 last_true = std::sqrt(ctx->scal_host->tr_ss);
                 restart = true;
             }
-        } else {
-            pending_x = true;
         }
         if (restart) {
             FSIM_TRY(precondition(ctx, S, flexible));
@@ -986,7 +993,7 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
         }
         copy = restart;
     }
-    if (pending_x) FSIM_TRY(update_x(ctx));
+    FSIM_TRY(join_x(ctx));
     res->iterations = it;
     return FSIM_OK;
 }
@@ -1119,7 +1126,7 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
     if (op == FSIM_OP_APPLY_A) {
         FSIM_TRY(fill_f32(ctx, ctx->q, 0.0f, nc));   // the SpMV writes WATER cells only
         hipLaunchKernelGGL(k_spmv<SPMV_PLAIN>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo, nullptr,
-                           ctx->r, nullptr, ctx->q, nullptr, 0, reduce_to(ctx, FIN_NONE));
+                           ctx->r, nullptr, ctx->q, reduce_to(ctx, FIN_NONE));
         FSIM_HIP(ctx, hipGetLastError());
         FSIM_HIP(ctx, hipMemcpyAsync(hout.data(), ctx->q, sizeof(float) * nc, hipMemcpyDeviceToHost, ctx->stream));
     } else if (op == FSIM_OP_APPLY_MIC0) {
