@@ -420,16 +420,26 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     // slot (reuse check).  The reads of step t+1 are issued right after step
     // t's ring write, so their latency overlaps the rest of step t; the tag
     // test at the top of step t+1 re-reads only if a value was not there yet.
+    // Sweeps: row 0's -y value comes from one ring, the previous wave's
+    // (w > 0, tagged t - 1) or, for wave 0, the -y packets (tagged t; a first
+    // y tile's ring holds zeros), so one read, one tag test and no select.
     uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
+    const uint64_t* a0_base = (KIND != 0 && w == 0) ? yp_base : up_base;
     int spy = 0, spz = 0;   // progress of the sending waves (read on reuse-check steps)
     auto rd_at = [&](auto dc2, int tt) {
         constexpr int d2 = decltype(dc2)::value;   // tt mod kPrefetch (= tt mod kRingV)
         constexpr int s_up = (d2 + kRingV - 1) % kRingV, s_bp = (d2 + kBpEvery) % kRingV;
         const int ps = tt % RINGP;
+        if constexpr (KIND != 0) {
+            const int aoff = w > 0 ? s_up * kSlotV : ps * NP * WF_TZ;
 #pragma unroll
-        for (int k = 0; k < NP; k++) {
-            a[k] = lpeek(up_base + s_up * kSlotV + k * WF_TZ);
-            c[k] = lpeek(yp_base + ps * NP * WF_TZ + k * WF_TZ);
+            for (int k = 0; k < NP; k++) a[k] = lpeek(a0_base + aoff + k * WF_TZ);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NP; k++) {
+                a[k] = lpeek(up_base + s_up * kSlotV + k * WF_TZ);
+                c[k] = lpeek(yp_base + ps * NP * WF_TZ + k * WF_TZ);
+            }
         }
         unroll<R>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
@@ -476,6 +486,18 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             // branch-free: per-lane masks, one ballot (the -z packet slots are
             // read by every lane from one address, so their test is uniform)
             auto ready = [&]() {
+                if constexpr (KIND != 0) {
+                    // ballots taken on the comparisons themselves (no boolean lane value to rebuild)
+                    const int ta = w > 0 ? t - 1 : t;
+                    const bool need_a = w > 0 ? need_up : need_y;
+                    uint64_t bad = __builtin_amdgcn_ballot_w64(need_a && tag_of(a[0]) != ta) |
+                                   __builtin_amdgcn_ballot_w64(need_bp && tag_of(bp) < t_bp);
+                    unroll<R>([&](auto rc) {
+                        constexpr int r = decltype(rc)::value;
+                        bad |= __builtin_amdgcn_ballot_w64(need_z[r] && tag_of(zq[r][0]) != t);
+                    });
+                    return bo_ok && bad == 0;
+                }
                 unsigned lane = (unsigned)(tag_of(bp) >= t_bp) | (unsigned)!need_bp;
 #pragma unroll
                 for (int k = 0; k < NP; k++) {
@@ -504,14 +526,25 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
             T vnew[R];
             unroll<R>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                T vy = 0, vz = shr1(prev[r]), vin;
-                if constexpr (r > 0) {
-                    vy = vnew[r - 1];   // the row above, same step (same x)
+                T vy = 0, vz, vin;
+                if constexpr (KIND != 0) {
+                    // -z: lane tz - 1's previous value; lane 0 keeps the -z packet (DPP "old" operand;
+                    // zeros on a first z tile).  Lane 0 is active exactly when its packet is needed.
+                    vz = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp((int)(uint32_t)zq[r][0],
+                                                                               __builtin_bit_cast(int, prev[r]),
+                                                                               0x138, 0xf, 0xf, false));
+                    if constexpr (r > 0) vy = vnew[r - 1];   // the row above, same step (same x)
+                    else from_words(vin, a), vy = vin;
                 } else {
-                    if (w > 0) from_words(vin, a), vy = vin;
-                    else if (need_y) from_words(vin, c), vy = vin;
+                    vz = shr1(prev[r]);
+                    if constexpr (r > 0) {
+                        vy = vnew[r - 1];
+                    } else {
+                        if (w > 0) from_words(vin, a), vy = vin;
+                        else if (need_y) from_words(vin, c), vy = vin;
+                    }
+                    if (tz == 0 && need_z[r]) from_words(vin, zq[r]), vz = vin;
                 }
-                if (tz == 0 && need_z[r]) from_words(vin, zq[r]), vz = vin;
                 if constexpr (KIND == 0) {
                     const uint32_t info = act[r] ? (uint32_t)wring[d][r] : 0u;
                     double v = 0;
@@ -546,6 +579,8 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                 } else {
                     const uint64_t wd = wring[d][r];
                     const float p = __uint_as_float((uint32_t)wd);
+                    // stores of WATER cells only: storing every interior cell doubles the stores, and
+                    // the in-order memory counter then delays the prefetch waits (measured: slower)
                     const bool fl = act[r] && ((uint32_t)(wd >> 32) & NB_FLUID) != 0;
                     // 0 + ms * prev in one rounding: fma(ms, prev, +0) equals the product, and +0 for -0
                     float acc = __builtin_fmaf(ms, prev[r], 0.f);
@@ -608,8 +643,10 @@ __global__ void __launch_bounds__(WF_THREADS) k_wave(WaveArgs A) {
     if (tile < 0) return;
     const int J = tile / A.TK, K = tile % A.TK;
     for (int i = threadIdx.x; i < kRingV * WF_NW * NP * WF_TZ; i += WF_THREADS) (&S.v[0][0][0][0])[i] = ~0ull;
-    for (int i = threadIdx.x; i < RINGP * NP * WF_TZ; i += WF_THREADS) (&S.yp[0][0][0])[i] = ~0ull;
-    for (int i = threadIdx.x; i < RINGP * NP * WF_TY; i += WF_THREADS) (&S.zp[0][0][0])[i] = ~0ull;
+    // sweeps on a first y / z tile read their -y / -z values from a ring of zeros (never refilled)
+    const uint64_t yp_init = KIND != 0 && J == 0 ? 0ull : ~0ull, zp_init = KIND != 0 && K == 0 ? 0ull : ~0ull;
+    for (int i = threadIdx.x; i < RINGP * NP * WF_TZ; i += WF_THREADS) (&S.yp[0][0][0])[i] = yp_init;
+    for (int i = threadIdx.x; i < RINGP * NP * WF_TY; i += WF_THREADS) (&S.zp[0][0][0])[i] = zp_init;
     for (int i = threadIdx.x; i < kRingO * NP * WF_TZ; i += WF_THREADS) (&S.yo[0][0][0])[i] = ~0ull;
     for (int i = threadIdx.x; i < kRingO * NP * WF_TY; i += WF_THREADS) (&S.zo[0][0][0])[i] = ~0ull;
     if (threadIdx.x < WF_NW) S.prog[threadIdx.x] = -1;
