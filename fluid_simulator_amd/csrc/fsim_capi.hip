@@ -105,7 +105,7 @@ static void free_all(fsim_ctx* ctx) {
     void* ptrs[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed, ctx->u, ctx->v, ctx->w,
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
                     ctx->scan_tmp, ctx->nbinfo, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
-                    ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->W, ctx->ybnd, ctx->zbnd, ctx->s2, ctx->red_count,
+                    ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->ybnd, ctx->zbnd, ctx->s2, ctx->red_count,
                     ctx->wave_ticket, ctx->wave_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

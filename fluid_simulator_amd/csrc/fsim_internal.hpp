@@ -101,7 +101,6 @@ struct WaveArgs {
     double scale;                 // dt / (rho dx^2)
     double* P64;
     float* P32;
-    uint64_t* W;                  // packed (nbinfo << 32 | P32 bits) per cell
     uint64_t* ybnd;               // packets across y tile edges: [tile J*TK+K][step][64][words]
     uint64_t* zbnd;               // packets across z tile edges: [tile J*TK+K][step][WF_TY][words]
     unsigned long long* ticket;
@@ -201,7 +200,6 @@ struct fsim_ctx {
     double last_dt = 0.0;
     // wavefront sweeps
     fsim::WaveArgs wave{};
-    uint64_t* W = nullptr;
     uint64_t* ybnd = nullptr;
     uint64_t* zbnd = nullptr;
     unsigned long long* wave_ticket = nullptr;   // 3 counters: factor, fwd, bwd

@@ -325,14 +325,6 @@ __global__ void __launch_bounds__(kBlock) k_mic_factor_plane(PlaneArgs A, double
     P32[i] = (float)p;
 }
 
-// (P32, nbinfo) packed per cell for the sweeps
-__global__ void __launch_bounds__(kBlock) k_pack_w(int64_t n, const uint16_t* __restrict__ nbinfo,
-                                                   const float* __restrict__ P32, uint64_t* __restrict__ W) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    W[i] = ((uint64_t)nbinfo[i] << 32) | __float_as_uint(P32[i]);
-}
-
 // ---- MIC(0) apply: forward and backward sweeps, one plane per launch (FP32) -------
 // Contraction is off in both sweep implementations so that they round alike.
 __global__ void __launch_bounds__(kBlock) k_fwd_plane(PlaneArgs A, const float* __restrict__ r, float* __restrict__ q,
@@ -785,7 +777,6 @@ static int mic_factor(fsim_ctx* ctx, const SolveArgs& S) {
         A.level = L;
         hipLaunchKernelGGL(k_mic_factor_plane, dim3(nblocks(nt)), dim3(kBlock), 0, ctx->stream, A, ctx->P64, ctx->P32);
     }
-    hipLaunchKernelGGL(k_pack_w, dim3(nblocks(S.nc)), dim3(kBlock), 0, ctx->stream, S.nc, ctx->nbinfo, ctx->P32, ctx->W);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
@@ -1081,13 +1072,11 @@ int wave_init(fsim_ctx* ctx) {
     const int64_t ny_pk = 2 * (int64_t)A.TJ * A.TK * A.SP * WF_TZ;   // per tile (J, K)
     const int64_t nz_pk = 2 * (int64_t)A.TJ * A.TK * A.SP * WF_TY;
     if (ctx->g.nc > kWaveMaxCells) ctx->sweep_mode = 1;
-    if (hipMalloc(&ctx->W, sizeof(uint64_t) * ctx->g.nc) != hipSuccess ||
-        hipMalloc(&ctx->ybnd, sizeof(uint64_t) * ny_pk) != hipSuccess ||
+    if (hipMalloc(&ctx->ybnd, sizeof(uint64_t) * ny_pk) != hipSuccess ||
         hipMalloc(&ctx->zbnd, sizeof(uint64_t) * nz_pk) != hipSuccess ||
         hipMalloc(&ctx->wave_ticket, sizeof(unsigned long long) * 3) != hipSuccess ||
         hipMalloc(&ctx->wave_err, sizeof(int)) != hipSuccess)
         return set_err(ctx, FSIM_ERR_OOM, "wavefront buffer allocation failed");
-    FSIM_HIP(ctx, hipMemsetAsync(ctx->W, 0, sizeof(uint64_t) * ctx->g.nc, ctx->stream));
     FSIM_HIP(ctx, hipMemsetAsync(ctx->ybnd, 0, sizeof(uint64_t) * ny_pk, ctx->stream));
     FSIM_HIP(ctx, hipMemsetAsync(ctx->zbnd, 0, sizeof(uint64_t) * nz_pk, ctx->stream));
     FSIM_HIP(ctx, hipMemsetAsync(ctx->wave_ticket, 0, sizeof(unsigned long long) * 3, ctx->stream));
@@ -1095,7 +1084,6 @@ int wave_init(fsim_ctx* ctx) {
     A.nbinfo = ctx->nbinfo;
     A.P64 = ctx->P64;
     A.P32 = ctx->P32;
-    A.W = ctx->W;
     A.ybnd = ctx->ybnd;
     A.zbnd = ctx->zbnd;
     A.err = ctx->wave_err;

@@ -322,18 +322,21 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     const uint32_t plane = (uint32_t)(A.ny * A.nz);
     const uint32_t nc = (uint32_t)nx * plane;   // below 2^28 (kWaveMaxCells)
     const uint32_t plane8 = plane * 8u, nc8 = nc * 8u;
-    // buffer resources.  factor: rA nbinfo, rB W, rC P64, rD P32;  sweeps: rA W, rB in, rC out
+    // buffer resources.  factor: rA nbinfo, rC P64, rD P32;  sweeps: rA P32, rB in, rC out.
+    // The sweeps read P32 alone: it is 0 for every non-WATER cell (cleared before each factor,
+    // which writes WATER cells only), so P != 0 is the WATER test and the neighbour flags are
+    // not needed (non-WATER values are 0 on the upstream side).
     __amdgpu_buffer_rsrc_t rA, rB, rC, rD;
     if constexpr (KIND == 0) {
         rA = rsrc(A.nbinfo, (int64_t)nc * 2);
-        rB = rsrc(A.W, (int64_t)nc * 8);
+        rB = rA;
         rC = rsrc(A.P64, (int64_t)nc * 8);
         rD = rsrc(A.P32, (int64_t)nc * 4);
     } else {
-        rA = rsrc(A.W, (int64_t)nc * 8);
+        // ranges one and two words longer than the arrays (never addressed): distinct descriptor
+        // words keep the compiler from rebuilding one descriptor from another at every access
+        rA = rsrc(A.P32, (int64_t)nc * 4 + 8);
         rB = rsrc(A.in, (int64_t)nc * 4);
-        // one word longer than the input's range (never addressed): distinct descriptor words
-        // keep the compiler from rebuilding one descriptor from the other at every store
         rC = rsrc(A.out, (int64_t)nc * 4 + 4);
         rD = rC;
     }
@@ -378,14 +381,14 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
         o = __builtin_amdgcn_readfirstlane(o);
     };
     // prefetch ring: per-cell inputs of the next kPrefetch steps
-    uint64_t wring[kPrefetch][R];
+    uint32_t wring[kPrefetch][R];   // factor: nbinfo; sweeps: P32 bits
     float iring[kPrefetch][R];
     auto fetch = [&](int d, int r) {
         if constexpr (KIND == 0) {
             wring[d][r] = bload16(rA, lo2[r], op >> 2);
             iring[d][r] = 0.f;
         } else {
-            wring[d][r] = bload64(rA, lo8[r], op);
+            wring[d][r] = __float_as_uint(bload32f(rA, lo4[r], op >> 1));
             iring[d][r] = bload32f(rB, lo4[r], op >> 1);
         }
     };
@@ -393,7 +396,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     // operations as the loop steps do (the stores of a step, here dropped as
     // out of range, then the loads): the wait for slot d then allows the same
     // number of younger operations on entry as at every later step
-    constexpr int kStores = (KIND == 0 ? 3 : 1) * R;
+    constexpr int kStores = (KIND == 0 ? 2 : 1) * R;
     unroll<kPrefetch>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
         unroll<kStores>([&](auto) { bstore32(0u, rC, kOob, 0); });
@@ -572,16 +575,14 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
                     const bool fl = (info & NB_FLUID) != 0;
                     const float p32 = (float)v;
                     vnew[r] = v;
-                    // W for every cell of the sweep, P64/P32 for WATER cells
-                    bstore64(((uint64_t)info << 32) | (fl ? __float_as_uint(p32) : 0u), rB, act[r] ? lo8[r] : kOob, oc);
+                    // P64/P32 for WATER cells (both were cleared before the factor)
                     bstore64(__builtin_bit_cast(uint64_t, v), rC, fl ? lo8[r] : kOob, oc);
                     bstore32(__float_as_uint(p32), rD, fl ? lo4[r] : kOob, oc >> 1);
                 } else {
-                    const uint64_t wd = wring[d][r];
-                    const float p = __uint_as_float((uint32_t)wd);
+                    const float p = __uint_as_float(wring[d][r]);
                     // stores of WATER cells only: storing every interior cell doubles the stores, and
                     // the in-order memory counter then delays the prefetch waits (measured: slower)
-                    const bool fl = act[r] && ((uint32_t)(wd >> 32) & NB_FLUID) != 0;
+                    const bool fl = act[r] && wring[d][r] != 0u;
                     // 0 + ms * prev in one rounding: fma(ms, prev, +0) equals the product, and +0 for -0
                     float acc = __builtin_fmaf(ms, prev[r], 0.f);
                     acc += ms * vy;
