@@ -104,7 +104,7 @@ static void free_all(fsim_ctx* ctx) {
     learned_free(ctx);
     void* ptrs[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed, ctx->u, ctx->v, ctx->w,
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
-                    ctx->scan_tmp, ctx->nbinfo, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
+                    ctx->scan_tmp, ctx->nbinfo, ctx->chunk_flags, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
                     ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->ybnd, ctx->zbnd, ctx->s2, ctx->red_count,
                     ctx->wave_ticket, ctx->wave_err};
     for (void* p : ptrs)
@@ -176,6 +176,7 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
         (rc = dalloc(ctx, &ctx->label, nc)) || (rc = dalloc(ctx, &ctx->valid, nc)) ||
         (rc = dalloc(ctx, &ctx->density, nc)) || (rc = dalloc(ctx, &ctx->cell_count, nc + 1)) ||
         (rc = dalloc(ctx, &ctx->cell_start, nc + 2)) || (rc = dalloc(ctx, &ctx->nbinfo, nc)) ||
+        (rc = dalloc(ctx, &ctx->chunk_flags, nc / 64 + 1)) ||
         (rc = dalloc(ctx, &ctx->b, nc)) || (rc = dalloc(ctx, &ctx->r, nc)) || (rc = dalloc(ctx, &ctx->z, nc)) ||
         (rc = dalloc(ctx, &ctx->s, nc)) || (rc = dalloc(ctx, &ctx->s2, nc)) || (rc = dalloc(ctx, &ctx->q, nc)) ||
         (rc = dalloc(ctx, &ctx->y, nc)) || (rc = dalloc(ctx, &ctx->red_count, 1)) ||

@@ -180,6 +180,7 @@ struct fsim_ctx {
     int32_t* cell_start = nullptr;   // Nc + 2
     int32_t* scan_tmp = nullptr;
     uint16_t* nbinfo = nullptr;
+    uint32_t* chunk_flags = nullptr;  // solver-vector chunks (64 or 256 cells) holding a WATER cell, per solve
 
     // pressure solve
     float *b = nullptr, *r = nullptr, *z = nullptr, *s = nullptr, *q = nullptr, *y = nullptr;

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "fsim_internal.hpp"
 
@@ -490,64 +491,123 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
     block_finish<AUX_NONE>(d, 0.0, red);
 }
 
-// Streaming kernels: 4 consecutive cells per thread and group (16-byte vector
-// loads).  A thread takes kGroups groups at a time (grid stride): their
-// nbinfo words first, then the vector loads of the groups holding a WATER
-// cell only, so all-AIR/SOLID stretches of the grid cost their nbinfo bytes
-// alone.  The n % 4 tail cells go to the first threads of block 0.
-constexpr int kGroups = 4;
-__device__ inline uint16_t info_at(const uint2& w, int c) {
-    const uint32_t h = c < 2 ? w.x : w.y;
-    return (uint16_t)((c & 1) ? h >> 16 : h & 0xffffu);
+// Streaming kernels without per-cell masks: the solver vectors r, q and z
+// are exactly zero off WATER (zeroed at the start of a solve, written on
+// WATER cells only), so the extra terms are exact zeros.  The vectors are cut
+// into chunks of CH cells (256 with float4 accesses when nc is a multiple of
+// 4, else 64); a chunk without WATER is skipped.  Its flag is one 32-bit word
+// (built with nbinfo for every solve), so a wave reads it with a scalar load;
+// a wave takes kUnits chunks per trip (grid stride) and issues all their
+// loads before using any, so several memory round trips are in flight.
+constexpr int kUnits = 4;
+__host__ __device__ constexpr int stream_chunk(bool vec4) { return vec4 ? 256 : 64; }
+
+template <int CH>
+__global__ void __launch_bounds__(kBlock) k_chunk_flags(int64_t nc, const uint16_t* __restrict__ nbinfo,
+                                                        uint32_t* __restrict__ flags) {
+    const int lane = threadIdx.x & 63;
+    const int nchunks = (int)((nc + CH - 1) / CH);
+    const int nwaves = (int)(gridDim.x * (blockDim.x / 64));
+    for (int u = (int)(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)); u < nchunks; u += nwaves) {
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < CH / 64; k++) {
+            const int64_t i = (int64_t)u * CH + k * 64 + lane;
+            any = any || (i < nc && (nbinfo[i] & NB_FLUID));
+        }
+        any = __builtin_amdgcn_ballot_w64(any) != 0;
+        if (lane == 0) flags[u] = any ? 1u : 0u;
+    }
 }
-__device__ inline float& lane_of(float4& v, int c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
-__device__ inline bool any_fluid(const uint2& w) {
-    constexpr uint32_t m = (uint32_t)NB_FLUID | ((uint32_t)NB_FLUID << 16);
-    return ((w.x | w.y) & m) != 0;
+
+template <bool VEC4, int NV, class Load, class Use>
+__device__ inline void for_flagged_chunks(int64_t nc, const uint32_t* __restrict__ flags, Load&& load, Use&& use) {
+    constexpr int CH = stream_chunk(VEC4);
+    using V = typename std::conditional<VEC4, float4, float>::type;
+    const int lane = threadIdx.x & 63;
+    const int nchunks = (int)((nc + CH - 1) / CH);
+    const int nwaves = (int)(gridDim.x * (blockDim.x / 64));
+    const int w0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)));
+    for (int u0 = w0; u0 < nchunks; u0 += kUnits * nwaves) {
+        uint32_t f[kUnits];
+#pragma unroll
+        for (int k = 0; k < kUnits; k++) {
+            const int u = u0 + k * nwaves;
+            f[k] = u < nchunks ? flags[u] : 0u;
+        }
+        // loads without branches (a skipped lane reads cell 0 and uses nothing), so that all
+        // kUnits x NV loads are issued before the first wait
+        V v[kUnits][NV];
+        bool on[kUnits];
+        int64_t at[kUnits];
+#pragma unroll
+        for (int k = 0; k < kUnits; k++) {
+            const int64_t i = (int64_t)(u0 + k * nwaves) * CH + (VEC4 ? 4 : 1) * lane;
+            on[k] = f[k] != 0u && i < nc;
+            at[k] = on[k] ? i : 0;
+            load(at[k], v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kUnits; k++)
+            if (on[k]) use(at[k], v[k]);
+    }
 }
 
 // r -= alpha q, with ||r||^2 and max|r|
-__global__ void __launch_bounds__(kBlock) k_update_r(int64_t n, const uint16_t* __restrict__ nbinfo,
+template <bool VEC4>
+__global__ void __launch_bounds__(kBlock) k_update_r(int64_t nc, const uint32_t* __restrict__ flags,
                                                      const float* __restrict__ q, float* __restrict__ r, Reduce red) {
+    using V = typename std::conditional<VEC4, float4, float>::type;
     double ss = 0.0, mx = 0.0;
     const float malpha = (float)(-red.sc->alpha);
-    auto cell = [&](uint16_t info, float qv, float& rv) {
-        if (!(info & NB_FLUID)) return false;
+    auto cell = [&](float qv, float& rv) {
         rv = rv + qv * malpha;
         ss += (double)rv * (double)rv;
         mx = fmax(mx, fabs((double)rv));
-        return true;
     };
-    const int64_t n4 = n >> 2, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < n4; v0 += kGroups * stride) {
-        uint2 iw[kGroups];
-        float4 qv[kGroups], rv[kGroups];
-#pragma unroll
-        for (int g = 0; g < kGroups; g++) {
-            const int64_t v = v0 + g * stride;
-            iw[g] = v < n4 ? reinterpret_cast<const uint2*>(nbinfo)[v] : make_uint2(0u, 0u);
-        }
-#pragma unroll
-        for (int g = 0; g < kGroups; g++) {
-            if (!any_fluid(iw[g])) continue;
-            const int64_t v = v0 + g * stride;
-            qv[g] = reinterpret_cast<const float4*>(q)[v];
-            rv[g] = reinterpret_cast<const float4*>(r)[v];
-        }
-#pragma unroll
-        for (int g = 0; g < kGroups; g++) {
-            if (!any_fluid(iw[g])) continue;
-#pragma unroll
-            for (int c = 0; c < 4; c++) cell(info_at(iw[g], c), lane_of(qv[g], c), lane_of(rv[g], c));
-            reinterpret_cast<float4*>(r)[v0 + g * stride] = rv[g];
-        }
-    }
-    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
-        const int64_t i = 4 * n4 + threadIdx.x;
-        float rv = r[i];
-        if (cell(nbinfo[i], q[i], rv)) r[i] = rv;
-    }
+    for_flagged_chunks<VEC4, 2>(
+        nc, flags,
+        [&](int64_t i, V(&v)[2]) {
+            v[0] = *reinterpret_cast<const V*>(q + i);
+            v[1] = *reinterpret_cast<const V*>(r + i);
+        },
+        [&](int64_t i, V(&v)[2]) {
+            if constexpr (VEC4) {
+                cell(v[0].x, v[1].x);
+                cell(v[0].y, v[1].y);
+                cell(v[0].z, v[1].z);
+                cell(v[0].w, v[1].w);
+            } else {
+                cell(v[0], v[1]);
+            }
+            *reinterpret_cast<V*>(r + i) = v[1];
+        });
     block_finish<AUX_MAX>(ss, mx, red);
+}
+
+// a . b (FP64 products of the FP32 values)
+template <bool VEC4>
+__global__ void __launch_bounds__(kBlock) k_dot(int64_t nc, const uint32_t* __restrict__ flags,
+                                                const float* __restrict__ a, const float* __restrict__ b, Reduce red) {
+    using V = typename std::conditional<VEC4, float4, float>::type;
+    double d = 0.0;
+    for_flagged_chunks<VEC4, 2>(
+        nc, flags,
+        [&](int64_t i, V(&v)[2]) {
+            v[0] = *reinterpret_cast<const V*>(a + i);
+            v[1] = *reinterpret_cast<const V*>(b + i);
+        },
+        [&](int64_t i, V(&v)[2]) {
+            if constexpr (VEC4) {
+                d += (double)v[0].x * (double)v[1].x;
+                d += (double)v[0].y * (double)v[1].y;
+                d += (double)v[0].z * (double)v[1].z;
+                d += (double)v[0].w * (double)v[1].w;
+            } else {
+                d += (double)v[0] * (double)v[1];
+            }
+        });
+    block_finish<AUX_NONE>(d, 0.0, red);
 }
 
 // x += alpha s (FP64) for WATER cells.  One cell per thread: it runs on a
@@ -561,39 +621,6 @@ __global__ void __launch_bounds__(kBlock) k_update_x(int64_t n, const uint16_t* 
     x[i] += (double)s[i] * sc->alpha;
 }
 
-__global__ void __launch_bounds__(kBlock) k_dot(int64_t n, const uint16_t* __restrict__ nbinfo, const float* __restrict__ a,
-                                                const float* __restrict__ b, Reduce red) {
-    double d = 0.0;
-    const int64_t n4 = n >> 2, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < n4; v0 += kGroups * stride) {
-        uint2 iw[kGroups];
-        float4 av[kGroups], bv[kGroups];
-#pragma unroll
-        for (int g = 0; g < kGroups; g++) {
-            const int64_t v = v0 + g * stride;
-            iw[g] = v < n4 ? reinterpret_cast<const uint2*>(nbinfo)[v] : make_uint2(0u, 0u);
-        }
-#pragma unroll
-        for (int g = 0; g < kGroups; g++) {
-            if (!any_fluid(iw[g])) continue;
-            const int64_t v = v0 + g * stride;
-            av[g] = reinterpret_cast<const float4*>(a)[v];
-            bv[g] = reinterpret_cast<const float4*>(b)[v];
-        }
-#pragma unroll
-        for (int g = 0; g < kGroups; g++) {
-            if (!any_fluid(iw[g])) continue;
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (info_at(iw[g], c) & NB_FLUID) d += (double)lane_of(av[g], c) * (double)lane_of(bv[g], c);
-        }
-    }
-    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
-        const int64_t i = 4 * n4 + threadIdx.x;
-        if (nbinfo[i] & NB_FLUID) d += (double)a[i] * (double)b[i];
-    }
-    block_finish<AUX_NONE>(d, 0.0, red);
-}
 
 // z = y + omega (r - A y) / diag(A) on WATER cells (0 elsewhere), same stencil and item split as k_spmv
 constexpr float kLearnedOmega = 2.0f / 3.0f;
@@ -759,6 +786,12 @@ static int build_nbinfo(fsim_ctx* ctx, const SolveArgs& S, bool with_rhs) {
     A.pressure_enabled = ctx->prm.pressure_enabled;
     A.red = reduce_to(ctx, FIN_BB);
     hipLaunchKernelGGL(k_nbinfo_rhs, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, A, with_rhs ? 1 : 0);
+    if (S.nc % 4 == 0)
+        hipLaunchKernelGGL(k_chunk_flags<stream_chunk(true)>, dim3(nblocks(S.nc / 4)), dim3(kBlock), 0, ctx->stream, S.nc,
+                           ctx->nbinfo, ctx->chunk_flags);
+    else
+        hipLaunchKernelGGL(k_chunk_flags<stream_chunk(false)>, dim3(nblocks(S.nc)), dim3(kBlock), 0, ctx->stream, S.nc,
+                           ctx->nbinfo, ctx->chunk_flags);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
@@ -816,8 +849,13 @@ static int true_residual(fsim_ctx* ctx, const SolveArgs& S, bool replace) {
 
 static int dot_with(fsim_ctx* ctx, const float* a, const float* b, int op) {
     FSIM_PROF(ctx, KID_DOT);
-    hipLaunchKernelGGL(k_dot, dim3(red_blocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, ctx->g.nc, ctx->nbinfo, a, b,
-                       reduce_to(ctx, op));
+    const int64_t nc = ctx->g.nc;
+    if (nc % 4 == 0)
+        hipLaunchKernelGGL(k_dot<true>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->chunk_flags, a, b,
+                           reduce_to(ctx, op));
+    else
+        hipLaunchKernelGGL(k_dot<false>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->chunk_flags, a, b,
+                           reduce_to(ctx, op));
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
@@ -940,8 +978,12 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
         FSIM_TRY(update_x_beside(ctx));
         {
             FSIM_PROF(ctx, KID_UPDATE);
-            hipLaunchKernelGGL(k_update_r, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->nbinfo, ctx->q,
-                               ctx->r, reduce_to(ctx, FIN_RES));
+            if (nc % 4 == 0)
+                hipLaunchKernelGGL(k_update_r<true>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
+                                   ctx->chunk_flags, ctx->q, ctx->r, reduce_to(ctx, FIN_RES));
+            else
+                hipLaunchKernelGGL(k_update_r<false>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
+                                   ctx->chunk_flags, ctx->q, ctx->r, reduce_to(ctx, FIN_RES));
         }
         FSIM_HIP(ctx, hipGetLastError());
         FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
