@@ -501,6 +501,17 @@ __global__ void __launch_bounds__(kBlock) k_spmv(SolveArgs S, const uint16_t* __
 // loads before using any, so several memory round trips are in flight.
 constexpr int kUnits = 4;
 __host__ __device__ constexpr int stream_chunk(bool vec4) { return vec4 ? 256 : 64; }
+// 1024 measured fastest at 256^3 (the dot: 38 us per launch with 2048 blocks, 29 us with 1024;
+// 512 is slower again)
+#ifndef FSIM_STREAM_BLOCKS
+#define FSIM_STREAM_BLOCKS 1024
+#endif
+static_assert(FSIM_STREAM_BLOCKS % 8 == 0 && FSIM_STREAM_BLOCKS <= (int)kMaxParts, "stream grid within the reduction");
+// blocks of the flagged-chunk kernels (at most kMaxParts, a multiple of 8)
+static unsigned stream_blocks(int64_t nc) {
+    const unsigned need = (unsigned)((nc / stream_chunk(true) + kUnits * 4 - 1) / (kUnits * 4));
+    return std::max(8u, std::min<unsigned>((need + 7) / 8 * 8, FSIM_STREAM_BLOCKS));
+}
 
 template <int CH>
 __global__ void __launch_bounds__(kBlock) k_chunk_flags(int64_t nc, const uint16_t* __restrict__ nbinfo,
@@ -760,6 +771,12 @@ static int check_wave_err(fsim_ctx* ctx) {
 // per CU keep HBM busy), a multiple of 8 for the XCD-aware row split, and
 // the last block sums the partials in one pass.
 static unsigned red_blocks(int64_t n) { return std::min<unsigned>((nblocks(n) + 7) / 8 * 8, kMaxParts); }
+// 2048 and 1024 measure alike at 256^3, 512 is slower
+#ifndef FSIM_SPMV_BLOCKS
+#define FSIM_SPMV_BLOCKS 2048
+#endif
+static_assert(FSIM_SPMV_BLOCKS % 8 == 0 && FSIM_SPMV_BLOCKS <= (int)kMaxParts, "SpMV grid within the reduction");
+static unsigned spmv_blocks(int64_t n) { return std::min<unsigned>(red_blocks(n), FSIM_SPMV_BLOCKS); }
 
 static Reduce reduce_to(fsim_ctx* ctx, int op) { return Reduce{ctx->partials, ctx->red_count, ctx->scal, op}; }
 
@@ -851,10 +868,10 @@ static int dot_with(fsim_ctx* ctx, const float* a, const float* b, int op) {
     FSIM_PROF(ctx, KID_DOT);
     const int64_t nc = ctx->g.nc;
     if (nc % 4 == 0)
-        hipLaunchKernelGGL(k_dot<true>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->chunk_flags, a, b,
+        hipLaunchKernelGGL(k_dot<true>, dim3(stream_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->chunk_flags, a, b,
                            reduce_to(ctx, op));
     else
-        hipLaunchKernelGGL(k_dot<false>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->chunk_flags, a, b,
+        hipLaunchKernelGGL(k_dot<false>, dim3(stream_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc, ctx->chunk_flags, a, b,
                            reduce_to(ctx, op));
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
@@ -866,7 +883,7 @@ static int dot_zr(fsim_ctx* ctx, int op) { return dot_with(ctx, ctx->z, ctx->r, 
 static int direction_step(fsim_ctx* ctx, const SolveArgs& S, bool copy) {
     {
         FSIM_PROF(ctx, KID_SPMV);
-        const dim3 grid(red_blocks(S.nc)), block(kBlock);
+        const dim3 grid(spmv_blocks(S.nc)), block(kBlock);
         const Reduce red = reduce_to(ctx, FIN_ALPHA);
         if (copy)
             hipLaunchKernelGGL(k_spmv<SPMV_COPY>, grid, block, 0, ctx->stream, S, ctx->nbinfo, ctx->z, ctx->s, ctx->s2,
@@ -979,10 +996,10 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
         {
             FSIM_PROF(ctx, KID_UPDATE);
             if (nc % 4 == 0)
-                hipLaunchKernelGGL(k_update_r<true>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
+                hipLaunchKernelGGL(k_update_r<true>, dim3(stream_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
                                    ctx->chunk_flags, ctx->q, ctx->r, reduce_to(ctx, FIN_RES));
             else
-                hipLaunchKernelGGL(k_update_r<false>, dim3(red_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
+                hipLaunchKernelGGL(k_update_r<false>, dim3(stream_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
                                    ctx->chunk_flags, ctx->q, ctx->r, reduce_to(ctx, FIN_RES));
         }
         FSIM_HIP(ctx, hipGetLastError());
