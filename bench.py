@@ -121,7 +121,7 @@ def cpu_baseline(scene, threads: int) -> tuple[dict, dict]:
 
 COPY_BYTES = 1 << 30   # bytes read (and written) by each calibration copy
 PMC_TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                                "sweep_fwd_256_pmc_traffic.md")
+                                "pcg_kernels_256_pmc_traffic.md")
 
 
 def copy_calibration() -> None:
@@ -137,23 +137,22 @@ def copy_calibration() -> None:
 
 
 def pmc_traffic(kernel: str, size: int) -> float | None:
-    """HBM bytes per launch of `kernel` at `size`^3 from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if it covers this run."""
+    """HBM bytes per launch of `kernel` at `size`^3 from the committed rocprofv3 PMC summary (its table row:
+    FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), if the summary covers this run."""
     try:
         lines = open(PMC_TRAFFIC_FILE).read().splitlines()
     except OSError:
         return None
-    kv = {}
+    if not any(ln.strip() == f"- size: {size}" for ln in lines):
+        return None
     for ln in lines:
-        if ":" in ln and not ln.startswith("#"):
-            k, v = ln.split(":", 1)
-            kv[k.strip().lstrip("- ").strip("`")] = v.strip().strip("`")
-    if kv.get("kernel") != kernel or kv.get("size") != str(size):
-        return None
-    try:
-        return float(kv["traffic_bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].split(" ")[0] == kernel:
+            try:
+                return float(cells[4].replace(",", ""))
+            except ValueError:
+                return None
+    return None
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16
