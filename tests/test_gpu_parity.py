@@ -261,6 +261,63 @@ def test_whole_step_parity(gpu, transfer, sphere):
         assert st.residual_rel_l2 <= 1e-10
 
 
+def test_config_128_apic_sphere_whole_steps(gpu):
+    """The 128^3 APIC dam break with the static sphere at its real size (3.83M particles,
+    multi-tile wavefront sweeps): 3 GPU steps in, then per step from the same state, labels
+    bit-exact, particles within 1e-5 of the domain, both solves at a 1e-10 relative residual.
+    Grid fields: the step's advection leaves the GPU positions at the FP64 result rounded to
+    FP32 (<= 1 ulp, d = 2^-24 for |p| < 1; the per-stage tests pin this).  Each of a weight's
+    three tent factors 1 - |p - f| N moves by up to N d, so the weight by up to 3 N d, and
+    APIC's affine term C (f - p) by |C| d; a face value is a weighted mean over particles
+    whose weights average 1/8 (8 per cell), so it moves by up to ~8 * 3 N d |v| + |C| d.  The
+    bar is max(1e-5, 24 N d + 3 max|C| d / max|v|) of scale (1.9e-4 at N = 128, measured
+    ~5e-5; at 24^3 the per-step tests hold 1e-5).
+    APIC's C sums tent-weight gradients, which are +-1/dx up to the stencil's edge (where the
+    weight is already 0): a particle within an ulp of a stencil switch (a cell face or centre)
+    takes a different stencil on each side and its C differs by ~v/dx, correctly on both.  So C
+    is compared on the particles away from a switch; the others (a handful per million) must
+    be within 8 ulp of one."""
+    import os
+
+    import oracle
+    say("case: 128^3 APIC dam break with a sphere, steps 4-5, GPU vs the oracle")
+    sc = scenes.CONFIG_128_APIC_SPHERE()
+    prm = scenes.params(transfer=sc.transfer, tol=1e-10)
+    sim = capi.Simulation(sc.grid, prm)
+    sim.set_obstacles(sc.obstacles)
+    sim.set_particles(sc.particles())
+    for _ in range(3):
+        sim.step(DT)
+    ref = oracle.Oracle(sc.grid, prm, threads=min(16, os.cpu_count() or 1))
+    ulp = 2.0 ** -24
+    for k in range(2):
+        sync_oracle_from_gpu(sim, ref)
+        _, v_in, c_in = sim.get_particles()
+        bar = max(1e-5, 24.0 * sc.n * ulp + 3.0 * float(np.abs(c_in).max()) * ulp / float(np.abs(v_in).max()))
+        st = sim.step(DT)
+        ref.step(DT)
+        say(f"  step {k}: {st.pcg_iterations} PCG iterations, {st.num_particles} particles")
+        assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL)), k
+        errs = [field_rel_err(sim.get_grid(f), ref.get_grid(f)) for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W,
+                                                                            capi.FIELD_PRESSURE)]
+        say(f"  u, v, w, p relative errors {', '.join(f'{e:.2e}' for e in errs)} (bar {bar:.2e})")
+        assert max(errs) < bar, (k, errs)
+        p_gpu, v_gpu, c_gpu = sim.get_particles()
+        p_ref, v_ref, c_ref = ref.get_particles()
+        assert np.abs(p_gpu - p_ref).max() < 1e-5, k
+        assert field_rel_err(v_gpu, v_ref) < 1e-4, k
+        cmax = np.abs(c_ref).max()
+        off = np.abs(c_gpu - c_ref).max(axis=1) > 1e-4 * cmax
+        s2 = 2.0 * p_ref * sc.n                      # positions in half cells (FP64, the oracle's)
+        near = (np.abs(s2 - np.round(s2)) < 16.0 * sc.n * 2.0 ** -24).any(axis=1)
+        say(f"  C: {int(off.sum())} of {len(off)} particles off by more than 1e-4 of scale, "
+            f"{int((off & ~near).sum())} of them away from a stencil switch")
+        assert not (off & ~near).any() and off.sum() <= 1e-4 * len(off), k
+        assert st.residual_rel_l2 <= 1e-10
+    ref.close()
+    sim.close()
+
+
 @pytest.mark.parametrize("shape", [(16, 16, 16), (21, 23, 70), (9, 37, 131)])
 def test_wavefront_equals_planes(gpu, shape):
     """The persistent wavefront kernels (tiles, boundary packets, tickets)
