@@ -367,66 +367,70 @@ __global__ void k_flip_weights(const __bf16* __restrict__ w, __bf16* __restrict_
     wb[i] = w[(int64_t)co * 27 * cin + (int64_t)(26 - tp) * cin + ci];
 }
 
-// Weight gradient of one 3x3x3 convolution, FP32: dW[co][t * Cin + ci] = sum_v g[v][co] x[v + off(t)][ci].
-// Block (tap t, voxel chunk): stages kChunk voxels of g and of the shifted x in LDS, each thread
-// accumulates its (co, ci) outputs; partials[chunk][co][t * Cin + ci] are summed in chunk order later.
-constexpr int kWgChunk = 32;
-__global__ void __launch_bounds__(256) k_conv3_wgrad(const float* __restrict__ g, const __bf16* __restrict__ x,
-                                                     float* __restrict__ partials, Dims d, int cin, int cout,
-                                                     int64_t vox_per_block) {
-    extern __shared__ float lds[];
-    float* sg = lds;                        // [kWgChunk][cout]
-    float* sx = lds + kWgChunk * cout;      // [kWgChunk][cin]
+// Weight gradient of one 3x3x3 convolution on MFMA: dW[co][t][ci] = sum_v g[v][co] x[v + off(t)][ci],
+// one GEMM per tap with M = Cout, N = Cin, K = voxels.  Block (voxel slice, tap t): its 4 waves take
+// the slice's 32-voxel chunks in turn, each accumulating every 16 x 16 (co, ci) tile with
+// v_mfma_f32_16x16x32_bf16; a wave's partial goes to partials[slice * 4 + wave][co][t][ci] and the
+// partials are summed in that order (k_sum_parts).  A (16 co x 32 v): lane l holds co = 16 mi + l % 16
+// at voxels v0 + 8 (l / 16) .. + 7; B (32 v x 16 ci): ci = 16 ni + l % 16 at the same voxels,
+// shifted by the tap (zero outside the grid; Cin = 8 is padded to one 16-wide tile).  Operands are
+// the layer's BF16 output gradient and its BF16 input activations, sums in FP32.
+constexpr int kWgWaves = 4;
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv3_wgrad(const __bf16* __restrict__ g, const __bf16* __restrict__ x,
+                                                     float* __restrict__ partials, Dims d, int64_t vox_per_block) {
+    constexpr int MT = COUT / 16, NT = (CIN + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = blockIdx.y;
     const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
     const int64_t V = d.V();
-    const int64_t v_begin = (int64_t)blockIdx.x * vox_per_block;
-    const int64_t v_end = min(V, v_begin + vox_per_block);
-    const int nout = cout * cin;
-    constexpr int kMaxPer = (64 * 192 + 255) / 256;   // largest cout * cin of the network (layer 4)
-    float acc[kMaxPer];
+    const int64_t vb = (int64_t)blockIdx.x * vox_per_block, ve = min(V, vb + vox_per_block);
+    const int cl = lane & 15, kq = lane >> 4;
+    const __bf16 zero = (__bf16)0.f;
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int k = 0; k < kMaxPer; k++) acc[k] = 0.f;
-    for (int64_t v0 = v_begin; v0 < v_end; v0 += kWgChunk) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < kWgChunk * cout; e += blockDim.x) {
-            const int j = e / cout, co = e - j * cout;
-            const int64_t v = v0 + j;
-            sg[e] = v < v_end ? g[v * cout + co] : 0.f;
-        }
-        for (int e = threadIdx.x; e < kWgChunk * cin; e += blockDim.x) {
-            const int j = e / cin, ci = e - j * cin;
-            const int64_t v = v0 + j;
-            float val = 0.f;
-            if (v < v_end) {
+    for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+        for (int ni = 0; ni < NT; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t v0 = vb + 32 * wave; v0 < ve; v0 += 32 * kWgWaves) {
+        bf16x8 a[MT], b[NT];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int64_t v = v0 + 8 * kq + j;
+            const bool vin = v < ve;
+            int64_t sv = -1;   // the tap's neighbour of voxel v
+            if (vin) {
                 const int z = (int)(v % d.W), y = (int)((v / d.W) % d.H), xx = (int)(v / ((int64_t)d.W * d.H));
                 const int X = xx + dx, Y = y + dy, Z = z + dz;
-                if (X >= 0 && X < d.D && Y >= 0 && Y < d.H && Z >= 0 && Z < d.W)
-                    val = (float)x[(((int64_t)X * d.H + Y) * d.W + Z) * cin + ci];
+                if ((unsigned)X < (unsigned)d.D && (unsigned)Y < (unsigned)d.H && (unsigned)Z < (unsigned)d.W)
+                    sv = ((int64_t)X * d.H + Y) * d.W + Z;
             }
-            sx[e] = val;
-        }
-        __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kMaxPer; k++) {
-            const int o = threadIdx.x + k * 256;
-            if (o < nout) {
-                const int co = o / cin, ci = o - co * cin;
-                float s = acc[k];
-                for (int j = 0; j < kWgChunk; j++) s += sg[j * cout + co] * sx[j * cin + ci];
-                acc[k] = s;
+            for (int mi = 0; mi < MT; mi++) a[mi][j] = vin ? g[v * COUT + 16 * mi + cl] : zero;
+#pragma unroll
+            for (int ni = 0; ni < NT; ni++) {
+                const int ci = 16 * ni + cl;
+                b[ni][j] = sv >= 0 && ci < CIN ? x[sv * CIN + ci] : zero;
             }
         }
-    }
-    float* out = partials + (int64_t)blockIdx.x * cout * 27 * cin;
 #pragma unroll
-    for (int k = 0; k < kMaxPer; k++) {
-        const int o = threadIdx.x + k * 256;
-        if (o < nout) {
-            const int co = o / cin, ci = o - co * cin;
-            out[(int64_t)co * 27 * cin + (int64_t)t * cin + ci] = acc[k];
-        }
+        for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+            for (int ni = 0; ni < NT; ni++) acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
     }
+    float* out = partials + ((int64_t)blockIdx.x * kWgWaves + wave) * COUT * 27 * CIN;
+#pragma unroll
+    for (int mi = 0; mi < MT; mi++)
+#pragma unroll
+        for (int ni = 0; ni < NT; ni++) {
+            const int ci = 16 * ni + cl;
+            if (ci >= CIN) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int co = 16 * mi + 4 * kq + i;
+                out[(int64_t)co * 27 * CIN + (int64_t)t * CIN + ci] = acc[mi][ni][i];
+            }
+        }
 }
 
 // sum of nparts partial vectors of length n, in partial order
@@ -855,19 +859,35 @@ int learned_forward_host(fsim_ctx* ctx, const float* b, const uint8_t* label, fl
     return FSIM_OK;
 }
 
-static int wgrad(fsim_ctx* ctx, int l, const float* g, const __bf16* x) {
+// weight and bias gradients of layer l from its output gradient (FP32 g32, BF16 g16) and input x
+static int wgrad(fsim_ctx* ctx, int l, const float* g32, const __bf16* g16, const __bf16* x) {
     LearnedNet* N = ctx->learned;
     const Dims d = N->lv[kLayers[l].level];
     const int cin = kLayers[l].cin, cout = kLayers[l].cout;
     const int64_t V = d.V();
     const int64_t n = (int64_t)cout * 27 * cin;
-    int nblk = (int)std::min<int64_t>(64, (V + 1023) / 1024);
-    while ((int64_t)nblk * n > N->wparts_cap) nblk--;
+    // voxel slices: at least one 32-voxel chunk per wave, at most 16, within the partial buffer
+    int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(16, V / (32 * kWgWaves)));
+    while (nblk > 1 && (int64_t)nblk * kWgWaves * n > N->wparts_cap) nblk--;
+    if ((int64_t)nblk * kWgWaves * n > N->wparts_cap)
+        return set_err(ctx, FSIM_ERR_INVALID, "learned network: weight-gradient partials exceed their buffer");
     const int64_t per = (V + nblk - 1) / nblk;
-    const size_t lds = sizeof(float) * kWgChunk * (cin + cout);
-    hipLaunchKernelGGL(k_conv3_wgrad, dim3(nblk, 27), dim3(256), lds, ctx->stream, g, x, N->wparts, d, cin, cout, per);
-    hipLaunchKernelGGL(k_sum_parts, dim3(gb(n)), dim3(256), 0, ctx->stream, N->wparts, nblk, n, N->grad + N->woff[l]);
-    hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, ctx->stream, g, V, cout, N->grad + N->boff[l]);
+    const dim3 grid(nblk, 27), block(256);
+    bool ok = true;
+    switch (l) {
+    case 0: hipLaunchKernelGGL((k_conv3_wgrad<8, 16>), grid, block, 0, ctx->stream, g16, x, N->wparts, d, per); break;
+    case 1: hipLaunchKernelGGL((k_conv3_wgrad<16, 32>), grid, block, 0, ctx->stream, g16, x, N->wparts, d, per); break;
+    case 2: hipLaunchKernelGGL((k_conv3_wgrad<32, 64>), grid, block, 0, ctx->stream, g16, x, N->wparts, d, per); break;
+    case 3: hipLaunchKernelGGL((k_conv3_wgrad<64, 128>), grid, block, 0, ctx->stream, g16, x, N->wparts, d, per); break;
+    case 4: hipLaunchKernelGGL((k_conv3_wgrad<192, 64>), grid, block, 0, ctx->stream, g16, x, N->wparts, d, per); break;
+    case 5: hipLaunchKernelGGL((k_conv3_wgrad<96, 32>), grid, block, 0, ctx->stream, g16, x, N->wparts, d, per); break;
+    case 6: hipLaunchKernelGGL((k_conv3_wgrad<48, 16>), grid, block, 0, ctx->stream, g16, x, N->wparts, d, per); break;
+    default: ok = false;
+    }
+    if (!ok) return set_err(ctx, FSIM_ERR_INVALID, "learned network: no weight-gradient kernel for this layer");
+    hipLaunchKernelGGL(k_sum_parts, dim3(gb(n)), dim3(256), 0, ctx->stream, N->wparts, nblk * kWgWaves, n,
+                       N->grad + N->woff[l]);
+    hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, ctx->stream, g32, V, cout, N->grad + N->boff[l]);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
@@ -918,7 +938,7 @@ static int net_backward(fsim_ctx* ctx, double* loss) {
         const int lev = kLayers[l].level;
         const int64_t n = L[lev].V() * kLayers[l].cout;
         hipLaunchKernelGGL(k_relu_bwd, dim3(gb(n)), dim3(256), 0, st, dy, N->dd[lev], N->g32[l], N->g16[l], n);
-        FSIM_TRY(wgrad(ctx, l, N->g32[l], N->cat[lev]));
+        FSIM_TRY(wgrad(ctx, l, N->g32[l], N->g16[l], N->cat[lev]));
         FSIM_TRY(dgrad(ctx, l, N->g16[l], N->dcat[lev]));
         const int Cs = kLayers[l].cin / 3, Cu = kLayers[l].cin - Cs;
         hipLaunchKernelGGL(k_upcat_bwd_up, dim3(gb(L[lev + 1].V() * Cu)), dim3(256), 0, st, N->dcat[lev], N->dcoarse[lev],
@@ -931,7 +951,7 @@ static int net_backward(fsim_ctx* ctx, double* loss) {
     {
         const int64_t n = L[3].V() * 128;
         hipLaunchKernelGGL(k_relu_bwd, dim3(gb(n)), dim3(256), 0, st, N->dcoarse[2], N->e[3], N->g32[3], N->g16[3], n);
-        FSIM_TRY(wgrad(ctx, 3, N->g32[3], N->pl[2]));
+        FSIM_TRY(wgrad(ctx, 3, N->g32[3], N->g16[3], N->pl[2]));
         FSIM_TRY(dgrad(ctx, 3, N->g16[3], N->dpool[2]));
     }
     // encoder layers 2, 1, 0: gradient of e[l] = skip gradient + pooled-input gradient of layer l + 1
@@ -939,7 +959,7 @@ static int net_backward(fsim_ctx* ctx, double* loss) {
         const int C = kLayers[l].cout;
         hipLaunchKernelGGL(k_pool_bwd_relu, dim3(gb(L[l].V() * C)), dim3(256), 0, st, N->dskip[l], N->dpool[l], N->e[l],
                            N->g32[l], N->g16[l], L[l], C);
-        FSIM_TRY(wgrad(ctx, l, N->g32[l], l == 0 ? N->x0 : N->pl[l - 1]));
+        FSIM_TRY(wgrad(ctx, l, N->g32[l], N->g16[l], l == 0 ? N->x0 : N->pl[l - 1]));
         if (l > 0) FSIM_TRY(dgrad(ctx, l, N->g16[l], N->dpool[l - 1]));
     }
     FSIM_HIP(ctx, hipGetLastError());
