@@ -158,39 +158,79 @@ def pmc_traffic(kernel: str, size: int) -> float | None:
 MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16
 
 
-def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int) -> dict:
+def _state_after(sc, steps: int):
+    """Particle state of scene `sc` after `steps` MIC(0)-PCG steps."""
+    sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
+    if sc.obstacles:
+        sim.set_obstacles(sc.obstacles)
+    sim.set_particles(sc.particles())
+    for _ in range(steps):
+        sim.step(scenes.DT)
+    state = sim.get_particles()
+    sim.close()
+    return state
+
+
+def _fpcg_iterations(sc, state, params, budget: int):
+    """FPCG iterations of one step from `state` on a fresh context (None: not converged within `budget`)."""
+    sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED,
+                                                 max_iterations=budget))
+    if sc.obstacles:
+        sim.set_obstacles(sc.obstacles)
+    sim.learned_set_params(params)
+    sim.set_particles(*state)
+    st = sim.step(scenes.DT)
+    sim.close()
+    return st.learned_iterations if st.learned_stop == 0 else None
+
+
+def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int, seeds: int = 3) -> dict:
     """Pressure-solve ms to 1e-6 relative residual, MIC(0)-PCG vs the learned FPCG, on the same right-hand side.
 
     The network is trained here on the GPU with the energy-norm loss (generate_samples + train: real
-    MIC(0)-PCG solves of seeded random dam breaks at train_n^3 and residual pairs), then set on an n^3
-    context.  The n^3 scene is the APIC dam break with a static sphere; it is stepped 3 times with
-    MIC(0)-PCG, its particle state is saved, and the next step is solved from that same state with each
-    preconditioner (the second of two runs is timed, so nothing is allocated in it).  `learned` is the
-    production path: FPCG with the MIC(0)-PCG fallback after twice the last MIC(0) iteration count.
-    `fpcg_iterations` compares the FPCG iteration counts, on fresh contexts with a budget of 10x the MIC(0)
-    count, of the trained network, the untrained (initial) network and an all-zero network (the same
-    solver with the damped-Jacobi steps alone)."""
+    MIC(0)-PCG solves of seeded random dam breaks at train_n^3 and residual pairs) from `seeds` initial
+    seeds; each is validated by its FPCG iteration count on a 64^3 double dam break (another size and
+    scene than the test), and the best is set on an n^3 context (model selection: a trained network
+    can stall FPCG, and the validation solve shows it).  The n^3 scene is the APIC dam break with a
+    static sphere; it is stepped 3 times with MIC(0)-PCG, its particle state is saved, and the next
+    step is solved from that same state with each preconditioner (the second of two runs is timed, so
+    nothing is allocated in it).  `learned` is the production path: FPCG with the MIC(0)-PCG fallback
+    after twice the last MIC(0) iteration count.  `fpcg_iterations` compares the FPCG iteration
+    counts, on fresh contexts with a budget of 10x the MIC(0) count, of the selected network, an
+    untrained (initial) network and an all-zero network (the same solver with its damped-Jacobi steps
+    alone)."""
     t0 = time.perf_counter()
     tgrid, B, L, P = learned.generate_samples(train_n, train_scenes, 4, seed=0, residuals_per_solve=4)
     t_gen = time.perf_counter() - t0
-    init = capi.Simulation(tgrid, scenes.params())
-    init.learned_init(1)
-    untrained = init.learned_get_params()
-    init.close()
+    val_sc = scenes.double_dam_break(64, capi.FLIP, seed=2)
+    val_state = _state_after(val_sc, 3)
+    cands = []
     t0 = time.perf_counter()
-    params, hist = learned.train(tgrid, B, L, P, epochs=epochs, batch=4, lr=1e-3, seed=1, params=untrained,
-                                 loss=learned.LOSS_ENERGY)
+    for seed in range(1, seeds + 1):
+        init = capi.Simulation(tgrid, scenes.params())
+        init.learned_init(seed)
+        p0 = init.learned_get_params()
+        init.close()
+        t1 = time.perf_counter()
+        prm, hist = learned.train(tgrid, B, L, P, epochs=epochs, batch=4, lr=1e-3, seed=seed, params=p0,
+                                  loss=learned.LOSS_ENERGY)
+        t_tr = time.perf_counter() - t1
+        its = _fpcg_iterations(val_sc, val_state, prm, 400)
+        cands.append({"seed": seed, "params": prm, "init": p0, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+                      "train_s": round(t_tr, 2), "validation_iterations": its})
+        print(f"# learned seed {seed}: loss {hist[0]:.3e} -> {hist[-1]:.3e} in {t_tr:.1f}s, "
+              f"validation FPCG {its} it", file=sys.stderr, flush=True)
     t_train = time.perf_counter() - t0
-    print(f"# learned: {len(B)} samples at {train_n}^3 in {t_gen:.1f}s, {epochs} epochs in {t_train:.1f}s, "
-          f"energy loss {hist[0]:.3e} -> {hist[-1]:.3e}", file=sys.stderr, flush=True)
+    ok = [c for c in cands if c["validation_iterations"] is not None]
+    best = min(ok, key=lambda c: c["validation_iterations"]) if ok else cands[0]
+    params, untrained = best["params"], cands[0]["init"]
+    print(f"# learned: {len(B)} samples at {train_n}^3 in {t_gen:.1f}s, {seeds} x {epochs} epochs in {t_train:.1f}s, "
+          f"selected seed {best['seed']}", file=sys.stderr, flush=True)
 
     sc = scenes.CONFIG_128_APIC_SPHERE() if n == 128 else scenes.dam_break(n, capi.APIC, seed=0, sphere=True)
+    state = _state_after(sc, 3)
     sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
     sim.set_obstacles(sc.obstacles)
-    sim.set_particles(sc.particles())
-    for _ in range(3):
-        sim.step(scenes.DT)
-    state = sim.get_particles()
     res = {}
 
     def solve(name, precond, prm=None, profile=False):
@@ -218,16 +258,8 @@ def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int) 
     kt = solve("learned", capi.PRECOND_LEARNED, params, profile=True)
     k_mic = res["mic0_pcg"]["pcg_iterations"]
     sim.close()
-    iters = {}
-    for name, prm in (("trained", params), ("untrained", untrained), ("zero_network", np.zeros_like(params))):
-        alone = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED,
-                                                         max_iterations=10 * k_mic))
-        alone.set_obstacles(sc.obstacles)
-        alone.learned_set_params(prm)
-        alone.set_particles(*state)
-        st = alone.step(scenes.DT)
-        alone.close()
-        iters[name] = st.learned_iterations if st.learned_stop == 0 else None
+    iters = {name: _fpcg_iterations(sc, state, prm, 10 * k_mic)
+             for name, prm in (("trained", params), ("untrained", untrained), ("zero_network", np.zeros_like(params)))}
     iters["budget"] = 10 * k_mic
     iters["mic0_pcg"] = k_mic
     print(f"# {n}^3 FPCG iterations (None: not converged within the budget): {iters}", file=sys.stderr, flush=True)
@@ -247,7 +279,9 @@ def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int) 
             "learned_over_mic0_time": round(l_ms / mic_ms, 3) if mic_ms > 0 else None,
             "training": {"samples": len(B), "grid": [train_n] * 3, "epochs": epochs, "loss": "energy norm",
                          "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
-                         "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]},
+                         "validation": "FPCG iterations (budget 400) of the 4th step of a 64^3 FLIP double dam break",
+                         "selected_seed": best["seed"],
+                         "candidates": [{k: v for k, v in c.items() if k not in ("params", "init")} for c in cands]},
             "network": network}
 
 
@@ -268,7 +302,7 @@ def main():
                     help="skip the pressure-solve comparison MIC(0)-PCG vs learned FPCG")
     ap.add_argument("--learned-size", type=int, default=128)
     ap.add_argument("--learned-train-size", type=int, default=32)
-    ap.add_argument("--learned-scenes", type=int, default=30)
+    ap.add_argument("--learned-scenes", type=int, default=60)
     ap.add_argument("--learned-epochs", type=int, default=15)
     args = ap.parse_args()
     if args.copy_calibration:
