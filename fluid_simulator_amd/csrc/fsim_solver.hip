@@ -917,6 +917,12 @@ static int join_x(fsim_ctx* ctx) {
     FSIM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->x_ev, 0));
     return FSIM_OK;
 }
+// joins the x stream on every exit of a solve, error returns included, so that no later work on the
+// main stream (the next solve clears x) runs beside a pending x update
+struct JoinXOnExit {
+    fsim_ctx* ctx;
+    ~JoinXOnExit() { (void)hipStreamWaitEvent(ctx->stream, ctx->x_ev, 0); }
+};
 
 static bool stop_met(const fsim_ctx* ctx, double ss, double mx, double bnorm) {
     if (ctx->prm.stop_rule == FSIM_STOP_REL_L2) return std::sqrt(ss) <= ctx->prm.residual_tolerance * bnorm;
@@ -972,6 +978,7 @@ struct PcgResult {
 // results are discarded.  x += alpha s runs on the second stream
 // (update_x_beside) and is joined before the next direction step.
 static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexible, int max_it, PcgResult* res) {
+    const JoinXOnExit join_on_exit{ctx};
     const int64_t nc = S.nc;
     fsim_step_stats& st = ctx->stats;
     FSIM_TRY(fill_f64(ctx, ctx->x, 0.0, nc));
