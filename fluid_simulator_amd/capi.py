@@ -113,7 +113,7 @@ PROTOTYPES = {
     "fsim_get_kernel_time": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "fsim_reset_kernel_times": (C.c_int, [_P]),
 }
-OPT_SWEEP_MODE, OPT_PROFILE, OPT_LEARNED_SMOOTHING, OPT_LEARNED_LOSS = 0, 1, 2, 3
+OPT_SWEEP_MODE, OPT_PROFILE, OPT_LEARNED_SMOOTHING, OPT_LEARNED_LOSS, OPT_LEARNED_CONV = 0, 1, 2, 3, 4
 
 _lib = None
 

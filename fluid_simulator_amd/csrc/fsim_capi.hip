@@ -523,6 +523,10 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
         if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "learned loss is 0 (MSE) or 1 (energy)");
         ctx->learned_loss = value;
         return FSIM_OK;
+    case FSIM_OPT_LEARNED_CONV:
+        if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "learned convolution kernel is 0 or 1");
+        ctx->learned_conv_lds = value;
+        return FSIM_OK;
     default: return set_err(ctx, FSIM_ERR_INVALID, "unknown option");
     }
 }

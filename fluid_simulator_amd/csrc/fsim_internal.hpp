@@ -210,6 +210,7 @@ struct fsim_ctx {
     int sweep_mode = 0;                          // 0 wavefront, 1 one launch per plane
     int learned_smoothing = 4;                   // damped-Jacobi steps after the network
     int learned_loss = 0;                        // training loss: 0 masked MSE, 1 energy norm
+    int learned_conv_lds = 1;                    // 1: LDS-staged convolutions where they fit
 
     // obstacles
     std::vector<fsim_obstacle> obs;

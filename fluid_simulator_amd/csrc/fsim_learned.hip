@@ -89,6 +89,7 @@ struct ConvArgs {
     int relu;
     const __bf16* xc;     // coarse parent [V/8][cu] when cu > 0
     int cu;
+    int lds;              // 1: the LDS-staged kernel where the layer and the level fit it
 };
 
 template <int CIN, int CU, int NT, int MT, bool OUT_F32>
@@ -176,6 +177,143 @@ static void conv3_launch(hipStream_t st, const ConvArgs& a) {
                        dim3(256), 0, st, a);
 }
 
+// LDS-staged variant for the large levels (rows a multiple of 32 voxels).  A block owns TY rows x 32
+// voxels of XS consecutive x planes and slides along x with a ring of four halo planes in LDS, each
+// (TY + 2) x 34 voxels x all CIN channels (zeros outside the grid; a decoder layer's channels below
+// CU from the coarse parent): per output plane it loads the one new input plane x + 1, waits at one
+// barrier, and its waves read the MFMA A operands from the planes x - 1 .. x + 1, one 16-byte LDS read
+// per lane and K chunk.  The slot written at plane x (ring index x + 1) was last read at plane x - 2,
+// which every wave finished before the previous barrier.  The voxel stride is padded by 8 channels when
+// CIN / 2 dwords is a multiple of 16, so that the eight lanes of a 16-byte read hit distinct banks.
+// K order and MFMA sequence are k_conv3's, so the output is bit-identical.
+constexpr int kLdsTZ = 32, kLdsSlots = 4;
+template <int CIN>
+constexpr int conv3_lds_stride() { return (CIN / 2) % 16 == 0 ? CIN + 8 : CIN; }
+template <int CIN, int TY>
+constexpr size_t conv3_lds_bytes() {
+    return (size_t)kLdsSlots * (TY + 2) * (kLdsTZ + 2) * conv3_lds_stride<CIN>() * sizeof(__bf16);
+}
+
+template <int CIN, int CU, int NT, int TY, int XS>
+__global__ void __launch_bounds__(256) k_conv3_lds(ConvArgs a) {
+    constexpr int TZ = kLdsTZ, HY = TY + 2, HZ = TZ + 2, PL = HY * HZ, VS = conv3_lds_stride<CIN>();
+    constexpr int MT = TY * TZ / (16 * 4);   // voxel tiles per wave
+    constexpr int KTOT = 27 * CIN, NCH = (KTOT + 31) / 32, CS = CIN - CU, RUNS = CIN / 8;
+    static_assert(CIN % 8 == 0 && MT >= 1 && TY * TZ == 64 * MT, "conv3_lds tile shape");
+    extern __shared__ uint4 lds_raw[];
+    __bf16* tile = reinterpret_cast<__bf16*>(lds_raw);   // [kLdsSlots][HY][HZ][VS]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nzb = a.W / TZ, nyb = a.H / TY;
+    const int zb = (int)(blockIdx.x % nzb), yb = (int)((blockIdx.x / nzb) % nyb);
+    const int x0 = (int)(blockIdx.x / (nzb * nyb)) * XS;
+    const int y0 = yb * TY, z0 = zb * TZ;
+    const int Hc = a.H >> 1, Wc = a.W >> 1;
+    const bf16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto load_plane = [&](int X) {
+        __bf16* slot = tile + (size_t)(X & (kLdsSlots - 1)) * PL * VS;
+        for (int e = threadIdx.x; e < PL * RUNS; e += blockDim.x) {
+            const int run = e % RUNS, hv = e / RUNS;
+            const int Y = y0 + hv / HZ - 1, Z = z0 + hv % HZ - 1;
+            const int ci = 8 * run;
+            bf16x8 val = zero;
+            if ((unsigned)X < (unsigned)a.D && (unsigned)Y < (unsigned)a.H && (unsigned)Z < (unsigned)a.W) {
+                if (CU > 0 && ci < CU)
+                    val = *reinterpret_cast<const bf16x8*>(
+                        a.xc + (((int64_t)(X >> 1) * Hc + (Y >> 1)) * Wc + (Z >> 1)) * (CU > 0 ? CU : 1) + ci);
+                else
+                    val = *reinterpret_cast<const bf16x8*>(a.x + (((int64_t)X * a.H + Y) * a.W + Z) * CS + (ci - CU));
+            }
+            *reinterpret_cast<bf16x8*>(slot + (size_t)hv * VS + ci) = val;
+        }
+    };
+    const int kq = lane >> 4;
+    int hbase[MT];   // in-plane halo voxel (ty + 1, tz + 1) of this lane's A row in each tile
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int f = 16 * (wave * MT + m) + (lane & 15);
+        hbase[m] = (f / TZ + 1) * HZ + f % TZ + 1;
+    }
+    // the block's weights stay in registers for all XS planes: B fragment of every K chunk
+    const __bf16* wl = a.w + (int64_t)(lane & 15) * KTOT + 8 * kq;
+    bf16x8 wreg[NCH][NT];
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            wreg[c][j] = ((KTOT % 32 == 0) || 32 * c + 8 * kq < KTOT)
+                             ? *reinterpret_cast<const bf16x8*>(wl + (int64_t)16 * j * KTOT + 32 * c) : zero;
+    load_plane(x0 - 1);
+    load_plane(x0);
+    for (int X = x0; X < x0 + XS; X++) {
+        load_plane(X + 1);
+        __syncthreads();
+        f32x4 acc[MT][NT];
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int j = 0; j < NT; j++) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int k0 = 32 * c + 8 * kq;
+            const bool kin = (KTOT % 32 == 0) || k0 < KTOT;
+            const int t = k0 / CIN, ci = k0 - t * CIN;
+            const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+            // a K run past the last tap reads nothing
+            const int hoff = kin ? ((X + dx) & (kLdsSlots - 1)) * PL + dy * HZ + dz : 0;
+            bf16x8 av[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+                av[m] = kin ? *reinterpret_cast<const bf16x8*>(tile + (size_t)(hbase[m] + hoff) * VS + ci) : zero;
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], wreg[c][j], acc[m][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const int co = 16 * j + (lane & 15);
+            const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int f = 16 * (wave * MT + m) + 4 * kq + i;
+                    const int64_t v = ((int64_t)X * a.H + y0 + f / TZ) * a.W + z0 + f % TZ;
+                    float o = acc[m][j][i] + b;
+                    if (a.relu) o = fmaxf(o, 0.f);
+                    static_cast<__bf16*>(a.y)[v * a.cout + co] = (__bf16)o;
+                }
+        }
+    }
+}
+
+template <int CIN, int CU, int COUT, int TY, int XS>
+static bool conv3_lds_launch(hipStream_t st, const ConvArgs& a) {
+    const size_t lds = conv3_lds_bytes<CIN, TY>();
+    // once per process: the ring must fit the device's per-block LDS (raised above the 64 KiB default
+    // for the kernel when needed); otherwise the layer stays on k_conv3
+    static const bool fits = [&] {
+        int dev = 0, max_lds = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            (size_t)max_lds < lds) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_lds<CIN, CU, COUT / 16, TY, XS>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        return true;
+    }();
+    if (!fits || !a.lds || a.W % kLdsTZ != 0 || a.H % TY != 0 || a.D % XS != 0) return false;
+    const unsigned blocks = (unsigned)((int64_t)(a.D / XS) * (a.H / TY) * (a.W / kLdsTZ));
+    hipLaunchKernelGGL((k_conv3_lds<CIN, CU, COUT / 16, TY, XS>), dim3(blocks), dim3(256), lds, st, a);
+    return true;
+}
+
 // The convolutions of the network, by (Cin, Cu, Cout) with their tile shapes
 // (NT output tiles x MT voxel tiles per wave): the 7 layers' forward
 // convolutions (BF16 out; the decoder reads cat(up, skip) directly), and the
@@ -189,6 +327,9 @@ static bool launch_conv3(hipStream_t st, const ConvArgs& a) {
         return true;                                            \
     }
     if constexpr (!OUT_F32) {
+        // the full-resolution layers through LDS when their rows are a multiple of 32 voxels
+        if (a.cin == 8 && a.cu == 0 && a.cout == 16 && conv3_lds_launch<8, 0, 16, 4, 16>(st, a)) return true;
+        if (a.cin == 48 && a.cu == 32 && a.cout == 16 && conv3_lds_launch<48, 32, 16, 4, 32>(st, a)) return true;
         FSIM_CONV3(8, 0, 16, 1, 4) FSIM_CONV3(16, 0, 32, 2, 4) FSIM_CONV3(32, 0, 64, 4, 2)
         FSIM_CONV3(64, 0, 128, 2, 1) FSIM_CONV3(192, 128, 64, 4, 2) FSIM_CONV3(96, 64, 32, 2, 4)
         FSIM_CONV3(48, 32, 16, 1, 4)
@@ -667,11 +808,11 @@ static int learned_alloc_train(fsim_ctx* ctx) {
 }
 
 // layer l reading x (or, for a decoder layer, the skip x and the coarse parent xc)
-static ConvArgs conv_args(const LearnedNet* N, int l, const __bf16* x, void* y, const __bf16* xc = nullptr) {
+static ConvArgs conv_args(const LearnedNet* N, int l, const __bf16* x, void* y, int lds, const __bf16* xc = nullptr) {
     const Dims d = N->lv[kLayers[l].level];
     const int cu = xc ? kLayers[l].cin - kLayers[l].cin / 3 : 0;
     return ConvArgs{x, N->w16 + N->woff[l], N->w32 + N->boff[l], y, d.D, d.H, d.W, kLayers[l].cin, kLayers[l].cout, 1,
-                    xc, cu};
+                    xc, cu, lds};
 }
 
 // forward from x0 (already packed) to out; for training the concatenated
@@ -680,11 +821,12 @@ static int net_forward(fsim_ctx* ctx, bool keep_cat) {
     LearnedNet* N = ctx->learned;
     hipStream_t st = ctx->stream;
     const Dims* L = N->lv;
-    bool ok = launch_conv3<false>(st, conv_args(N, 0, N->x0, N->e[0]));
+    const int lds = ctx->learned_conv_lds;
+    bool ok = launch_conv3<false>(st, conv_args(N, 0, N->x0, N->e[0], lds));
     for (int l = 1; l < 4; l++) {
         hipLaunchKernelGGL(k_pool2, dim3(gb(L[l].V() * kLayers[l].cin)), dim3(256), 0, st, N->e[l - 1], N->pl[l - 1],
                            L[l - 1], kLayers[l].cin);
-        ok = ok && launch_conv3<false>(st, conv_args(N, l, N->pl[l - 1], N->e[l]));
+        ok = ok && launch_conv3<false>(st, conv_args(N, l, N->pl[l - 1], N->e[l], lds));
     }
     // decoder: layer 4 at level 2 (up e3 + e2), 5 at level 1 (up d2 + e1), 6 at level 0 (up d1 + e0)
     const __bf16* coarse = N->e[3];
@@ -694,7 +836,7 @@ static int net_forward(fsim_ctx* ctx, bool keep_cat) {
         if (keep_cat)
             hipLaunchKernelGGL(k_upcat, dim3(gb(L[lev].V() * kLayers[l].cin)), dim3(256), 0, st, coarse, N->e[lev],
                                N->cat[lev], L[lev], Cu, Cs);
-        ok = ok && launch_conv3<false>(st, conv_args(N, l, N->e[lev], N->dd[lev], coarse));
+        ok = ok && launch_conv3<false>(st, conv_args(N, l, N->e[lev], N->dd[lev], lds, coarse));
         coarse = N->dd[lev];
     }
     hipLaunchKernelGGL(k_conv1_fwd, dim3(gb(L[0].V())), dim3(256), 0, st, N->dd[0], N->w32 + N->out_w, N->w32 + N->out_b,
@@ -899,7 +1041,7 @@ static int dgrad(fsim_ctx* ctx, int l, const __bf16* g16, float* dx) {
     const int cin = kLayers[l].cin, cout = kLayers[l].cout;
     hipLaunchKernelGGL(k_flip_weights, dim3(gb((int64_t)cout * 27 * cin)), dim3(256), 0, ctx->stream, N->w16 + N->woff[l],
                        N->wflip, cin, cout);
-    ConvArgs a{g16, N->wflip, nullptr, dx, d.D, d.H, d.W, cout, cin, 0, nullptr, 0};
+    ConvArgs a{g16, N->wflip, nullptr, dx, d.D, d.H, d.W, cout, cin, 0, nullptr, 0, 0};
     if (!launch_conv3<true>(ctx->stream, a))
         return set_err(ctx, FSIM_ERR_INVALID, "learned network: convolution shape without a kernel");
     FSIM_HIP(ctx, hipGetLastError());

@@ -135,12 +135,14 @@ def load(path: str) -> np.ndarray:
     return p.astype(np.float32)
 
 
-def forward_timing(n: int, reps: int = 10, seed: int = 1) -> dict:
+def forward_timing(n: int, reps: int = 10, seed: int = 1, conv: int = 1) -> dict:
     """Wall time of fsim_learned_forward on an n^3 grid (random right-hand side, every cell WATER,
-    initial parameters), host copies included; for profiling the network's kernels."""
+    initial parameters), host copies included; for profiling the network's kernels.
+    conv: FSIM_OPT_LEARNED_CONV (1 LDS-staged convolutions where they fit, 0 cached loads only)."""
     import time
     grid = scenes.grid(n)
     sim = capi.Simulation(grid, scenes.params())
+    sim.set_option(capi.OPT_LEARNED_CONV, conv)
     sim.learned_init(seed)
     b = np.random.default_rng(seed).standard_normal(grid.nc).astype(np.float32)
     lab = np.full(grid.nc, capi.WATER, np.uint8)
@@ -150,7 +152,7 @@ def forward_timing(n: int, reps: int = 10, seed: int = 1) -> dict:
         out = sim.learned_forward(b, lab)
     dt = (time.perf_counter() - t0) / reps
     sim.close()
-    return {"grid": [n] * 3, "reps": reps, "ms_per_forward_with_copies": dt * 1e3,
+    return {"grid": [n] * 3, "reps": reps, "conv": conv, "ms_per_forward_with_copies": dt * 1e3,
             "gflop_per_forward": flops_per_apply(grid.shape) / 1e9, "checksum": float(np.abs(out).sum())}
 
 
@@ -161,5 +163,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="Time the learned network's forward pass (fsim_learned_forward)")
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--conv", type=int, default=1, choices=(0, 1))
     a = ap.parse_args()
-    print(json.dumps(forward_timing(a.size, a.reps)), flush=True)
+    print(json.dumps(forward_timing(a.size, a.reps, conv=a.conv)), flush=True)

@@ -239,7 +239,12 @@ typedef enum fsim_option {
      * fsim_learned_train_batch): 0 (default) masked MSE of out against p/s
      * over WATER cells; 1 energy norm e^T A e / t^T A t of e = out - p/s, with
      * A the unscaled pressure matrix on WATER cells */
-    FSIM_OPT_LEARNED_LOSS = 3
+    FSIM_OPT_LEARNED_LOSS = 3,
+    /* learned preconditioner forward convolutions: 1 (default) the full- and
+     * half-resolution layers stage their input tile in LDS when the level's
+     * rows are a multiple of 32 cells; 0 every layer reads its operands
+     * through the caches (same arithmetic, bit-identical output) */
+    FSIM_OPT_LEARNED_CONV = 4
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
 

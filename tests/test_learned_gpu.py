@@ -53,6 +53,26 @@ def test_forward_matches_oracle(gpu):
     assert err < 2e-2
 
 
+def test_lds_convolutions_match_the_cached_kernel_and_the_oracle(gpu):
+    # 64^3: level-0 rows of 64 cells and level-1 rows of 32, so the 8->16, 48->16 and 96->32 layers all take
+    # the LDS-staged kernel; it sums in the cached kernel's order, so the outputs are bit-identical
+    say("case: learned forward 64^3, seed 5, LDS-staged vs cached convolutions")
+    grid, _, b, lab, _, _ = solved_sample(64, [0.0, 0.0, 0.0, 0.4, 0.7, 0.9], seed=2)
+    sim = capi.Simulation(grid, scenes.params())
+    sim.learned_init(5)
+    params = sim.learned_get_params()
+    out_lds = sim.learned_forward(b, lab)
+    sim.set_option(capi.OPT_LEARNED_CONV, 0)
+    out_cached = sim.learned_forward(b, lab)
+    sim.close()
+    assert np.array_equal(out_lds, out_cached)
+    x, s, fl = net.make_input(b, lab, grid.shape)
+    ref, _ = net.forward(net.unpack(params), x, emulate_bf16=True)
+    err = np.linalg.norm(out_lds - ref.reshape(-1)) / np.linalg.norm(ref)
+    say(f"  rel L2 error {err:.2e}")
+    assert err < 2e-2
+
+
 def test_gradient_matches_oracle(gpu):
     say("case: learned gradient 16^3, seed 5")
     grid, _, b, lab, p, _ = solved_sample(16, [0.0, 0.0, 0.0, 0.5, 0.6, 1.0])
