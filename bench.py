@@ -184,15 +184,25 @@ def _fpcg_iterations(sc, state, params, budget: int):
     return st.learned_iterations if st.learned_stop == 0 else None
 
 
-def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int, seeds: int = 3) -> dict:
+def _compare_scene(n: int):
+    """The n^3 scene of the solver comparison: the 128^3 APIC dam break with a static sphere, the 256^3
+    FLIP double dam break of the step-rate measurement, another n: the APIC dam break with a sphere."""
+    if n == 128:
+        return scenes.CONFIG_128_APIC_SPHERE(), "dam break 128^3 APIC with a static sphere"
+    if n == 256:
+        return scenes.CONFIG_256_FLIP_DOUBLE(), "double dam break 256^3 FLIP"
+    return scenes.dam_break(n, capi.APIC, seed=0, sphere=True), f"dam break {n}^3 APIC with a static sphere"
+
+
+def learned_solve_compare(sizes, train_n: int, train_scenes: int, epochs: int, seeds: int = 3) -> dict:
     """Pressure-solve ms to 1e-6 relative residual, MIC(0)-PCG vs the learned FPCG, on the same right-hand side.
 
     The network is trained here on the GPU with the energy-norm loss (generate_samples + train: real
     MIC(0)-PCG solves of seeded random dam breaks at train_n^3 and residual pairs) from `seeds` initial
     seeds; each is validated by its FPCG iteration count on a 64^3 double dam break (another size and
-    scene than the test), and the best is set on an n^3 context (model selection: a trained network
-    can stall FPCG, and the validation solve shows it).  The n^3 scene is the APIC dam break with a
-    static sphere; it is stepped 3 times with MIC(0)-PCG, its particle state is saved, and the next
+    scene than the tests), and the best is set on a context of each size n in `sizes` (model
+    selection: a trained network can stall FPCG, and the validation solve shows it).  Each n^3 scene
+    (_compare_scene) is stepped 3 times with MIC(0)-PCG, its particle state is saved, and the next
     step is solved from that same state with each preconditioner (the second of two runs is timed, so
     nothing is allocated in it).  `learned` is the production path: FPCG with the MIC(0)-PCG fallback
     after twice the last MIC(0) iteration count.  `fpcg_iterations` compares the FPCG iteration
@@ -226,11 +236,21 @@ def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int, 
     params, untrained = best["params"], cands[0]["init"]
     print(f"# learned: {len(B)} samples at {train_n}^3 in {t_gen:.1f}s, {seeds} x {epochs} epochs in {t_train:.1f}s, "
           f"selected seed {best['seed']}", file=sys.stderr, flush=True)
+    training = {"samples": len(B), "grid": [train_n] * 3, "epochs": epochs, "loss": "energy norm",
+                "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+                "validation": "FPCG iterations (budget 400) of the 4th step of a 64^3 FLIP double dam break",
+                "selected_seed": best["seed"],
+                "candidates": [{k: v for k, v in c.items() if k not in ("params", "init")} for c in cands]}
+    return {"training": training, "grids": [_compare_at(n, params, untrained) for n in sizes]}
 
-    sc = scenes.CONFIG_128_APIC_SPHERE() if n == 128 else scenes.dam_break(n, capi.APIC, seed=0, sphere=True)
+
+def _compare_at(n: int, params, untrained) -> dict:
+    """MIC(0)-PCG vs learned FPCG on the n^3 comparison scene (learned_solve_compare)."""
+    sc, what = _compare_scene(n)
     state = _state_after(sc, 3)
     sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
-    sim.set_obstacles(sc.obstacles)
+    if sc.obstacles:
+        sim.set_obstacles(sc.obstacles)
     res = {}
 
     def solve(name, precond, prm=None, profile=False):
@@ -273,15 +293,9 @@ def learned_solve_compare(n: int, train_n: int, train_scenes: int, epochs: int, 
                    "ms_per_apply": round(net_ms / net_calls, 4), "applies": net_calls,
                    "note": "ms_per_apply covers the network and its damped-Jacobi steps"}
     mic_ms, l_ms = res["mic0_pcg"]["solve_ms"], res["learned"]["solve_ms"]
-    return {"grid": [n] * 3, "workload": f"dam break {n}^3 APIC with a static sphere, 4th step, "
-                                          "same right-hand side for both solvers",
+    return {"grid": [n] * 3, "workload": f"{what}, 4th step, same right-hand side for both solvers",
             "mic0_pcg": res["mic0_pcg"], "learned": res["learned"], "fpcg_iterations": iters,
             "learned_over_mic0_time": round(l_ms / mic_ms, 3) if mic_ms > 0 else None,
-            "training": {"samples": len(B), "grid": [train_n] * 3, "epochs": epochs, "loss": "energy norm",
-                         "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
-                         "validation": "FPCG iterations (budget 400) of the 4th step of a 64^3 FLIP double dam break",
-                         "selected_seed": best["seed"],
-                         "candidates": [{k: v for k, v in c.items() if k not in ("params", "init")} for c in cands]},
             "network": network}
 
 
@@ -300,7 +314,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the pressure-solve comparison MIC(0)-PCG vs learned FPCG")
-    ap.add_argument("--learned-size", type=int, default=128)
+    ap.add_argument("--learned-sizes", default="128,256",
+                    help="comma-separated grid sizes of the MIC(0)-PCG vs learned FPCG comparison")
     ap.add_argument("--learned-train-size", type=int, default=32)
     ap.add_argument("--learned-scenes", type=int, default=60)
     ap.add_argument("--learned-epochs", type=int, default=15)
@@ -367,8 +382,8 @@ def main():
 
     solve_cmp = None
     if not args.no_learned:
-        solve_cmp = learned_solve_compare(args.learned_size, args.learned_train_size, args.learned_scenes,
-                                          args.learned_epochs)
+        sizes = [int(v) for v in args.learned_sizes.split(",") if v.strip()]
+        solve_cmp = learned_solve_compare(sizes, args.learned_train_size, args.learned_scenes, args.learned_epochs)
 
     cpu, accuracy = None, None
     if not args.no_cpu_baseline:
