@@ -107,7 +107,7 @@ struct P2GArgs {
 // in chunks (coalesced loads; each particle's (int)(p / dx) and half-cell
 // bits computed once), then every thread walks its own sub-range of the
 // chunk in order.
-constexpr int P2G_CAP_FLIP = 1024, P2G_CAP_APIC = 512;   // particles per staged chunk
+constexpr int P2G_CAP_FLIP = 512, P2G_CAP_APIC = 512;   // particles per staged chunk
 
 template <int APIC>
 __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
