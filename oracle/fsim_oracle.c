@@ -18,10 +18,13 @@
  * brute-force histogram for the binning, dense (L L^T)^-1 for MIC(0) apply,
  * a scipy direct solve for PCG, post-projection divergence, closed-form wall
  * bounces and clamp invariants for advection, count/radius/speed invariants
- * for spawning.  Functions without such a pin are listed as "parity unpinned"
- * in the design notes: the exact averaging of extrapolateVelocities and the
- * sphere/box back-stepping response (the reference cannot be built here:
- * glm is absent from the image).
+ * for spawning, and hand-worked cases (tests/handworked.py) for the
+ * averaging of extrapolateVelocities, the top-row gravity wrap, the density
+ * splat and the density term of the RHS, and the sphere (static) and box
+ * (moving, low-x face) collision responses.  Parity unpinned: whole-step
+ * trajectories, the moving-sphere response and the back-step cap for deep
+ * penetration (the reference cannot be built here: glm is absent from the
+ * image).
  */
 #include "fsim_oracle.h"
 
