@@ -20,11 +20,10 @@
  * bounces and clamp invariants for advection, count/radius/speed invariants
  * for spawning, and hand-worked cases (tests/handworked.py) for the
  * averaging of extrapolateVelocities, the top-row gravity wrap, the density
- * splat and the density term of the RHS, and the sphere (static) and box
- * (moving, low-x face) collision responses.  Parity unpinned: whole-step
- * trajectories, the moving-sphere response and the back-step cap for deep
- * penetration (the reference cannot be built here: glm is absent from the
- * image).
+ * splat and the density term of the RHS, and the sphere (static and moving,
+ * with the back-step cap) and box (moving, low-x face) collision responses.
+ * Parity unpinned: whole-step trajectories, which compose those stages (the
+ * reference cannot be built here: glm is absent from the image).
  */
 #include "fsim_oracle.h"
 

@@ -170,38 +170,72 @@ def _sphere_exit_time(d, v, R):
     return (dv + math.sqrt(dv * dv - a * c)) / a
 
 
-def sphere_bounce(make, oblique: bool):
-    """advectParticles' sphere response, simulator.cpp:178-213, for a static
-    sphere (r 0.1 at the centre of a 16^3 grid) and one particle at 10 m/s
-    along +x for dt = 0.01 that ends inside it (no wall within reach).
-    The reference steps back in 0.0002 s steps while inside (:196-197), takes
-    one more step (:200), reflects the velocity about the normal at the
-    stepped-back point with restitution 1 (:201-208) and moves on for the
-    time it stepped back (:209).  Expected from geometry: k = ceil(tau / 0.0002)
-    for the exit time tau along -v, back-step (k+1) 0.0002 s, velocity
-    v - 2 (v.n) n."""
+def sphere_bounce(make, oblique: bool, speed_x: float = 0.0):
+    """advectParticles' sphere response, simulator.cpp:178-213, for a sphere
+    (r 0.1, now at the centre of a 16^3 grid, moving at speed_x along x) and
+    one particle at 10 m/s along +x for dt = 0.01 that ends inside it (no wall
+    within reach).  The reference steps the particle back along -v and the
+    sphere along -speed in 0.0002 s steps while they overlap (:196-197),
+    takes one more step (:200), and reflects the velocity relative to the
+    sphere about the normal from the stepped-back sphere centre, restitution
+    1 (:201-208); the particle then moves on for the time it stepped back
+    (:209).  Expected from geometry: k = ceil(tau / 0.0002) for the exit time
+    tau of the relative motion, back-step (k+1) 0.0002 s, velocity
+    w - 2 (w.n) n + speed with w = v - speed."""
+    grid = scenes.grid(16)
+    c = np.array([0.5, 0.5, 0.5])
+    s = np.array([speed_x, 0.0, 0.0])
+    R = 0.1 + grid.particle_r
+    v = np.array([10.0, 0.0, 0.0])
+    w = v - s
+    dt = 0.01
+    p_end = c + (np.array([-0.108, 0.03, 0.0]) if oblique else np.array([-(R - 2.5 * BACK_STEP * w[0]), 0.0, 0.0]))
+    p0 = p_end - v * dt
+    b = make(grid)
+    b.set_particles(p0[None, :], v[None, :])
+    b.set_obstacles([capi.make_obstacle(capi.OBS_SPHERE, tuple(c), speed=tuple(s), r=0.1)])
+    b.run_stage(capi.STAGE_ADVECT, dt)
+    p, vel, _ = b.get_particles()
+    tau = _sphere_exit_time(p_end - c, w, R)
+    steps = tau / BACK_STEP
+    assert 0.1 < steps - math.floor(steps) < 0.9, "the case must not sit on a back-step boundary"
+    assert steps < 4.5, "the case must stay below the back-step cap"
+    back = (math.ceil(steps) + 1) * BACK_STEP
+    q = p_end - back * v
+    centre = c - s * back
+    nrm = (q - centre) / np.linalg.norm(q - centre)
+    v1 = w - 2.0 * float(w @ nrm) * nrm + s
+    p1 = q + back * v1
+    assert np.linalg.norm(p1 - c) > R, "the case must end with one bounce"
+    return [("position after the bounce", p[0], p1), ("velocity after the bounce", vel[0], v1),
+            ("particle count", len(p), 1)]
+
+
+def sphere_back_step_cap(make):
+    """The back-step loop of simulator.cpp:196 also stops once backTime is no
+    longer below t + 0.001f (a float literal, 0.0010000000475).  Five steps of
+    0.0002 s add up to 0.001 in double, which is still below it, so a first
+    collision (t = 0) backs off at most six steps, plus the extra step (:200):
+    7 x 0.0002 s.  Case: a static sphere (r 0.1) and a particle at 10 m/s
+    along +x ending 0.02 deep, which would need 10 steps to get out; after
+    backing off 0.014 it is still 0.006 inside, the normal is -x, its
+    velocity reverses, and the remaining 0.0014 s carry it 0.014 further out
+    (0.008 outside)."""
     grid = scenes.grid(16)
     c = np.array([0.5, 0.5, 0.5])
     R = 0.1 + grid.particle_r
     v = np.array([10.0, 0.0, 0.0])
     dt = 0.01
-    p_end = c + (np.array([-0.108, 0.03, 0.0]) if oblique else np.array([-(R - 2.5 * BACK_STEP * 10.0), 0.0, 0.0]))
-    p0 = p_end - v * dt
+    p_end = c - np.array([R - 0.02, 0.0, 0.0])
     b = make(grid)
-    b.set_particles(p0[None, :], v[None, :])
+    b.set_particles((p_end - v * dt)[None, :], v[None, :])
     b.set_obstacles([capi.make_obstacle(capi.OBS_SPHERE, tuple(c), r=0.1)])
     b.run_stage(capi.STAGE_ADVECT, dt)
     p, vel, _ = b.get_particles()
-    tau = _sphere_exit_time(p_end - c, v, R)
-    steps = tau / BACK_STEP
-    assert 0.1 < steps - math.floor(steps) < 0.9, "the case must not sit on a back-step boundary"
-    back = (math.ceil(steps) + 1) * BACK_STEP
-    q = p_end - back * v
-    nrm = (q - c) / np.linalg.norm(q - c)
-    v1 = v - 2.0 * float(v @ nrm) * nrm
-    p1 = q + back * v1
-    return [("position after the bounce", p[0], p1), ("velocity after the bounce", vel[0], v1),
-            ("particle count", len(p), 1)]
+    back = 7 * BACK_STEP
+    v1 = -v
+    p1 = p_end - back * v + back * v1
+    return [("position after the capped bounce", p[0], p1), ("velocity after the capped bounce", vel[0], v1)]
 
 
 def moving_box_bounce(make):

@@ -66,3 +66,14 @@ def test_sphere_bounce_geometry(gpu, oblique):
 def test_moving_box_bounce_hand_worked(gpu):
     say("case: moving box bounce")
     _check(handworked.moving_box_bounce(make), 1e-6, 2e-6)
+
+
+@pytest.mark.parametrize("oblique", [False, True])
+def test_moving_sphere_bounce_geometry(gpu, oblique):
+    say(f"case: moving sphere bounce, oblique {oblique}")
+    _check(handworked.sphere_bounce(make, oblique, speed_x=-2.0), 1e-6, 2e-6)
+
+
+def test_sphere_back_step_cap_hand_worked(gpu):
+    say("case: sphere bounce at the back-step cap")
+    _check(handworked.sphere_back_step_cap(make), 1e-6, 2e-6)

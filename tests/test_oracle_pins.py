@@ -439,3 +439,12 @@ def test_sphere_bounce_geometry(oblique):
 
 def test_moving_box_bounce_hand_worked():
     _check(handworked.moving_box_bounce(mk), 1e-12)
+
+
+@pytest.mark.parametrize("oblique", [False, True])
+def test_moving_sphere_bounce_geometry(oblique):
+    _check(handworked.sphere_bounce(mk, oblique, speed_x=-2.0), 1e-12)
+
+
+def test_sphere_back_step_cap_hand_worked():
+    _check(handworked.sphere_back_step_cap(mk), 1e-12)
