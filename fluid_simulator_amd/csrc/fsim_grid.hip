@@ -177,6 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
                 }
                 __syncthreads();
                 const int e = min(j1, base + cnt);
+#pragma unroll 4
                 for (int j = max(j0, base); j < e; j++) {
                     const int k = j - base;
                     const float p[3] = {s_pos[0][k], s_pos[1][k], s_pos[2][k]};
