@@ -17,6 +17,8 @@
 //                 counting sort by (cell, particle id), bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -493,13 +495,52 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint32_t* __restri
     perm[start[key[i]] + rank[i]] = (int32_t)i;
 }
 
-// Canonical order inside a cell: ascending particle id (insertion sort; a
-// cell holds ~8 particles in a resting fluid).
+// Canonical order inside a cell: ascending particle id.  A cell of up to
+// kSegRegs particles (a resting fluid holds ~8) is read once into registers,
+// sorted by a bitonic network (compile-time indices, padding keys INT_MAX last)
+// and written back only if its order changed; larger cells use an insertion
+// sort in place.  Particle ids are unique, so both give the same order.
+constexpr int kSegRegs = 16;
 __global__ void __launch_bounds__(kBlock) k_bin_segsort(const int32_t* __restrict__ start, int32_t* __restrict__ perm,
                                                         const int32_t* __restrict__ pid, int64_t nc) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     int b = start[c], e = start[c + 1];
+    if (e - b < 2) return;
+    if (e - b <= kSegRegs) {
+        const int n = e - b;
+        int32_t key[kSegRegs], val[kSegRegs];
+#pragma unroll
+        for (int i = 0; i < kSegRegs; i++) {
+            val[i] = i < n ? perm[b + i] : 0;
+            key[i] = i < n ? pid[val[i]] : INT_MAX;
+        }
+        bool sorted = true;
+#pragma unroll
+        for (int i = 0; i + 1 < kSegRegs; i++) sorted = sorted && key[i] <= key[i + 1];
+        if (sorted) return;
+#pragma unroll
+        for (int k = 2; k <= kSegRegs; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < kSegRegs; i++) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const bool up = (i & k) == 0;
+                        const bool swap = up ? key[i] > key[l] : key[i] < key[l];
+                        const int32_t ki = key[i], vi = val[i];
+                        key[i] = swap ? key[l] : ki;
+                        val[i] = swap ? val[l] : vi;
+                        key[l] = swap ? ki : key[l];
+                        val[l] = swap ? vi : val[l];
+                    }
+                }
+#pragma unroll
+        for (int i = 0; i < kSegRegs; i++)
+            if (i < n) perm[b + i] = val[i];
+        return;
+    }
     for (int i = b + 1; i < e; i++) {
         int32_t v = perm[i];
         int32_t kv = pid[v];
