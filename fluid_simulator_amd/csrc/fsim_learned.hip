@@ -330,7 +330,7 @@ static bool launch_conv3(hipStream_t st, const ConvArgs& a) {
         // the full-resolution layers through LDS when their rows are a multiple of 32 voxels
         if (a.cin == 8 && a.cu == 0 && a.cout == 16 && conv3_lds_launch<8, 0, 16, 4, 16>(st, a)) return true;
         if (a.cin == 48 && a.cu == 32 && a.cout == 16 && conv3_lds_launch<48, 32, 16, 4, 32>(st, a)) return true;
-        FSIM_CONV3(8, 0, 16, 1, 4) FSIM_CONV3(16, 0, 32, 2, 4) FSIM_CONV3(32, 0, 64, 4, 2)
+        FSIM_CONV3(8, 0, 16, 1, 4) FSIM_CONV3(16, 0, 32, 2, 4) FSIM_CONV3(32, 0, 64, 2, 2)
         FSIM_CONV3(64, 0, 128, 2, 1) FSIM_CONV3(192, 128, 64, 4, 2) FSIM_CONV3(96, 64, 32, 2, 4)
         FSIM_CONV3(48, 32, 16, 1, 4)
     } else {
