@@ -361,3 +361,68 @@ def test_whole_step_wavefront_equals_planes(gpu):
     assert res[0][1] == res[1][1]
     for a, b in zip(res[0][0], res[1][0]):
         assert np.array_equal(a, b)
+
+
+def test_config_256_flip_double_dam_break_properties(gpu):
+    """The 256^3 FLIP double dam break (62.8M particles), two whole steps: the
+    oracle needs ~25 s per step here, so this size is held to properties that
+    do not depend on it.  Particles conserved and inside the clamp box; the
+    binning is a counting sort of the particles' cells (slots in cell order,
+    counts = a histogram, offsets = their exclusive scan); WATER exactly where
+    a cell holds particles; the solve reaches 1e-6 and the projected faces'
+    divergence equals that residual; velocities finite."""
+    import bench
+
+    sc = scenes.CONFIG_256_FLIP_DOUBLE()
+    grid = sc.grid
+    n = grid.nx
+    say(f"case: 256^3 FLIP double dam break, 2 steps, properties")
+    pos0 = sc.particles()
+    npart = len(pos0)
+    sim = capi.Simulation(grid, scenes.params(transfer=sc.transfer))
+    sim.set_particles(pos0)
+    del pos0
+    lo = grid.cell_d + 1.01 * grid.particle_r
+    for step in range(2):
+        st = sim.step(DT)
+        say(f"  step {step}: pcg {st.pcg_iterations}, residual {st.residual_rel_l2:.2e}")
+        assert st.num_particles == npart and st.early_exit == 0
+        assert st.residual_rel_l2 <= 1e-6 and 50 < st.pcg_iterations < 400
+        pos, vel, _ = sim.get_particles()
+        assert len(pos) == npart
+        assert pos.min() >= np.float32(lo) and pos.max() <= np.float32(1.0 - lo)
+        assert np.isfinite(vel).all()
+        cell = np.floor(pos.astype(np.float64) * n).astype(np.int64)
+        key = (cell[:, 0] * n + cell[:, 1]) * n + cell[:, 2]   # by particle id
+        del pos, vel, cell
+        ids = sim.slot_ids()   # particle id of each device slot
+        assert np.array_equal(np.sort(ids), np.arange(npart, dtype=ids.dtype))
+        key_slot = key[ids]
+        dk = np.diff(key_slot)
+        assert np.all(dk >= 0)                     # slots in cell order
+        assert np.all(np.diff(ids)[dk == 0] > 0)   # ascending id inside a cell
+        del ids, key_slot, dk
+        count = sim.get_grid(capi.FIELD_CELL_COUNT).astype(np.int64)
+        start = sim.get_grid(capi.FIELD_CELL_START).astype(np.int64)
+        assert np.array_equal(count, np.bincount(key, minlength=grid.nc))
+        assert start[0] == 0 and np.array_equal(np.diff(start), count) and start[-1] == npart
+        del key, start
+        lab = sim.get_grid(capi.FIELD_LABEL)
+        assert np.array_equal(lab == capi.WATER, count > 0)
+        # the FP64 residual b - A p and the divergence of the projected faces agree up to the FP32
+        # rounding of those faces: half an ulp per face, two roundings in u2 + s (p_i - p_j), six
+        # faces per cell, summed in L2 over the WATER cells
+        water = (lab == capi.WATER).reshape(grid.shape)
+        del lab, count
+        halfulp = np.zeros(grid.shape)
+        for ax, f in enumerate((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2)):
+            h = np.spacing(np.abs(sim.get_grid(f))).reshape(grid.shape).astype(np.float64)
+            halfulp += h + np.roll(h, 1, axis=ax)   # 2 x half an ulp, +a and -a face
+            del h
+        bound = float(np.sqrt(np.sum(np.where(water, halfulp, 0.0) ** 2))) / grid.cell_d / st.b_norm_l2
+        del halfulp, water
+        div = bench.divergence_l2(sim, grid) / st.b_norm_l2
+        say(f"  divergence / |b| {div:.3e}, residual {st.residual_rel_l2:.3e}, rounding bound {bound:.2e}")
+        assert abs(div - st.residual_rel_l2) <= bound
+        assert div <= 1e-6 + bound
+    sim.close()
