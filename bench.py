@@ -324,8 +324,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU decomposition is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the step has no multi-GPU implementation)", file=sys.stderr)
         sys.exit(2)
 
     scene = scenes.double_dam_break(args.size, capi.FLIP, seed=1)
