@@ -693,16 +693,21 @@ __global__ void __launch_bounds__(kBlock) k_true_residual(SolveArgs S, const uin
             const int zc = z0 + lane + 64 * c;
             const int64_t i = rowbase + zc;
             const RNeighbours n = rneighbours(S, i, p, y, zc);
-            const int cnt = (info[c] >> NB_NONSOLID_SHIFT) & 7;
+            // all 7 loads first (a WATER cell is interior: every neighbour index is in range), then the
+            // reference's terms in order, each kept by its flag
+            const double xi = x[i], xpx = x[n.px], xpy = x[n.py], xpz = x[n.pz], xmx = x[n.mx], xmy = x[n.my],
+                         xmz = x[n.mz];
+            const uint16_t f = info[c];
+            const int cnt = (f >> NB_NONSOLID_SHIFT) & 7;
             double diag = 0;
             for (int j = 0; j < cnt; j++) diag += S.scale;
-            double val = diag * x[i];
-            if (info[c] & NB_PX) val += -S.scale * x[n.px];
-            if (info[c] & NB_PY) val += -S.scale * x[n.py];
-            if (info[c] & NB_PZ) val += -S.scale * x[n.pz];
-            if (info[c] & NB_MX) val += -S.scale * x[n.mx];
-            if (info[c] & NB_MY) val += -S.scale * x[n.my];
-            if (info[c] & NB_MZ) val += -S.scale * x[n.mz];
+            double val = diag * xi;
+            val = (f & NB_PX) ? val + -S.scale * xpx : val;
+            val = (f & NB_PY) ? val + -S.scale * xpy : val;
+            val = (f & NB_PZ) ? val + -S.scale * xpz : val;
+            val = (f & NB_MX) ? val + -S.scale * xmx : val;
+            val = (f & NB_MY) ? val + -S.scale * xmy : val;
+            val = (f & NB_MZ) ? val + -S.scale * xmz : val;
             const double t = (double)b[i] - val;
             ss += t * t;
             mx = fmax(mx, fabs(t));
