@@ -178,6 +178,7 @@ def _fpcg_iterations(sc, state, params, budget: int):
     if sc.obstacles:
         sim.set_obstacles(sc.obstacles)
     sim.learned_set_params(params)
+    sim.set_option(capi.OPT_LEARNED_MIC_COUNT, max(budget // 2, 1))   # FPCG budget 2 x this count
     sim.set_particles(*state)
     st = sim.step(scenes.DT)
     sim.close()

@@ -62,12 +62,20 @@ class Obstacle(C.Structure):
                 ("last_spawn_fraction", C.c_double)]
 
 
+class GfxParticle(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("v", C.c_float), ("density", C.c_float)]
+
+
+GFX_DTYPE = np.dtype([("pos", np.float32, 3), ("v", np.float32), ("density", np.float32)])
+
+
 class StepStats(C.Structure):
     _fields_ = [("timer_ms", C.c_double * 8), ("step_ms", C.c_double), ("pcg_iterations", C.c_int),
                 ("early_exit", C.c_int), ("fallback", C.c_int), ("reliable_updates", C.c_int),
                 ("residual_abs_inf", C.c_double), ("residual_rel_l2", C.c_double), ("b_norm_l2", C.c_double),
                 ("solve_ms", C.c_double), ("num_particles", C.c_int64), ("num_fluid_cells", C.c_int64),
-                ("learned_iterations", C.c_int), ("learned_stop", C.c_int)]
+                ("learned_iterations", C.c_int), ("learned_stop", C.c_int), ("solve_nan", C.c_int),
+                ("learned_budget", C.c_int)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "timer_ms"}
@@ -107,13 +115,15 @@ PROTOTYPES = {
     "fsim_learned_train_batch": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_float, C.POINTER(C.c_double)]),
     "fsim_scene_dam_break": (C.c_int, [C.POINTER(GridDesc), C.c_int, _P, C.c_int, C.POINTER(Obstacle), C.c_uint64,
                                        C.c_int64, _P, C.POINTER(C.c_int64)]),
+    "fsim_gfx_snapshot": (C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_int64)]),
     "fsim_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
     "fsim_num_kernels": (C.c_int, []),
     "fsim_kernel_name": (C.c_char_p, [C.c_int]),
     "fsim_get_kernel_time": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "fsim_reset_kernel_times": (C.c_int, [_P]),
 }
-OPT_SWEEP_MODE, OPT_PROFILE, OPT_LEARNED_SMOOTHING, OPT_LEARNED_LOSS, OPT_LEARNED_CONV = 0, 1, 2, 3, 4
+(OPT_SWEEP_MODE, OPT_PROFILE, OPT_LEARNED_SMOOTHING, OPT_LEARNED_LOSS, OPT_LEARNED_CONV, OPT_LEARNED_REFRESH,
+ OPT_LEARNED_MIC_COUNT) = range(7)
 
 _lib = None
 
@@ -308,6 +318,14 @@ class Simulation:
     def set_grid(self, field: int, values: np.ndarray):
         a = np.ascontiguousarray(values, dtype=FIELD_DTYPES[field]).reshape(-1)
         self._check(self.lib.fsim_set_grid(self._ctx, field, _ptr(a), a.nbytes))
+
+    def gfx_snapshot(self) -> np.ndarray:
+        """SimulationManager's ParticleGfxData of every particle, in particle order (GFX_DTYPE records)."""
+        n = self.num_particles()
+        out = np.zeros(max(n, 1), GFX_DTYPE)
+        got = C.c_int64()
+        self._check(self.lib.fsim_gfx_snapshot(self._ctx, n, _ptr(out), C.byref(got)))
+        return out[:n]
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.fsim_set_option(self._ctx, option, value))

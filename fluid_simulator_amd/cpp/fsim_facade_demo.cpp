@@ -64,9 +64,10 @@ int main(int argc, char** argv) {
         }
         const auto d = sim.getStepDuration();
         std::printf("{\"particles\": %zu, \"water_cells\": %d, \"pcg_iterations\": %d, \"ymin\": %.6f, \"ymax\": %.6f, "
-                    "\"Incompressibility_us\": %lld, \"P2GTransfer_us\": %lld, \"step_duration_keys\": %zu}\n",
+                    "\"Incompressibility_us\": %lld, \"P2GTransfer_us\": %lld, \"it_count\": %lld, "
+                    "\"step_duration_keys\": %zu}\n",
                     parts->getParticleNum(), water, sim.lastStats().pcg_iterations, ymin, ymax,
-                    d.at("Incompressibility"), d.at("P2GTransfer"), d.size());
+                    d.at("Incompressibility"), d.at("P2GTransfer"), d.at("Incompressibility it count"), d.size());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 3;

@@ -289,6 +289,8 @@ void Simulator::simulate(double dt) {
         const long long us = (long long)(stats_.timer_ms[k] * 1000.0);
         d = (long long)(d * kSlidingAvg + us * (1.0 - kSlidingAvg));
     }
+    // the solver's iteration count, set raw after every step (simulator.cpp:91)
+    stepDuration["Incompressibility it count"] = stats_.pcg_iterations;
     hashedParticles->host_stale_ = true;
     for (bool& s : macGrid->stale_) s = true;
 }

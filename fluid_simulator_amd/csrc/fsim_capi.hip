@@ -3,7 +3,7 @@
 // fsim_step is Simulator::simulate(double dt) (simulator.cpp:51-100) as a
 // sequence of HIP kernels on the context's stream:
 //   spawn -> advect + sink removal + push-out + stop (fused per particle)
-//   -> bin by cell -> P2G (+ fluid marking) [+ density] -> obstacles
+//   -> bin by cell -> P2G (+ fluid marking and particle density) -> obstacles
 //   -> borders -> v2 + gravity -> pressure solve -> extrapolation -> G2P.
 // Stage GPU times are HIP-event intervals grouped by the reference's
 // stepDuration keys (simulator.cpp:31-41).
@@ -102,6 +102,7 @@ static int dalloc(fsim_ctx* ctx, T** p, int64_t n) {
 
 static void free_all(fsim_ctx* ctx) {
     learned_free(ctx);
+    gfx_free(ctx);
     void* ptrs[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed, ctx->u, ctx->v, ctx->w,
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
                     ctx->scan_tmp, ctx->nbinfo, ctx->chunk_flags, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
@@ -244,12 +245,7 @@ int fsim_set_particle_radius(fsim_ctx* ctx, double r) {
 
 int fsim_set_particles(fsim_ctx* ctx, int64_t n, const float* pos, const float* vel, const float* c) {
     if (!ctx || n < 0 || (n > 0 && !pos)) return FSIM_ERR_INVALID;
-    FSIM_TRY(particles_reserve(ctx, std::max<int64_t>(n, 1)));
-    if (ctx->scan_tmp_cap < (ctx->P.cap + 2) / 4096 + 2) {
-        (void)hipFree(ctx->scan_tmp);
-        ctx->scan_tmp_cap = (ctx->P.cap + 2) / 4096 + 2;
-        FSIM_TRY(dalloc(ctx, &ctx->scan_tmp, ctx->scan_tmp_cap));
-    }
+    FSIM_TRY(particles_reserve(ctx, std::max<int64_t>(n, 1)));   // also grows the scan buffer
     std::vector<float> plane((size_t)n);
     auto put = [&](float* dst) -> int {
         FSIM_HIP(ctx, hipMemcpy(dst, plane.data(), sizeof(float) * n, hipMemcpyHostToDevice));
@@ -386,8 +382,7 @@ int fsim_step(fsim_ctx* ctx, double dt, fsim_step_stats* stats) {
     FSIM_TRY(stage_move(ctx, dt, 1 | 2 | 4));
     FSIM_HIP(ctx, hipEventRecord(ev[1], ctx->stream));   // SimulateParticles (+ fused push-out)
     FSIM_TRY(stage_bin(ctx));
-    FSIM_TRY(stage_p2g_fused(ctx));
-    if (ctx->prm.pressure_enabled) FSIM_TRY(stage_mark(ctx));
+    FSIM_TRY(stage_p2g_fused(ctx));   // reset + P2G + fluid marking + particle density
     FSIM_HIP(ctx, hipEventRecord(ev[2], ctx->stream));   // P2GTransfer
     FSIM_TRY(stage_obstacles(ctx));
     FSIM_TRY(stage_borders(ctx));
@@ -482,6 +477,14 @@ int fsim_set_grid(fsim_ctx* ctx, int field, const void* src, size_t bytes) {
     return FSIM_OK;
 }
 
+int fsim_gfx_snapshot(fsim_ctx* ctx, int64_t capacity, fsim_gfx_particle* out, int64_t* n_out) {
+    if (!ctx) return FSIM_ERR_INVALID;
+    if (n_out) *n_out = ctx->P.n;
+    if (!out) return FSIM_OK;
+    if (capacity < ctx->P.n) return set_err(ctx, FSIM_ERR_INVALID, "capacity smaller than the particle count");
+    return gfx_snapshot(ctx, out);
+}
+
 int fsim_num_kernels(void) { return KID_COUNT; }
 
 const char* fsim_kernel_name(int kid) { return (kid >= 0 && kid < KID_COUNT) ? kKernelNames[kid] : "unknown"; }
@@ -526,6 +529,15 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
     case FSIM_OPT_LEARNED_CONV:
         if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "learned convolution kernel is 0 or 1");
         ctx->learned_conv_lds = value;
+        return FSIM_OK;
+    case FSIM_OPT_LEARNED_REFRESH:
+        if (value < 0) return set_err(ctx, FSIM_ERR_INVALID, "learned refresh period must be >= 0");
+        ctx->learned_refresh = value;
+        return FSIM_OK;
+    case FSIM_OPT_LEARNED_MIC_COUNT:
+        if (value <= 0) return set_err(ctx, FSIM_ERR_INVALID, "the MIC(0) iteration count must be positive");
+        ctx->last_mic_iterations = value;
+        ctx->learned_since_mic = 0;
         return FSIM_OK;
     default: return set_err(ctx, FSIM_ERR_INVALID, "unknown option");
     }

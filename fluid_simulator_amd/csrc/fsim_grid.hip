@@ -100,6 +100,10 @@ struct P2GArgs {
 // three faces and sums, in a fixed order (neighbour cells dx, dy, dz, then
 // particles in sorted order), every particle whose 2x2x2 face stencil of an
 // axis holds the face (base from face_base, as in the reference's scatter).
+// The same walk sums the particle density of the cell (avgPNum, the
+// cell-centre stencil of markFluidCellsAndCalculateParticleDensities), which
+// the reference computes on every step whether or not the pressure term of
+// the right-hand side reads it.
 //
 // A block owns 256 consecutive cells.  For one (dx, dy) the three dz cells of
 // every thread are consecutive, so the particles the block needs are one
@@ -136,6 +140,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
     }
     const int cc[3] = {x, y, z};
     float swv[3] = {0.f, 0.f, 0.f}, sw[3] = {0.f, 0.f, 0.f};
+    float dens = 0.f;   // avgPNum: tent weights of the cell centre (simulator.cpp:362-366)
     const int64_t plane = (int64_t)G.ny * G.nz;
     for (int dx = -1; dx <= 1; dx++) {
         for (int dy = -1; dy <= 1; dy++) {
@@ -193,6 +198,9 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
                         tal[d] = 1.0f - fabsf((p[d] - fal[d]) * G.invf);
                         tce[d] = 1.0f - fabsf((p[d] - fce[d]) * G.invf);
                     }
+                    // getCellsAround (macGrid.cpp:188-202): base (int)(p/dx - 0.5) on every axis
+                    if ((unsigned)uc[0] <= 1u && (unsigned)uc[1] <= 1u && (unsigned)uc[2] <= 1u)
+                        dens += tce[0] * tce[1] * tce[2];
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
                         bool in = true;
@@ -227,7 +235,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
         A.uvw2[a][i] = 0.0f;
     }
     A.label[i] = (A.fuse_mark && A.count[i] > 0) ? (uint8_t)FSIM_WATER : (uint8_t)FSIM_AIR;
-    A.density[i] = 0.0f;
+    A.density[i] = dens;
 }
 
 static int p2g(fsim_ctx* ctx, int fuse_mark) {
@@ -263,55 +271,19 @@ int stage_p2g_fused(fsim_ctx* ctx) { return p2g(ctx, 1); }
 
 // ---- markFluidCellsAndCalculateParticleDensities (simulator.cpp:354-368) --------
 // WATER where the cell holds a particle (the bin key is the same
-// ivec3(pos*cellDInv)), density = sum of cell-centre tent weights gathered from
-// the 3x3x3 neighbourhood with the getCellsAround stencil (macGrid.cpp:188-202).
-__global__ void __launch_bounds__(kBlock) k_mark(P2GArgs A, int with_density) {
-    const GridArgs& G = A.G;
+// ivec3(pos*cellDInv)).  The density half of the reference function is summed
+// by the P2G gather above, which walks the same particles in the same order.
+__global__ void __launch_bounds__(kBlock) k_mark(const int32_t* __restrict__ count, uint8_t* __restrict__ label,
+                                                 int64_t nc) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G.nc) return;
-    if (A.count[i] > 0) A.label[i] = FSIM_WATER;
-    if (!with_density) return;
-    const int z = (int)(i % G.nz);
-    const int y = (int)((i / G.nz) % G.ny);
-    const int x = (int)(i / ((int64_t)G.nz * G.ny));
-    const float cp[3] = {(float)((x + 0.5) * G.cd), (float)((y + 0.5) * G.cd), (float)((z + 0.5) * G.cd)};
-    float sum = 0.f;
-    for (int dx = -1; dx <= 1; dx++) {
-        int cx = x + dx;
-        if (cx < 0 || cx >= G.nx) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-            int cy = y + dy;
-            if (cy < 0 || cy >= G.ny) continue;
-            for (int dz = -1; dz <= 1; dz++) {
-                int cz = z + dz;
-                if (cz < 0 || cz >= G.nz) continue;
-                int64_t n = cidx(G, cx, cy, cz);
-                int b = A.start[n], e = A.start[n + 1];
-                for (int j = b; j < e; j++) {
-                    const float px = A.pos[0][j], py = A.pos[1][j], pz = A.pos[2][j];
-                    unsigned ox = (unsigned)(x - (int)((double)px * G.cdinv - 0.5));
-                    unsigned oy = (unsigned)(y - (int)((double)py * G.cdinv - 0.5));
-                    unsigned oz = (unsigned)(z - (int)((double)pz * G.cdinv - 0.5));
-                    if (ox > 1u || oy > 1u || oz > 1u) continue;
-                    sum += tent3(cp, px, py, pz, G.invf);
-                }
-            }
-        }
-    }
-    A.density[i] += sum;
+    if (i >= nc) return;
+    if (count[i] > 0) label[i] = FSIM_WATER;
 }
 
 int stage_mark(fsim_ctx* ctx) {
     FSIM_PROF(ctx, KID_MARK);
-    P2GArgs A{};
-    A.G = grid_args(ctx);
-    for (int a = 0; a < 3; a++) A.pos[a] = ctx->P.pos[a];
-    A.start = ctx->cell_start;
-    A.count = ctx->cell_count;
-    A.label = ctx->label;
-    A.density = ctx->density;
-    hipLaunchKernelGGL(k_mark, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A,
-                       ctx->prm.pressure_enabled ? 1 : 0);
+    hipLaunchKernelGGL(k_mark, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, ctx->cell_count, ctx->label,
+                       ctx->g.nc);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }

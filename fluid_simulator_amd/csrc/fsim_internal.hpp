@@ -194,6 +194,8 @@ struct fsim_ctx {
     hipEvent_t dir_ev = nullptr;      // PCG: the direction step (s, alpha) is done
     hipEvent_t x_ev = nullptr;        // PCG: the x update on xstream is done
     int last_mic_iterations = 0;      // MIC(0)-PCG iterations of the last solve that used it
+    int learned_since_mic = 0;        // learned solves since that MIC(0) solve
+    int learned_refresh = 50;         // learned solves between MIC(0) solves that renew the count (0: never)
     fsim::LearnedNet* learned = nullptr;   // learned preconditioner (network parameters and buffers)
     int partials_cap = 0;
     fsim::SolveScalars* scal = nullptr;   // device
@@ -211,6 +213,10 @@ struct fsim_ctx {
     int learned_smoothing = 4;                   // damped-Jacobi steps after the network
     int learned_loss = 0;                        // training loss: 0 masked MSE, 1 energy norm
     int learned_conv_lds = 1;                    // 1: LDS-staged convolutions where they fit
+
+    // renderer snapshot staging (grown on demand)
+    fsim_gfx_particle* gfx_staging = nullptr;
+    int64_t gfx_cap = 0;
 
     // obstacles
     std::vector<fsim_obstacle> obs;
@@ -294,6 +300,10 @@ int stage_solve(fsim_ctx* ctx, double dt);
 int wave_launch(fsim_ctx* ctx, int kind, const float* in, float* out, double scale);
 int wave_init(fsim_ctx* ctx);
 int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out);
+
+// ---- renderer snapshot (fsim_gfx.hip) --------------------------------------
+int gfx_snapshot(fsim_ctx* ctx, fsim_gfx_particle* host_out);
+void gfx_free(fsim_ctx* ctx);
 
 // ---- learned preconditioner (fsim_learned.hip) ------------------------------
 int64_t learned_num_params();

@@ -400,49 +400,57 @@ static int alloc_planes(fsim_ctx* ctx, Particles& P, int64_t cap) {
 int particles_reserve(fsim_ctx* ctx, int64_t n) {
     if (n <= ctx->P.cap) return FSIM_OK;
     if (n > INT32_MAX) return set_err(ctx, FSIM_ERR_INVALID, "particle count exceeds int32 range");
-    int64_t cap = std::max<int64_t>(n + n / 8, 1024);
-    Particles NP, NA;
-    FSIM_TRY(alloc_planes(ctx, NP, cap));
-    FSIM_TRY(alloc_planes(ctx, NA, cap));
-    if (ctx->P.n > 0) {
-        for (int k = 0; k < 16; k++)
-            FSIM_HIP(ctx, hipMemcpyAsync(NP.base + (size_t)cap * k, ctx->P.base + (size_t)ctx->P.cap * k,
-                                         sizeof(float) * ctx->P.n, hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    uint32_t* key = nullptr;
-    int32_t *rank = nullptr, *perm = nullptr;
-    uint8_t* removed = nullptr;
-    if (hipMalloc(&key, sizeof(uint32_t) * (cap + 1)) != hipSuccess || hipMalloc(&rank, sizeof(int32_t) * cap) != hipSuccess ||
-        hipMalloc(&perm, sizeof(int32_t) * cap) != hipSuccess || hipMalloc(&removed, cap) != hipSuccess)
-        return set_err(ctx, FSIM_ERR_OOM, "particle work buffer allocation failed");
-    FSIM_HIP(ctx, hipMemsetAsync(removed, 0, cap, ctx->stream));
-    if (ctx->P.n > 0)
-        FSIM_HIP(ctx, hipMemcpyAsync(removed, ctx->removed, ctx->P.n, hipMemcpyDeviceToDevice, ctx->stream));
-    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(ctx->P.base);
-    (void)hipFree(ctx->Palt.base);
-    (void)hipFree(ctx->key);
-    (void)hipFree(ctx->rank);
-    (void)hipFree(ctx->perm);
-    (void)hipFree(ctx->removed);
-    NP.n = ctx->P.n;
-    ctx->P = NP;
-    ctx->Palt = NA;
-    ctx->key = key;
-    ctx->rank = rank;
-    ctx->perm = perm;
-    ctx->removed = removed;
-    ctx->pwork_cap = cap;
+    const int64_t cap = std::max<int64_t>(n + n / 8, 1024);
     // the id-renumbering scan runs over up to cap particles
     const int64_t need_tmp = (cap + 2) / 4096 + 2;
-    if (ctx->scan_tmp_cap < need_tmp) {
-        int32_t* tmp = nullptr;
-        if (hipMalloc(&tmp, sizeof(int32_t) * need_tmp) != hipSuccess)
-            return set_err(ctx, FSIM_ERR_OOM, "scan buffer allocation failed");
+    // every new buffer is owned here until all of them exist: a failure frees
+    // the ones already made and leaves the context as it was
+    struct Fresh {
+        Particles NP, NA;
+        uint32_t* key = nullptr;
+        int32_t *rank = nullptr, *perm = nullptr, *tmp = nullptr;
+        uint8_t* removed = nullptr;
+        bool keep = false;
+        ~Fresh() {
+            if (keep) return;
+            void* ptrs[] = {NP.base, NA.base, key, rank, perm, tmp, removed};
+            for (void* p : ptrs)
+                if (p) (void)hipFree(p);
+        }
+    } f;
+    FSIM_TRY(alloc_planes(ctx, f.NP, cap));
+    FSIM_TRY(alloc_planes(ctx, f.NA, cap));
+    if (hipMalloc(&f.key, sizeof(uint32_t) * (cap + 1)) != hipSuccess ||
+        hipMalloc(&f.rank, sizeof(int32_t) * cap) != hipSuccess ||
+        hipMalloc(&f.perm, sizeof(int32_t) * cap) != hipSuccess || hipMalloc(&f.removed, cap) != hipSuccess ||
+        (ctx->scan_tmp_cap < need_tmp && hipMalloc(&f.tmp, sizeof(int32_t) * need_tmp) != hipSuccess))
+        return set_err(ctx, FSIM_ERR_OOM, "particle buffer allocation failed");
+    if (ctx->P.n > 0) {
+        for (int k = 0; k < 16; k++)
+            FSIM_HIP(ctx, hipMemcpyAsync(f.NP.base + (size_t)cap * k, ctx->P.base + (size_t)ctx->P.cap * k,
+                                         sizeof(float) * ctx->P.n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    FSIM_HIP(ctx, hipMemsetAsync(f.removed, 0, cap, ctx->stream));
+    if (ctx->P.n > 0)
+        FSIM_HIP(ctx, hipMemcpyAsync(f.removed, ctx->removed, ctx->P.n, hipMemcpyDeviceToDevice, ctx->stream));
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    void* old[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    f.NP.n = ctx->P.n;
+    ctx->P = f.NP;
+    ctx->Palt = f.NA;
+    ctx->key = f.key;
+    ctx->rank = f.rank;
+    ctx->perm = f.perm;
+    ctx->removed = f.removed;
+    ctx->pwork_cap = cap;
+    if (f.tmp) {
         (void)hipFree(ctx->scan_tmp);
-        ctx->scan_tmp = tmp;
+        ctx->scan_tmp = f.tmp;
         ctx->scan_tmp_cap = need_tmp;
     }
+    f.keep = true;
     return FSIM_OK;
 }
 

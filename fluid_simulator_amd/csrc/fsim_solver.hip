@@ -623,12 +623,14 @@ __global__ void __launch_bounds__(kBlock) k_dot(int64_t nc, const uint32_t* __re
 
 // x += alpha s (FP64) for WATER cells.  One cell per thread: it runs on a
 // second stream beside the r update, and a lighter HBM load there costs the
-// r update less than a vectorised kernel does (measured at 256^3).
+// r update less than a vectorised kernel does (measured at 256^3).  A NaN
+// alpha leaves x as it is: the reference breaks before multAdd
+// (bridsonSolverGrid.cpp:270-272) and applies the last pressure.
 __global__ void __launch_bounds__(kBlock) k_update_x(int64_t n, const uint16_t* __restrict__ nbinfo,
                                                      const float* __restrict__ s, double* __restrict__ x,
                                                      const SolveScalars* __restrict__ sc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !(nbinfo[i] & NB_FLUID)) return;
+    if (i >= n || !(nbinfo[i] & NB_FLUID) || (sc->nan_flag & 1)) return;
     x[i] += (double)s[i] * sc->alpha;
 }
 
@@ -1072,6 +1074,8 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     st.pcg_iterations = 0;
     st.learned_iterations = 0;
     st.learned_stop = 0;
+    st.solve_nan = 0;
+    st.learned_budget = 0;
     const bool learned = ctx->prm.precond == FSIM_PRECOND_LEARNED;
     if (learned && !learned_ready(ctx))
         return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner selected but it has no parameters "
@@ -1095,10 +1099,18 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     const int max_it = ctx->prm.max_iterations;
     PcgResult res;
     bool use_mic = !learned;
-    if (learned) {
+    if (learned && (ctx->last_mic_iterations <= 0 ||
+                    (ctx->learned_refresh > 0 && ctx->learned_since_mic >= ctx->learned_refresh))) {
+        // no MIC(0) count to budget against yet (or it is old): this solve is
+        // MIC(0)-PCG, which measures it
+        use_mic = true;
+        st.learned_stop = 3;
+    } else if (learned) {
         // flexible PCG with the network; MIC(0)-PCG from scratch when it does not
         // reach the tolerance within twice the last MIC(0) iteration count
-        const int budget = ctx->last_mic_iterations > 0 ? std::min(max_it, 2 * ctx->last_mic_iterations) : max_it;
+        const int budget = std::min(max_it, 2 * ctx->last_mic_iterations);
+        st.learned_budget = budget;
+        ctx->learned_since_mic++;
         FSIM_TRY(pcg_run(ctx, S, bnorm, true, budget, &res));
         st.learned_iterations = res.iterations;
         st.learned_stop = res.converged ? 0 : res.nan ? 2 : 1;
@@ -1111,7 +1123,10 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     if (use_mic) {
         FSIM_TRY(mic_factor(ctx, S));
         FSIM_TRY(pcg_run(ctx, S, bnorm, false, max_it, &res));
-        if (res.converged) ctx->last_mic_iterations = res.iterations;
+        if (res.converged) {
+            ctx->last_mic_iterations = res.iterations;
+            ctx->learned_since_mic = 0;
+        }
     }
     st.pcg_iterations = res.iterations;
     const double sc = dt / (ctx->prm.fluid_density * ctx->g.cd);
@@ -1125,8 +1140,10 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     FSIM_TRY(read_scalars(ctx));
     st.residual_abs_inf = ctx->scal_host->tr_max;
     st.residual_rel_l2 = bnorm > 0 ? std::sqrt(ctx->scal_host->tr_ss) / bnorm : 0.0;
+    // a NaN alpha or sigma ended the solve: like the reference, the last
+    // finite pressure has been applied and the step goes on
+    st.solve_nan = res.nan ? 1 : 0;
     FSIM_TRY(check_wave_err(ctx));
-    if (ctx->scal_host->nan_flag) return set_err(ctx, FSIM_ERR_NUMERIC, "NaN in the pressure solve");
     return FSIM_OK;
 }
 

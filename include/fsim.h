@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FSIM_ABI_VERSION 1
+#define FSIM_ABI_VERSION 2
 
 typedef struct fsim_ctx fsim_ctx;
 
@@ -39,7 +39,8 @@ typedef enum fsim_status {
     FSIM_ERR_HIP = 2,         /* a HIP runtime call failed */
     FSIM_ERR_UNSUPPORTED = 3, /* feature outside the implemented path (2D mode, push-apart) */
     FSIM_ERR_OOM = 4,         /* device allocation failed */
-    FSIM_ERR_NUMERIC = 5,     /* NaN/Inf detected in the pressure solve */
+    FSIM_ERR_NUMERIC = 5,     /* reserved: a NaN in the pressure solve ends the solve as in the
+                                 reference and is reported in fsim_step_stats.solve_nan */
     FSIM_ERR_TIMEOUT = 6,     /* a bounded device-side wait expired */
     FSIM_ERR_NO_DEVICE = 7    /* no usable HIP device */
 } fsim_status;
@@ -112,6 +113,14 @@ typedef struct fsim_obstacle {
     double last_spawn_fraction; /* SOURCE: fractional carry (updated by each step) */
 } fsim_obstacle;
 
+/* One particle of the renderer's snapshot, SimulationManager::ParticleGfxData
+ * (simulationManager.h:74-78): position, speed |v|, interpolated density. */
+typedef struct fsim_gfx_particle {
+    float pos[3];
+    float v;
+    float density;
+} fsim_gfx_particle;
+
 /* Stage ids, in step order (simulator.cpp:51-100). */
 typedef enum fsim_stage {
     FSIM_STAGE_SPAWN = 0,       /* spawnParticles, simulator.cpp:106-125 */
@@ -157,7 +166,11 @@ typedef struct fsim_step_stats {
     int64_t num_particles;
     int64_t num_fluid_cells;
     int learned_iterations;             /* flexible-PCG iterations run with the learned preconditioner */
-    int learned_stop;                   /* why FPCG stopped: 0 converged, 1 iteration budget, 2 NaN */
+    int learned_stop;                   /* why FPCG stopped: 0 converged, 1 iteration budget, 2 NaN,
+                                           3 no FPCG: this solve ran MIC(0)-PCG to set the fallback budget */
+    int solve_nan;                      /* 1 if a NaN alpha or sigma ended the solve (bridsonSolverGrid.cpp:
+                                           270-272, 285-286); the last finite pressure was applied */
+    int learned_budget;                 /* FPCG iteration budget of the solve: twice the last MIC(0) count */
 } fsim_step_stats;
 
 /* Grid fields for fsim_get_grid / fsim_set_grid (host layout above). */
@@ -224,6 +237,15 @@ int fsim_get_grid(fsim_ctx* ctx, int field, void* dst, size_t bytes);
 int fsim_set_grid(fsim_ctx* ctx, int field, const void* src, size_t bytes);
 int fsim_solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out);
 
+/* ---- renderer snapshot ------------------------------------------------------ */
+/* The per-particle data SimulationManager::simulationThreadWorker builds for
+ * the renderer before each step (simulationManager.cpp:218-231) and
+ * getParticleGfxData returns (:40-43): out[i] is particle i of the reference's
+ * particle order, {pos, glm::length(v), sum over getCellsAround's 8 cells of
+ * trilinearInterpoll(pos, cell centre) * avgPNum}.  Computed on the device
+ * from the current state (call it between steps), one device-to-host copy. */
+int fsim_gfx_snapshot(fsim_ctx* ctx, int64_t capacity, fsim_gfx_particle* out, int64_t* n_out);
+
 /* ---- implementation options (no reference counterpart) ------------------- */
 typedef enum fsim_option {
     /* 0 (default): persistent wavefront kernels for the MIC(0) factor and
@@ -244,7 +266,16 @@ typedef enum fsim_option {
      * half-resolution layers stage their input tile in LDS when the level's
      * rows are a multiple of 32 cells; 0 every layer reads its operands
      * through the caches (same arithmetic, bit-identical output) */
-    FSIM_OPT_LEARNED_CONV = 4
+    FSIM_OPT_LEARNED_CONV = 4,
+    /* learned preconditioner: a solve runs MIC(0)-PCG, renewing the count the
+     * fallback budget is twice of, when no MIC(0) solve has run on the
+     * context yet and after this many learned solves (default 50; 0: only
+     * the first) */
+    FSIM_OPT_LEARNED_REFRESH = 5,
+    /* learned preconditioner: sets the MIC(0) iteration count the fallback
+     * budget is twice of, as if a MIC(0)-PCG solve had just taken that many
+     * (> 0); the next solves then run FPCG */
+    FSIM_OPT_LEARNED_MIC_COUNT = 6
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
 
@@ -261,8 +292,10 @@ int fsim_reset_kernel_times(fsim_ctx* ctx);
  * (r / max|r|, WATER mask) to an approximation of A^-1 r.  With
  * fsim_params.precond = FSIM_PRECOND_LEARNED the pressure solve runs flexible
  * PCG with it and falls back to MIC(0)-PCG (fsim_step_stats.fallback = 1) when
- * the tolerance is not reached within twice the last MIC(0) iteration count
- * (max_iterations if MIC(0) has not run yet).  The network is fully
+ * the tolerance is not reached within twice the last MIC(0) iteration count.
+ * That count comes from a MIC(0)-PCG solve of the context's own: the first
+ * solve with FSIM_PRECOND_LEARNED, and every FSIM_OPT_LEARNED_REFRESH solves
+ * after it, run MIC(0)-PCG (fsim_step_stats.learned_stop = 3).  The network is fully
  * convolutional: parameters trained on a small grid apply to any grid.
  * Parameter vector order: for the 7 conv layers (8->16, 16->32, 32->64,
  * 64->128, 192->64, 96->32, 48->16) the weights [cout][27 taps][cin] (tap =

@@ -37,6 +37,7 @@ _PROTOS = {
     "fo_solver_apply": (C.c_int, [_P, C.c_int, C.c_double, _P, _P]),
     "fo_bin": (None, [_P, _P, _P, _P]),
     "fo_splitmix64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "fo_gfx_snapshot": (None, [_P, _P]),
 }
 
 _lib = None
@@ -158,6 +159,13 @@ class Oracle:
         if self.lib.fo_solver_apply(self._s, op, dt, _ptr(v), _ptr(out)):
             raise ValueError("unknown solver op")
         return out
+
+    def gfx_snapshot(self) -> np.ndarray:
+        """(n, 5): pos, |v|, density of each particle (simulationManager.cpp:223-231)."""
+        n = self.num_particles()
+        out = np.empty((max(n, 1), 5))
+        self.lib.fo_gfx_snapshot(self._s, _ptr(out))
+        return out[:n]
 
     def bin(self):
         nc = self.grid.nc

@@ -821,6 +821,7 @@ static int fo_solve(fo_sim* s, double dt) {
     for (int64_t i = 0; i < nf; i++) total += r[i] * r[i];
     s->stats.b_norm_l2 = sqrt(total);
     s->stats.early_exit = 0;
+    s->stats.solve_nan = 0;
     s->stats.residual_abs_inf = 0;
     s->stats.residual_rel_l2 = 0;
     int it = 0;
@@ -844,7 +845,7 @@ static int fo_solve(fo_sim* s, double dt) {
     for (; it < s->prm.max_iterations; it++) {
         apply_a(s, sv, z);
         double alpha = sigma / dotv(s, sv, z, nf);
-        if (alpha != alpha) break;
+        if (alpha != alpha) { s->stats.solve_nan = 1; break; }
         for (int64_t i = 0; i < nf; i++) pres[i] += sv[i] * alpha;
         for (int64_t i = 0; i < nf; i++) r[i] += z[i] * -alpha;
         if (s->prm.stop_rule == FSIM_STOP_REL_L2) {
@@ -858,7 +859,7 @@ static int fo_solve(fo_sim* s, double dt) {
         }
         apply_precon(s, r, q, z);
         double sigma_new = dotv(s, z, r, nf);
-        if (sigma != sigma) break;
+        if (sigma != sigma) { s->stats.solve_nan = 1; break; }
         double beta = sigma_new / sigma;
         for (int64_t i = 0; i < nf; i++) sv[i] = sv[i] * beta + z[i];
         sigma = sigma_new;
@@ -969,6 +970,27 @@ void fo_bin(const fo_sim* s, int32_t* count, int32_t* start, int64_t* perm) {
     for (int64_t i = 0; i < s->np; i++) perm[start[key[i]] + fill[key[i]]++] = i;
     free(fill);
     free(key);
+}
+
+/* ---- SimulationManager's gfx snapshot, simulationManager.cpp:223-231 ---- */
+void fo_gfx_snapshot(const fo_sim* s, double* out) {
+    for (int64_t i = 0; i < s->np; i++) {
+        const fo_particle* P = &s->p[i];
+        /* getCellsAround, macGrid.cpp:188-202 */
+        int base[3];
+        for (int d = 0; d < 3; d++) base[d] = (int)(P->pos[d] * s->cdinv - 0.5);
+        float density = 0; /* the reference's float accumulator (:226) */
+        for (int k = 0; k < 8; k++) {
+            const int x = base[0] + CORNER[k][0], y = base[1] + CORNER[k][1], z = base[2] + CORNER[k][2];
+            const fo_cell* c = &s->cells[cidx(s, x, y, z)];
+            density += tri_w(P->pos, c->pos, s->cdinv) * c->avg; /* trilinearInterpoll(p.pos, c.cell.pos) */
+        }
+        out[5 * i + 0] = (float)P->pos[0];
+        out[5 * i + 1] = (float)P->pos[1];
+        out[5 * i + 2] = (float)P->pos[2];
+        out[5 * i + 3] = (float)sqrt(dot3(P->v, P->v)); /* glm::length(p.v) */
+        out[5 * i + 4] = density;
+    }
 }
 
 int fo_run_stage(fo_sim* s, int stage, double dt) {

@@ -50,6 +50,10 @@ int fo_solver_apply(fo_sim* s, int op, double dt, const double* in, double* out)
  * (cell, id) order. */
 void fo_bin(const fo_sim* s, int32_t* count, int32_t* start, int64_t* perm);
 
+/* The renderer's snapshot (simulationManager.cpp:223-231): out[5*i ..] =
+ * pos[3], |v| and the FP32-accumulated density of particle i. */
+void fo_gfx_snapshot(const fo_sim* s, double* out);
+
 /* SplitMix64 (Steele, Lea, Flood 2014): n-th output of the generator seeded
  * with `seed`. */
 uint64_t fo_splitmix64(uint64_t seed, uint64_t n);

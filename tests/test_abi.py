@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings_and_version():
     lib = capi.load()
-    assert lib.fsim_abi_version() == 1
+    assert lib.fsim_abi_version() == 2
     assert lib.fsim_status_string(0) == b"ok"
     assert lib.fsim_status_string(3) == b"unsupported feature"
 
@@ -50,11 +50,14 @@ int main(void) {
   printf("fsim_params %zu\n", sizeof(fsim_params));
   printf("fsim_obstacle %zu\n", sizeof(fsim_obstacle));
   printf("fsim_step_stats %zu\n", sizeof(fsim_step_stats));
+  printf("fsim_gfx_particle %zu\n", sizeof(fsim_gfx_particle));
   P(fsim_grid_desc, cell_d) P(fsim_grid_desc, two_d)
   P(fsim_params, pressure_k) P(fsim_params, max_iterations) P(fsim_params, fluid_density)
   P(fsim_params, stop_rule) P(fsim_params, seed)
   P(fsim_obstacle, pos) P(fsim_obstacle, r) P(fsim_obstacle, last_spawn_fraction)
   P(fsim_step_stats, step_ms) P(fsim_step_stats, residual_abs_inf) P(fsim_step_stats, num_fluid_cells)
+  P(fsim_step_stats, solve_nan) P(fsim_step_stats, learned_budget)
+  P(fsim_gfx_particle, v) P(fsim_gfx_particle, density)
   return 0;
 }
 """
@@ -73,12 +76,13 @@ def test_ctypes_structs_match_c_layout():
     assert lay["fsim_params"] == C.sizeof(capi.Params)
     assert lay["fsim_obstacle"] == C.sizeof(capi.Obstacle)
     assert lay["fsim_step_stats"] == C.sizeof(capi.StepStats)
+    assert lay["fsim_gfx_particle"] == C.sizeof(capi.GfxParticle) == capi.GFX_DTYPE.itemsize
     for key, val in lay.items():
         if "." not in key:
             continue
         struct, field = key.split(".")
         cls = {"fsim_grid_desc": capi.GridDesc, "fsim_params": capi.Params, "fsim_obstacle": capi.Obstacle,
-               "fsim_step_stats": capi.StepStats}[struct]
+               "fsim_step_stats": capi.StepStats, "fsim_gfx_particle": capi.GfxParticle}[struct]
         assert getattr(cls, field).offset == val, key
 
 

@@ -58,7 +58,8 @@ def test_facade_matches_c_abi(gpu, tmp_path, transfer):
     sim.close()
 
     assert info["particles"] == len(pos0) == len(pos)
-    assert info["step_duration_keys"] == 8
-    assert info["pcg_iterations"] == st.pcg_iterations
+    # the 8 timing keys plus "Incompressibility it count" (simulator.cpp:33-40, 91)
+    assert info["step_duration_keys"] == 9
+    assert info["pcg_iterations"] == info["it_count"] == st.pcg_iterations
     assert info["water_cells"] == int(np.count_nonzero(labels == capi.WATER))
     assert np.array_equal(facade_pos, pos.astype(np.float32))

@@ -77,3 +77,35 @@ def test_moving_sphere_bounce_geometry(gpu, oblique):
 def test_sphere_back_step_cap_hand_worked(gpu):
     say("case: sphere bounce at the back-step cap")
     _check(handworked.sphere_back_step_cap(make), 1e-6, 2e-6)
+
+
+@pytest.mark.parametrize("sphere", [False, True])
+def test_obstacle_face_speeds_hand_worked(gpu, sphere):
+    say(f"case: obstacle face speeds, 8^3, sphere {sphere}")
+    _check(handworked.obstacle_face_speeds(make, sphere), 2.4e-7, 0.0)
+
+
+@pytest.mark.parametrize("top_solid", [0, 1])
+def test_border_wall_faces_hand_worked(gpu, top_solid):
+    say(f"case: wall faces, 8^3, top solid {top_solid}")
+    _check(handworked.border_wall_faces(make, top_solid), 0.0, 0.0)
+
+
+def test_box_push_out_hand_worked(gpu):
+    say("case: box push-out, 16^3")
+    _check(handworked.box_push_out(make), 0.0, 0.0)
+
+
+def test_sink_removal_order_hand_worked(gpu):
+    say("case: sink removal keeps the survivors' order, 16^3")
+    _check(handworked.sink_removal_order(make), 0.0, 0.0)
+
+
+def test_solve_nan_break_hand_worked(gpu):
+    say("case: pressure solve ends on a NaN alpha (an infinite face), 8^3")
+    _check(handworked.solve_nan_break(make, DT), 0.0, 0.0)
+
+
+def test_gfx_snapshot_linear_density_hand_worked(gpu):
+    say("case: gfx snapshot of a linear density field, 16^3")
+    _check(handworked.gfx_snapshot_linear_density(make), 4e-6, 4e-6)

@@ -115,12 +115,14 @@ def test_spawn_parity(gpu):
     assert abs(sim.get_obstacles()[0].last_spawn_fraction - ref.get_obstacles()[0].last_spawn_fraction) < 1e-12
 
 
-@pytest.mark.parametrize("transfer", [capi.PIC, capi.FLIP, capi.APIC])
-def test_p2g_mark_parity(gpu, transfer):
+@pytest.mark.parametrize("transfer,pressure_enabled", [(capi.PIC, 1), (capi.FLIP, 1), (capi.APIC, 1), (capi.FLIP, 0)])
+def test_p2g_mark_parity(gpu, transfer, pressure_enabled):
+    """The density (avgPNum) is computed whether or not the right-hand side reads it (simulator.cpp:82,
+    354-368), as the gfx snapshot reads it (simulationManager.cpp:223-230)."""
     grid = scenes.grid(16)
-    say(f"case: p2g+mark 16^3 transfer={transfer}, seed 11")
+    say(f"case: p2g+mark 16^3 transfer={transfer} pressure_enabled={pressure_enabled}, seed 11")
     pos, vel, c = random_particles(grid, 30000, seed=11, apic=transfer == capi.APIC)
-    prm = scenes.params(transfer=transfer, pressure_enabled=1)
+    prm = scenes.params(transfer=transfer, pressure_enabled=pressure_enabled)
     sim, ref = pair(grid, prm, pos, vel, c)
     for stage in (capi.STAGE_P2G, capi.STAGE_MARK):
         sim.run_stage(stage, DT)
@@ -251,7 +253,7 @@ def test_whole_step_parity(gpu, transfer, sphere):
         st = sim.step(DT)
         ref.step(DT)
         assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL)), k
-        for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W):
+        for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W, capi.FIELD_DENSITY):
             assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-5, (k, f)
         assert field_rel_err(sim.get_grid(capi.FIELD_PRESSURE), ref.get_grid(capi.FIELD_PRESSURE)) < 1e-5, k
         p_gpu, v_gpu, _ = sim.get_particles()
@@ -426,3 +428,21 @@ def test_config_256_flip_double_dam_break_properties(gpu):
         assert abs(div - st.residual_rel_l2) <= bound
         assert div <= 1e-6 + bound
     sim.close()
+
+
+def test_gfx_snapshot_parity(gpu):
+    """The renderer's snapshot (simulationManager.cpp:218-231) after P2G of a dam-break state: records in
+    particle order, |v| and the interpolated density as the oracle computes them."""
+    say("case: gfx snapshot 16^3 FLIP dam break with a sphere, 3 steps in")
+    sc, prm, pos, vel, c = dam_state(16, capi.FLIP, steps=3, sphere=True)
+    sim, ref = pair(sc.grid, prm, pos, vel, c, obstacles=sc.obstacles)
+    for stage in (capi.STAGE_P2G, capi.STAGE_MARK):
+        sim.run_stage(stage, DT)
+        ref.run_stage(stage, DT)
+    g = sim.gfx_snapshot()
+    r = ref.gfx_snapshot()
+    assert len(g) == len(r) == len(pos)
+    assert np.array_equal(g["pos"], pos)
+    assert field_rel_err(g["v"], r[:, 3]) < 1e-6
+    assert field_rel_err(g["density"], r[:, 4]) < 1e-5
+    assert np.all(r[:, 4] > 0)

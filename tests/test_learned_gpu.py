@@ -141,9 +141,11 @@ def test_training_lowers_the_loss(gpu):
     assert np.isfinite(last) and last < 0.5 * first
 
 
-def fpcg_step(grid, pos, params):
+def fpcg_step(grid, pos, params, mic_count):
+    """One learned step on a fresh context whose fallback budget is 2 x mic_count."""
     sim = capi.Simulation(grid, scenes.params(precond=capi.PRECOND_LEARNED))
     sim.learned_set_params(params)
+    sim.set_option(capi.OPT_LEARNED_MIC_COUNT, mic_count)
     sim.set_particles(pos)
     st = sim.step(DT)
     p = sim.get_grid(capi.FIELD_PRESSURE)
@@ -166,8 +168,8 @@ def test_trained_network_beats_the_solver_without_it(gpu):
     sim.close()
     B, L, P = np.concatenate([b[None], rb]), np.concatenate([lab[None], rl]), np.concatenate([p_mic[None], rp])
     params, hist = learned.train(grid, B, L, P, epochs=60, batch=4, lr=1e-3, seed=1, loss=learned.LOSS_ENERGY)
-    st, p = fpcg_step(grid, pos, params)
-    st0, _ = fpcg_step(grid, pos, np.zeros(capi.learned_num_params(), np.float32))
+    st, p = fpcg_step(grid, pos, params, st_mic.pcg_iterations)
+    st0, _ = fpcg_step(grid, pos, np.zeros(capi.learned_num_params(), np.float32), st_mic.pcg_iterations)
     say(f"  loss {hist[0]:.3e} -> {hist[-1]:.3e}; MIC(0)-PCG {st_mic.pcg_iterations} it; trained FPCG "
         f"{st.learned_iterations} it (stop {st.learned_stop}); zero network {st0.learned_iterations} it")
     assert st.fallback == 0 and st.learned_stop == 0
@@ -183,13 +185,46 @@ def test_fallback_is_forced_by_an_overflowing_network(gpu):
     grid, pos, b, lab, p_mic, st_mic = solved_sample(16, [0.0, 0.0, 0.0, 0.5, 0.6, 1.0])
     sim = capi.Simulation(grid, scenes.params(precond=capi.PRECOND_LEARNED))
     sim.learned_set_params(np.full(capi.learned_num_params(), 1e30, np.float32))
+    sim.set_option(capi.OPT_LEARNED_MIC_COUNT, st_mic.pcg_iterations)
     sim.set_particles(pos)
     st = sim.step(DT)
     p = sim.get_grid(capi.FIELD_PRESSURE)
     sim.close()
-    assert st.fallback == 1 and st.learned_stop == 2
-    assert st.pcg_iterations == st_mic.pcg_iterations
+    assert st.fallback == 1 and st.learned_stop == 2 and st.learned_budget == 2 * st_mic.pcg_iterations
+    assert st.pcg_iterations == st_mic.pcg_iterations and st.solve_nan == 0
     assert np.array_equal(p, p_mic)
+
+
+def test_fallback_budget_comes_from_the_context_s_own_mic0_solves(gpu):
+    """A fresh context has no MIC(0) count: its first learned solve runs MIC(0)-PCG (learned_stop 3),
+    and the FPCG solves after it get twice that count; every FSIM_OPT_LEARNED_REFRESH learned solves
+    a MIC(0)-PCG solve renews the count."""
+    say("case: FPCG fallback budget from the context's MIC(0) solves, 16^3, refresh period 2")
+    grid, pos, b, lab, p_mic, st_mic = solved_sample(16, [0.0, 0.0, 0.0, 0.5, 0.6, 1.0])
+    sim = capi.Simulation(grid, scenes.params(precond=capi.PRECOND_LEARNED))
+    sim.learned_set_params(np.zeros(capi.learned_num_params(), np.float32))
+    sim.set_option(capi.OPT_LEARNED_REFRESH, 2)
+    sim.set_particles(pos)
+    stats = [sim.step(DT) for _ in range(5)]
+    sim.close()
+    s0 = stats[0]
+    assert s0.learned_stop == 3 and s0.learned_iterations == 0 and s0.fallback == 0
+    assert s0.pcg_iterations == st_mic.pcg_iterations
+    assert s0.residual_rel_l2 <= 1e-6 * 1.01
+    mic, since, calibrations = s0.pcg_iterations, 0, 0
+    for st in stats[1:]:
+        assert st.residual_rel_l2 <= 1e-6 * 1.01
+        if since == 2:   # two learned solves since the last MIC(0) solve
+            assert st.learned_stop == 3 and st.learned_iterations == 0
+            mic, since, calibrations = st.pcg_iterations, 0, calibrations + 1
+            continue
+        assert st.learned_stop in (0, 1) and st.learned_budget == 2 * mic
+        assert st.learned_iterations <= st.learned_budget
+        assert st.fallback == (st.learned_stop == 1)
+        since += 1
+        if st.fallback:   # the fallback's MIC(0)-PCG solve renews the count too
+            mic, since = st.pcg_iterations, 0
+    assert calibrations >= 1 or any(st.fallback for st in stats)
 
 
 def test_learned_without_parameters_is_an_error(gpu):

@@ -448,3 +448,30 @@ def test_moving_sphere_bounce_geometry(oblique):
 
 def test_sphere_back_step_cap_hand_worked():
     _check(handworked.sphere_back_step_cap(mk), 1e-12)
+
+
+@pytest.mark.parametrize("sphere", [False, True])
+def test_obstacle_face_speeds_hand_worked(sphere):
+    _check(handworked.obstacle_face_speeds(mk, sphere), 1e-15)
+
+
+@pytest.mark.parametrize("top_solid", [0, 1])
+def test_border_wall_faces_hand_worked(top_solid):
+    _check(handworked.border_wall_faces(mk, top_solid), 0.0)
+
+
+def test_box_push_out_hand_worked():
+    _check(handworked.box_push_out(mk), 1e-15)
+
+
+def test_sink_removal_order_hand_worked():
+    _check(handworked.sink_removal_order(mk), 0.0)
+
+
+def test_solve_nan_break_hand_worked():
+    _check(handworked.solve_nan_break(mk, DT), 0.0)
+
+
+def test_gfx_snapshot_linear_density_hand_worked():
+    # the density is summed in FP32 as in the reference (float density += double)
+    _check(handworked.gfx_snapshot_linear_density(mk), 4e-6)
