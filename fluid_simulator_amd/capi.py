@@ -25,10 +25,10 @@ WATER, AIR, SOLID = 0, 1, 2
 # fsim_obstacle_kind (obstacles.hpp:7-65)
 OBS_BOX, OBS_SPHERE, OBS_SOURCE, OBS_SINK = 0, 1, 2, 3
 STOP_ABS_INF, STOP_REL_L2 = 0, 1
-PRECOND_MIC0, PRECOND_LEARNED = 0, 1
+PRECOND_MIC0, PRECOND_LEARNED, SOLVER_BASIC_SOR = 0, 1, 2
 # fsim_stage
 (STAGE_SPAWN, STAGE_ADVECT, STAGE_PUSH_OUT, STAGE_STOP, STAGE_BIN, STAGE_P2G, STAGE_MARK, STAGE_OBSTACLES,
- STAGE_BORDERS, STAGE_POST_P2G, STAGE_SOLVE, STAGE_EXTRAPOLATE, STAGE_G2P) = range(13)
+ STAGE_BORDERS, STAGE_POST_P2G, STAGE_SOLVE, STAGE_EXTRAPOLATE, STAGE_G2P, STAGE_PUSH_APART) = range(14)
 # fsim_timer, named as the reference's stepDuration keys (simulator.cpp:31-41)
 TIMER_KEYS = ("SimulateParticles", "PushParticlesApart", "PushParticlesOutOfObstacles", "P2GTransfer",
               "IncompressibilityPrep", "Incompressibility", "VelocityExtrapolation", "G2PTransfer")

@@ -30,7 +30,7 @@ int stage_p2g_fused(fsim_ctx* ctx);
 static const char* const kKernelNames[KID_COUNT] = {
     "spawn", "move", "bin_keys", "bin_scan", "bin_scatter", "bin_segsort", "bin_gather", "p2g", "mark",
     "obstacles", "borders", "post_p2g", "rhs", "mic_factor", "sweep_fwd", "sweep_bwd", "spmv", "update", "dot",
-    "true_residual", "pressure", "extrapolate", "g2p", "learned"};
+    "true_residual", "pressure", "extrapolate", "g2p", "learned", "basic_sor", "push_apart"};
 
 int set_profiling(fsim_ctx* ctx, bool on) {
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -89,7 +89,8 @@ static int check_params(const fsim_params* p) {
     if (p->transfer < FSIM_PIC || p->transfer > FSIM_APIC) return FSIM_ERR_INVALID;
     if (p->max_iterations < 0 || !(p->fluid_density > 0.0)) return FSIM_ERR_INVALID;
     if (p->stop_rule != FSIM_STOP_ABS_INF && p->stop_rule != FSIM_STOP_REL_L2) return FSIM_ERR_INVALID;
-    if (p->precond != FSIM_PRECOND_MIC0 && p->precond != FSIM_PRECOND_LEARNED) return FSIM_ERR_INVALID;
+    if (p->precond != FSIM_PRECOND_MIC0 && p->precond != FSIM_PRECOND_LEARNED && p->precond != FSIM_SOLVER_BASIC_SOR)
+        return FSIM_ERR_INVALID;
     return FSIM_OK;
 }
 
@@ -107,7 +108,7 @@ static void free_all(fsim_ctx* ctx) {
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
                     ctx->scan_tmp, ctx->nbinfo, ctx->chunk_flags, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
                     ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->ybnd, ctx->zbnd, ctx->s2, ctx->red_count,
-                    ctx->wave_ticket, ctx->wave_err};
+                    ctx->wave_ticket, ctx->wave_err, ctx->sor64};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
@@ -358,6 +359,7 @@ static int run_stage(fsim_ctx* ctx, int stage, double dt) {
     case FSIM_STAGE_SOLVE: return stage_solve(ctx, dt);
     case FSIM_STAGE_EXTRAPOLATE: return stage_extrapolate(ctx);
     case FSIM_STAGE_G2P: return stage_g2p(ctx);
+    case FSIM_STAGE_PUSH_APART: return stage_push_apart(ctx);
     default: return set_err(ctx, FSIM_ERR_INVALID, "unknown stage");
     }
 }
@@ -371,16 +373,23 @@ int fsim_run_stage(fsim_ctx* ctx, int stage, double dt) {
 
 int fsim_step(fsim_ctx* ctx, double dt, fsim_step_stats* stats) {
     if (!ctx) return FSIM_ERR_INVALID;
-    if (ctx->prm.push_apart_enabled)
-        return set_err(ctx, FSIM_ERR_UNSUPPORTED, "pushParticlesApart is not implemented; disable push_apart_enabled");
     if (!(dt > 0.0)) return set_err(ctx, FSIM_ERR_INVALID, "dt must be positive");
     auto t0 = std::chrono::steady_clock::now();
     hipEvent_t* ev = ctx->ev;
+    const bool apart = ctx->prm.push_apart_enabled != 0;
     FSIM_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
     FSIM_TRY(upload_obstacles(ctx));
     if (ctx->prm.spawning_enabled) FSIM_TRY(stage_spawn(ctx, dt));
-    FSIM_TRY(stage_move(ctx, dt, 1 | 2 | 4));
-    FSIM_HIP(ctx, hipEventRecord(ev[1], ctx->stream));   // SimulateParticles (+ fused push-out)
+    if (apart) {   // advect | push apart | push out + stop (simulator.cpp:57-74)
+        FSIM_TRY(stage_move(ctx, dt, 1));
+        FSIM_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
+        FSIM_TRY(stage_push_apart(ctx));
+        FSIM_HIP(ctx, hipEventRecord(ev[8], ctx->stream));
+        FSIM_TRY(stage_move(ctx, dt, 2 | 4));
+    } else {
+        FSIM_TRY(stage_move(ctx, dt, 1 | 2 | 4));   // advect and push-out fused per particle
+    }
+    FSIM_HIP(ctx, hipEventRecord(ev[1], ctx->stream));   // SimulateParticles (+ push-out when fused)
     FSIM_TRY(stage_bin(ctx));
     FSIM_TRY(stage_p2g_fused(ctx));   // reset + P2G + fluid marking + particle density
     FSIM_HIP(ctx, hipEventRecord(ev[2], ctx->stream));   // P2GTransfer
@@ -400,6 +409,15 @@ int fsim_step(fsim_ctx* ctx, double dt, fsim_step_stats* stats) {
     for (int k = 0; k < 6; k++) FSIM_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
     std::memset(st.timer_ms, 0, sizeof(st.timer_ms));
     st.timer_ms[FSIM_T_SIMULATE_PARTICLES] = ms[0];
+    if (apart) {
+        float a[3];
+        FSIM_HIP(ctx, hipEventElapsedTime(&a[0], ev[0], ev[7]));
+        FSIM_HIP(ctx, hipEventElapsedTime(&a[1], ev[7], ev[8]));
+        FSIM_HIP(ctx, hipEventElapsedTime(&a[2], ev[8], ev[1]));
+        st.timer_ms[FSIM_T_SIMULATE_PARTICLES] = a[0];
+        st.timer_ms[FSIM_T_PUSH_APART] = a[1];
+        st.timer_ms[FSIM_T_PUSH_OUT] = a[2];
+    }
     st.timer_ms[FSIM_T_P2G] = ms[1];
     st.timer_ms[FSIM_T_INCOMPRESSIBILITY_PREP] = ms[2];
     st.timer_ms[FSIM_T_INCOMPRESSIBILITY] = ms[3];

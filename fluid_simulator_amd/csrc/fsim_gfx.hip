@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(kBlock) k_gfx_snapshot(GfxArgs A) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int c[3] = {base[0] + kGfxCorner[k][0], base[1] + kGfxCorner[k][1], base[2] + kGfxCorner[k][2]};
-        // particles stay inside the clamp box, so the 8 cells exist; the test keeps a stray one in bounds
+        // particles stay inside the clamp box, so the 8 cells exist; any other position skips the missing ones
         if ((unsigned)c[0] >= (unsigned)A.nx || (unsigned)c[1] >= (unsigned)A.ny || (unsigned)c[2] >= (unsigned)A.nz)
             continue;
         double w = 1.0;

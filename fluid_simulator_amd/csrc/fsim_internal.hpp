@@ -134,7 +134,7 @@ enum KernelId {
     KID_SPAWN = 0, KID_MOVE, KID_BIN_KEYS, KID_BIN_SCAN, KID_BIN_SCATTER, KID_BIN_SEGSORT, KID_BIN_GATHER,
     KID_P2G, KID_MARK, KID_OBSTACLES, KID_BORDERS, KID_POST_P2G, KID_RHS, KID_MIC_FACTOR, KID_SWEEP_FWD,
     KID_SWEEP_BWD, KID_SPMV, KID_UPDATE, KID_DOT, KID_TRUE_RES, KID_PRESSURE, KID_EXTRAPOLATE, KID_G2P,
-    KID_LEARNED, KID_COUNT
+    KID_LEARNED, KID_SOR, KID_PUSH_APART, KID_COUNT
 };
 
 struct Particles {
@@ -188,6 +188,7 @@ struct fsim_ctx {
     float* P32 = nullptr;
     double* P64 = nullptr;
     double* x = nullptr;
+    double* sor64 = nullptr;          // BasicMacGrid SOR: FP64 faces u, v, w (allocated on first use)
     double* partials = nullptr;       // block partial sums (FP64)
     unsigned* red_count = nullptr;    // blocks finished in the running reduction (0 between launches)
     hipEvent_t res_ev = nullptr;      // PCG: the iteration's scalars are on the host
@@ -279,6 +280,7 @@ int stage_spawn(fsim_ctx* ctx, double dt);
 // flags: 1 = advect, 2 = push out, 4 = stop particles
 int stage_move(fsim_ctx* ctx, double dt, int flags);
 int stage_bin(fsim_ctx* ctx);
+int stage_push_apart(fsim_ctx* ctx);
 int upload_obstacles(fsim_ctx* ctx);
 
 // ---- grid stages (fsim_grid.hip) ------------------------------------------

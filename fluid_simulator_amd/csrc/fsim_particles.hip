@@ -297,6 +297,86 @@ int stage_move(fsim_ctx* ctx, double dt, int flags) {
     return FSIM_OK;
 }
 
+// ---- pushParticlesApart (hashedParticles.cpp:64-107) -----------------------------
+// Deterministic Jacobi form of the reference's racy parallel loop: every
+// particle sums, from the positions before the pass, its own offset
+// (p_i - p_j) (D - |p_i - p_j|) / |p_i - p_j| * 0.5 for each other particle
+// closer than the diameter D = 2r and not closer than 1e-4 (:86-95), visiting
+// them through the cell binning in (cell, id) order (all lie within
+// R = ceil(D / cellD) cells), and is clamped once (:96-101).  An isolated pair
+// ends exactly D apart, as in the reference.  FP64 sums in that fixed order;
+// the result goes to the alternate position planes.
+struct ApartArgs {
+    const float* pos[3];
+    float* out[3];
+    const int32_t* start;
+    int64_t n;
+    int nx, ny, nz, R;
+    double cdinv, D, D2;
+    double lo[3], hi[3];
+};
+
+__global__ void __launch_bounds__(kBlock) k_push_apart(ApartArgs A) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const double pi[3] = {(double)A.pos[0][i], (double)A.pos[1][i], (double)A.pos[2][i]};
+    const int n3[3] = {A.nx, A.ny, A.nz};
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) c[a] = min(max((int)(pi[a] * A.cdinv), 0), n3[a] - 1);   // the bin key's cell
+    double q[3] = {pi[0], pi[1], pi[2]};
+    for (int x = max(c[0] - A.R, 0); x <= min(c[0] + A.R, A.nx - 1); x++)
+        for (int y = max(c[1] - A.R, 0); y <= min(c[1] + A.R, A.ny - 1); y++) {
+            const int64_t row = ((int64_t)x * A.ny + y) * A.nz;
+            const int z0 = max(c[2] - A.R, 0), z1 = min(c[2] + A.R, A.nz - 1);
+            // the cells z0..z1 of one row are one contiguous slot range
+            const int b = A.start[row + z0], e = A.start[row + z1 + 1];
+            for (int j = b; j < e; j++) {
+                if (j == (int)i) continue;
+                const double d[3] = {pi[0] - (double)A.pos[0][j], pi[1] - (double)A.pos[1][j],
+                                     pi[2] - (double)A.pos[2][j]};
+                const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (d2 > A.D2 || d2 < 1e-8) continue;
+                const double dist = sqrt(d2);
+                const double tmp = (A.D - dist) / dist;
+#pragma unroll
+                for (int a = 0; a < 3; a++) q[a] += d[a] * tmp * 0.5;
+            }
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++) A.out[a][i] = (float)(q[a] < A.lo[a] ? A.lo[a] : (q[a] > A.hi[a] ? A.hi[a] : q[a]));
+}
+
+int stage_push_apart(fsim_ctx* ctx) {
+    FSIM_TRY(stage_bin(ctx));   // sorted by (cell, id); sink removals compacted
+    const int64_t n = ctx->P.n;
+    if (n == 0) return FSIM_OK;
+    FSIM_PROF(ctx, KID_PUSH_APART);
+    ApartArgs A{};
+    for (int a = 0; a < 3; a++) {
+        A.pos[a] = ctx->P.pos[a];
+        A.out[a] = ctx->Palt.pos[a];
+        A.lo[a] = ctx->g.cd + ctx->g.pr * 1.01;
+        A.hi[a] = ctx->g.dims[a] - A.lo[a];
+    }
+    A.start = ctx->cell_start;
+    A.n = n;
+    A.nx = ctx->g.nx;
+    A.ny = ctx->g.ny;
+    A.nz = ctx->g.nz;
+    A.cdinv = ctx->g.cdinv;
+    A.D = ctx->g.pr * 2.0;
+    A.D2 = A.D * A.D;
+    A.R = (int)std::ceil(A.D * ctx->g.cdinv);
+    hipLaunchKernelGGL(k_push_apart, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, A);
+    FSIM_HIP(ctx, hipGetLastError());
+    for (int a = 0; a < 3; a++)
+        FSIM_HIP(ctx, hipMemcpyAsync(ctx->P.pos[a], ctx->Palt.pos[a], sizeof(float) * (size_t)n,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    return FSIM_OK;
+}
+
 // ---- spawning -----------------------------------------------------------------
 struct SpawnArgs {
     float* pos[3];

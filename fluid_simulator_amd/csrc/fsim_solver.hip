@@ -1062,6 +1062,120 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
     return FSIM_OK;
 }
 
+// ---- BasicMacGrid::solveIncompressibility, basicMacGrid.cpp:5-102 -------------
+// The reference's release (parallel) branch: each iteration one pass over the
+// cells with x + y + z odd, then one over the even ones.  A WATER cell with s
+// non-SOLID neighbours computes d = (-u - v - w + u(-x) + v(-y) + w(-z)
+// [+ (avgPNum - averagePressure) pressureK]) * 1.98 / s and moves its six
+// faces by +-d where the cell across is not SOLID.  Cells of one colour share
+// no face, so one thread per cell of the colour needs no atomics and the
+// result does not depend on the schedule.  The faces are FP64 during the
+// sweeps (the reference's Face::v2 type): over many iterations of an
+// over-relaxed sweep, FP32 rounding would grow far beyond the oracle's.
+struct SorArgs {
+    int nx, ny, nz, kz;
+    const uint8_t* label;
+    const float* density;
+    double* f[3];
+    double pk, avg_p;
+    int pressure_enabled;
+};
+
+template <int ODD>
+__global__ void __launch_bounds__(kBlock) k_sor_pass(SorArgs A) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t rows = (int64_t)(A.nx - 2) * (A.ny - 2);
+    if (t >= rows * A.kz) return;
+    const int k = (int)(t % A.kz);
+    const int64_t r = t / A.kz;
+    const int y = 1 + (int)(r % (A.ny - 2));
+    const int x = 1 + (int)(r / (A.ny - 2));
+    const int z = (ODD ? 1 + ((x + y) & 1) : 2 - ((x + y) & 1)) + 2 * k;
+    if (z > A.nz - 2) return;
+    const int64_t sx = (int64_t)A.ny * A.nz, sy = A.nz;
+    const int64_t i = ((int64_t)x * A.ny + y) * A.nz + z;
+    if (A.label[i] != FSIM_WATER) return;
+    const int s1 = A.label[i + 1] != FSIM_SOLID, s2 = A.label[i - 1] != FSIM_SOLID;
+    const int s3 = A.label[i + sy] != FSIM_SOLID, s4 = A.label[i - sy] != FSIM_SOLID;
+    const int s5 = A.label[i + sx] != FSIM_SOLID, s6 = A.label[i - sx] != FSIM_SOLID;
+    const int sum = s1 + s2 + s3 + s4 + s5 + s6;
+    if (sum == 0) return;
+    double* const u = A.f[0];
+    double* const v = A.f[1];
+    double* const w = A.f[2];
+    double d = -u[i] - v[i] - w[i] + u[i - sx] + v[i - sy] + w[i - 1] +
+               (A.pressure_enabled ? ((double)A.density[i] - A.avg_p) * A.pk : 0.0);
+    d = d * 1.98 / sum;   // overRelaxation, basicMacGrid.cpp:13
+    if (s1) w[i] += d;
+    if (s2) w[i - 1] -= d;
+    if (s3) v[i] += d;
+    if (s4) v[i - sy] -= d;
+    if (s5) u[i] += d;
+    if (s6) u[i - sx] -= d;
+}
+
+__global__ void __launch_bounds__(kBlock) k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+__global__ void __launch_bounds__(kBlock) k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+static int stage_basic_sor(fsim_ctx* ctx, const SolveArgs& S) {
+    fsim_step_stats& st = ctx->stats;
+    const int64_t nc = S.nc;
+    if (!ctx->sor64) {
+        if (hipMalloc(reinterpret_cast<void**>(&ctx->sor64), sizeof(double) * 3 * (size_t)nc) != hipSuccess) {
+            ctx->sor64 = nullptr;
+            return set_err(ctx, FSIM_ERR_OOM, "SOR face buffer allocation failed");
+        }
+    }
+    FSIM_HIP(ctx, hipMemsetAsync(ctx->scal, 0, sizeof(SolveScalars), ctx->stream));
+    FSIM_TRY(fill_f64(ctx, ctx->x, 0.0, nc));   // no pressure field
+    FSIM_TRY(build_nbinfo(ctx, S, true));
+    FSIM_TRY(read_scalars(ctx));
+    const double b0 = std::sqrt(ctx->scal_host->bb);
+    st.b_norm_l2 = b0;
+    st.num_fluid_cells = (int64_t)ctx->scal_host->nfluid;
+    SorArgs A{};
+    A.nx = S.nx;
+    A.ny = S.ny;
+    A.nz = S.nz;
+    A.kz = (S.nz - 1) / 2;   // z values of one colour in a row: at most this many
+    A.label = ctx->label;
+    A.density = ctx->density;
+    float* f32[3] = {ctx->u2, ctx->v2, ctx->w2};
+    for (int a = 0; a < 3; a++) A.f[a] = ctx->sor64 + (size_t)a * nc;
+    A.pk = ctx->prm.pressure_k;
+    A.avg_p = ctx->prm.average_pressure;
+    A.pressure_enabled = ctx->prm.pressure_enabled;
+    const int iters = ctx->prm.max_iterations;
+    {
+        FSIM_PROF(ctx, KID_SOR);
+        for (int a = 0; a < 3; a++)
+            hipLaunchKernelGGL(k_f32_to_f64, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, nc, f32[a], A.f[a]);
+        const int64_t work = (int64_t)(S.nx - 2) * (S.ny - 2) * A.kz;
+        for (int n = 0; n < iters; n++) {
+            hipLaunchKernelGGL(k_sor_pass<1>, dim3(nblocks(work)), dim3(kBlock), 0, ctx->stream, A);
+            hipLaunchKernelGGL(k_sor_pass<0>, dim3(nblocks(work)), dim3(kBlock), 0, ctx->stream, A);
+        }
+        for (int a = 0; a < 3; a++)
+            hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, nc, A.f[a], f32[a]);
+        FSIM_HIP(ctx, hipGetLastError());
+    }
+    // the right-hand side left on the projected faces (FSIM_FIELD_RHS reads it)
+    FSIM_HIP(ctx, hipMemsetAsync(ctx->scal, 0, sizeof(SolveScalars), ctx->stream));
+    FSIM_TRY(build_nbinfo(ctx, S, true));
+    FSIM_TRY(read_scalars(ctx));
+    st.residual_rel_l2 = b0 > 0 ? std::sqrt(ctx->scal_host->bb) / b0 : 0.0;
+    st.residual_abs_inf = 0.0;
+    st.pcg_iterations = iters;   // BasicMacGrid returns its iteration count (basicMacGrid.cpp:10)
+    return FSIM_OK;
+}
+
 int stage_solve(fsim_ctx* ctx, double dt) {
     const SolveArgs S = solve_args(ctx, dt);
     const int64_t nc = S.nc;
@@ -1076,6 +1190,7 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     st.learned_stop = 0;
     st.solve_nan = 0;
     st.learned_budget = 0;
+    if (ctx->prm.precond == FSIM_SOLVER_BASIC_SOR) return stage_basic_sor(ctx, S);
     const bool learned = ctx->prm.precond == FSIM_PRECOND_LEARNED;
     if (learned && !learned_ready(ctx))
         return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner selected but it has no parameters "

@@ -57,8 +57,35 @@ def double_dam_break(n: int, transfer=capi.FLIP, seed=1, steps=100) -> Scene:
                  [[0.0, 0.0, 0.0, 0.4, 0.6, 1.0], [0.6, 0.0, 0.0, 1.0, 0.6, 1.0]], [], seed, steps)
 
 
+def boxes_and_source(n: int, transfer=capi.FLIP, seed=5, steps=500, per_step: int = 2000) -> Scene:
+    """The double dam break with 3-8 solid boxes and a particle source.
+
+    Box count, edge lengths (each U[0.05, 0.2]) and centres come from a seeded
+    numpy generator: centre x, z ~ U[0.2, 0.8], y ~ U[size_y / 2 + 0.05, 0.45]
+    (in the water's path).  The source (SphericalParticleSource,
+    obstacles.hpp:45-56) sits at (0.5, 0.8, 0.5) with radius 0.04 and spawns
+    per_step particles per step at DT (rate per_step / DT, simulator.cpp:
+    112-114), launched outwards at 0.5 m/s.  Cells inside a box or the source
+    start without particles."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    nbox = int(rng.integers(3, 9))
+    obs = []
+    for _ in range(nbox):
+        size = rng.uniform(0.05, 0.2, 3)
+        ctr = (rng.uniform(0.2, 0.8), rng.uniform(size[1] / 2 + 0.05, 0.45), rng.uniform(0.2, 0.8))
+        obs.append(capi.make_obstacle(capi.OBS_BOX, ctr, size=tuple(float(v) for v in size)))
+    obs.append(capi.make_obstacle(capi.OBS_SOURCE, (0.5, 0.8, 0.5), r=0.04, spawn_rate=per_step / DT,
+                                  spawn_speed=0.5))
+    return Scene(f"boxes_and_source_{n}", n, transfer,
+                 [[0.0, 0.0, 0.0, 0.4, 0.6, 1.0], [0.6, 0.0, 0.0, 1.0, 0.6, 1.0]], obs, seed, steps)
+
+
 # The benchmark configurations: 32^3 CPU-path case, 128^3 APIC with a sphere,
-# 256^3 FLIP double dam break.
+# 256^3 FLIP double dam break, and the 256^3 learned run with boxes and a
+# particle source.
 CONFIG_32_FLIP = lambda: dam_break(32, capi.FLIP, seed=0, steps=100)  # noqa: E731
 CONFIG_128_APIC_SPHERE = lambda: dam_break(128, capi.APIC, seed=0, sphere=True, steps=200)  # noqa: E731
 CONFIG_256_FLIP_DOUBLE = lambda: double_dam_break(256, capi.FLIP, seed=1, steps=100)  # noqa: E731
+CONFIG_256_BOXES_SOURCE = lambda: boxes_and_source(256, capi.FLIP, seed=5, steps=500)  # noqa: E731

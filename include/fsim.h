@@ -37,7 +37,7 @@ typedef enum fsim_status {
     FSIM_OK = 0,
     FSIM_ERR_INVALID = 1,     /* bad argument or state */
     FSIM_ERR_HIP = 2,         /* a HIP runtime call failed */
-    FSIM_ERR_UNSUPPORTED = 3, /* feature outside the implemented path (2D mode, push-apart) */
+    FSIM_ERR_UNSUPPORTED = 3, /* feature outside the implemented path */
     FSIM_ERR_OOM = 4,         /* device allocation failed */
     FSIM_ERR_NUMERIC = 5,     /* reserved: a NaN in the pressure solve ends the solve as in the
                                  reference and is reported in fsim_step_stats.solve_nan */
@@ -64,10 +64,18 @@ typedef enum fsim_obstacle_kind {
  * (bridsonSolverGrid.cpp:276-281); REL_L2 is ||r||_2 / ||b||_2 <= tol. */
 typedef enum fsim_stop_rule { FSIM_STOP_ABS_INF = 0, FSIM_STOP_REL_L2 = 1 } fsim_stop_rule;
 
-/* Preconditioner of the pressure solve. MIC0 = Bridson's MIC(0)
- * (bridsonSolverGrid.cpp:91-163).  LEARNED = flexible PCG with the learned
- * conv preconditioner (no reference counterpart), MIC(0)-PCG as fallback. */
-typedef enum fsim_precond { FSIM_PRECOND_MIC0 = 0, FSIM_PRECOND_LEARNED = 1 } fsim_precond;
+/* The pressure-solve plugin, the reference's MacGrid::solveIncompressibility
+ * (macGrid.h:165) chosen by SimulationConfig::GridSolverType
+ * (simulationManager.h:35-38).  MIC0 = BridsonSolverGrid, MIC(0)-PCG
+ * (bridsonSolverGrid.cpp:91-293).  LEARNED = flexible PCG with the learned
+ * preconditioner (no reference counterpart), MIC(0)-PCG as fallback.
+ * BASIC_SOR = BasicMacGrid (GridSolverType::BASIC): max_iterations red-black
+ * SOR sweeps on the faces (basicMacGrid.cpp:5-102), no pressure field. */
+typedef enum fsim_precond {
+    FSIM_PRECOND_MIC0 = 0,
+    FSIM_PRECOND_LEARNED = 1,
+    FSIM_SOLVER_BASIC_SOR = 2
+} fsim_precond;
 
 typedef struct fsim_grid_desc {
     int nx, ny, nz;      /* MacGrid::gridSize (macGrid.cpp:11), every axis >= 4 */
@@ -82,7 +90,7 @@ typedef struct fsim_params {
     float flip_ratio;
     float gravity;           /* added to y-face v2 as gravity*dt (macGrid.cpp:209-210) */
     int gravity_enabled;
-    int push_apart_enabled;  /* must be 0 (push-apart is not implemented) */
+    int push_apart_enabled;  /* pushParticlesApart after advection (FSIM_STAGE_PUSH_APART) */
     int spawning_enabled;
     int despawning_enabled;
     int stop_particles;
@@ -136,7 +144,11 @@ typedef enum fsim_stage {
     FSIM_STAGE_SOLVE = 10,      /* solveIncompressibility, bridsonSolverGrid.cpp:244-293 */
     FSIM_STAGE_EXTRAPOLATE = 11,/* extrapolateVelocities, macGrid.cpp:294-336 */
     FSIM_STAGE_G2P = 12,        /* g2pTransfer, simulator.cpp:375-418 */
-    FSIM_NUM_STAGES = 13
+    /* HashedParticles::pushParticlesApart (hashedParticles.cpp:64-107), run
+     * between ADVECT and PUSH_OUT when push_apart_enabled (simulator.cpp:61-64);
+     * a deterministic Jacobi form of the reference's racy parallel loop */
+    FSIM_STAGE_PUSH_APART = 13,
+    FSIM_NUM_STAGES = 14
 } fsim_stage;
 
 /* Timing groups, the reference's stepDuration keys (simulator.cpp:31-41). */
@@ -184,7 +196,8 @@ typedef enum fsim_field {
     FSIM_FIELD_LABEL = 6,    /* uint8 fsim_cell_type */
     FSIM_FIELD_DENSITY = 7,  /* float, avgPNum */
     FSIM_FIELD_PRESSURE = 8, /* double, 0 outside WATER cells (get only) */
-    FSIM_FIELD_RHS = 9,      /* float, b of the last solve, 0 outside WATER (get only) */
+    FSIM_FIELD_RHS = 9,      /* float, b of the last solve, 0 outside WATER (get only); after
+                                FSIM_SOLVER_BASIC_SOR, the b left on the projected faces */
     FSIM_FIELD_PRECON = 10,  /* double, MIC(0) diagonal 1/sqrt(e), 0 outside WATER (get only) */
     FSIM_FIELD_CELL_COUNT = 11, /* int32, particles per cell after FSIM_STAGE_BIN (get only) */
     FSIM_FIELD_CELL_START = 12, /* int32[Nc+1], exclusive offsets after BIN (get only) */

@@ -461,6 +461,71 @@ static void fo_push_out(fo_sim* s) {
     }
 }
 
+/* ---- HashedParticles::pushParticlesApart, hashedParticles.cpp:64-107 -----
+ * For every pair closer than the particle diameter D = 2r (and not closer
+ * than 1e-4, :88), each particle moves by half the overlap along the line
+ * between them, offset = (p_i - p_j) (D - |p_i - p_j|) / |p_i - p_j| * 0.5
+ * (:91-95), and is clamped to [cellD + 1.01 r, L - cellD - 1.01 r] (:96-101).
+ * The reference runs this as a parallel loop whose iterations move both
+ * particles of a pair while other iterations read them (a data race, so its
+ * result depends on the schedule).  This is its deterministic Jacobi form:
+ * every particle sums its own offsets from all pairs, computed from the
+ * positions before the pass (an isolated pair ends exactly D apart, as in
+ * the reference), visiting the other particles in (grid cell, id) order
+ * through the cell binning (all pairs within D lie within ceil(D / cellD)
+ * cells), then is clamped once. */
+static void fo_push_apart(fo_sim* s) {
+    const int64_t np = s->np;
+    if (np == 0) return;
+    const double D = s->pr * 2.0, D2 = D * D;
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = s->cd + s->pr * 1.01;
+        hi[a] = s->dims[a] - lo[a];
+    }
+    int32_t* count = (int32_t*)malloc(sizeof(int32_t) * (size_t)s->nc);
+    int32_t* start = (int32_t*)malloc(sizeof(int32_t) * (size_t)(s->nc + 1));
+    int64_t* perm = (int64_t*)malloc(sizeof(int64_t) * (size_t)np);
+    fo_bin(s, count, start, perm);
+    const int R = (int)ceil(D * s->cdinv);
+    double* out = (double*)malloc(sizeof(double) * 3 * (size_t)np);
+#pragma omp parallel for num_threads(fo_threads(s)) schedule(static)
+    for (int64_t i = 0; i < np; i++) {
+        const double* pi = s->p[i].pos;
+        int c[3];
+        for (int a = 0; a < 3; a++) c[a] = (int)(pi[a] * s->cdinv);
+        double q[3] = {pi[0], pi[1], pi[2]};
+        for (int x = c[0] - R; x <= c[0] + R; x++) {
+            if (x < 0 || x >= s->n[0]) continue;
+            for (int y = c[1] - R; y <= c[1] + R; y++) {
+                if (y < 0 || y >= s->n[1]) continue;
+                for (int z = c[2] - R; z <= c[2] + R; z++) {
+                    if (z < 0 || z >= s->n[2]) continue;
+                    const int64_t cell = cidx(s, x, y, z);
+                    for (int32_t k = start[cell]; k < start[cell + 1]; k++) {
+                        const int64_t j = perm[k];
+                        if (j == i) continue;
+                        const double* pj = s->p[j].pos;
+                        const double d[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+                        const double d2 = dot3(d, d);
+                        if (d2 > D2 || d2 < 1e-8) continue;
+                        const double dist = sqrt(d2);
+                        const double tmp = (D - dist) / dist;
+                        for (int a = 0; a < 3; a++) q[a] += d[a] * tmp * 0.5;
+                    }
+                }
+            }
+        }
+        for (int a = 0; a < 3; a++) out[3 * i + a] = q[a] < lo[a] ? lo[a] : (q[a] > hi[a] ? hi[a] : q[a]);
+    }
+    for (int64_t i = 0; i < np; i++)
+        for (int a = 0; a < 3; a++) s->p[i].pos[a] = out[3 * i + a];
+    free(out);
+    free(perm);
+    free(start);
+    free(count);
+}
+
 /* ---- stopParticles branch, simulator.cpp:71-74 --------------------------- */
 static void fo_stop(fo_sim* s) {
     for (int64_t i = 0; i < s->np; i++) s->p[i].v[0] = s->p[i].v[1] = s->p[i].v[2] = 0.0;
@@ -802,7 +867,79 @@ static void true_residual(fo_sim* s, const double* b, const double* pres, double
 /* BridsonSolverGrid::solveIncompressibility, bridsonSolverGrid.cpp:244-293.
  * stop_rule ABS_INF is the reference's max|r| < tol; REL_L2 (extension)
  * tests ||r||_2 <= tol * ||b||_2 at the same point of the loop. */
+/* ---- BasicMacGrid::solveIncompressibility, basicMacGrid.cpp:5-102 --------
+ * The release build's parallel branch (paralellDefine.h:5-7): per iteration
+ * one pass over the cells with x + y + z odd (z from 1 + (x+y)%2, :20), then
+ * one over the even ones (z from 2 - (x+y)%2, :62).  A WATER cell with s > 0
+ * non-SOLID neighbours moves its 6 faces by d = (-u - v - w + u(-x) + v(-y) +
+ * w(-z) [+ (avgPNum - averagePressure) pressureK]) * 1.98 / s, each face whose
+ * other cell is not SOLID (:31-55).  Cells of one colour share no face, so a
+ * pass's result does not depend on its order. */
+static void sor_pass(fo_sim* s, int odd) {
+    const double omega = 1.98; /* overRelaxation, :13 */
+    const double term_k = s->prm.pressure_k, avgp = s->prm.average_pressure;
+    const int pe = s->prm.pressure_enabled;
+#pragma omp parallel for num_threads(fo_threads(s)) schedule(static)
+    for (int x = 1; x < s->n[0] - 1; x++)
+        for (int y = 1; y < s->n[1] - 1; y++)
+            for (int z = odd ? 1 + ((x + y) % 2) : 2 - ((x + y) % 2); z < s->n[2] - 1; z += 2) {
+                const int64_t i = cidx(s, x, y, z);
+                fo_cell* c = &s->cells[i];
+                if (c->type != FSIM_WATER) continue;
+                fo_cell* xn = (fo_cell*)nbc(s, i, 0, -1);
+                fo_cell* yn = (fo_cell*)nbc(s, i, 1, -1);
+                fo_cell* zn = (fo_cell*)nbc(s, i, 2, -1);
+                const int s1 = nbc(s, i, 2, +1)->type != FSIM_SOLID, s2 = zn->type != FSIM_SOLID;
+                const int s3 = nbc(s, i, 1, +1)->type != FSIM_SOLID, s4 = yn->type != FSIM_SOLID;
+                const int s5 = nbc(s, i, 0, +1)->type != FSIM_SOLID, s6 = xn->type != FSIM_SOLID;
+                const int sum = s1 + s2 + s3 + s4 + s5 + s6;
+                if (sum == 0) continue;
+                double d = -c->f[0].v2 - c->f[1].v2 - c->f[2].v2 + xn->f[0].v2 + yn->f[1].v2 + zn->f[2].v2 +
+                           (pe ? (c->avg - avgp) * term_k : 0.0);
+                d = d * omega / sum;
+                if (s1) c->f[2].v2 += d;
+                if (s2) zn->f[2].v2 -= d;
+                if (s3) c->f[1].v2 += d;
+                if (s4) yn->f[1].v2 -= d;
+                if (s5) c->f[0].v2 += d;
+                if (s6) xn->f[0].v2 -= d;
+            }
+}
+
+static double rhs_norm(fo_sim* s, double* r) {
+    calc_rhs(s, r);
+    double ss = 0;
+    for (int64_t p = 0; p < s->nfluid; p++) ss += r[p] * r[p];
+    return sqrt(ss);
+}
+
+static int fo_basic_sor(fo_sim* s) {
+    const int64_t nf = s->nfluid;
+    double* r = (double*)malloc((size_t)(nf ? nf : 1) * sizeof(double));
+    const double b0 = rhs_norm(s, r);
+    memset(s->pressure, 0, sizeof(double) * s->nc);
+    const int iters = s->prm.max_iterations;
+    for (int n = 0; n < iters; n++) {
+        sor_pass(s, 1);
+        sor_pass(s, 0);
+    }
+    /* the right-hand side left on the projected faces */
+    const double b1 = rhs_norm(s, r);
+    memset(s->rhs, 0, sizeof(double) * s->nc);
+    for (int64_t p = 0; p < nf; p++) s->rhs[s->fluid[p]] = r[p];
+    free(r);
+    s->stats.early_exit = 0;
+    s->stats.solve_nan = 0;
+    s->stats.b_norm_l2 = b0;
+    s->stats.residual_rel_l2 = b0 > 0 ? b1 / b0 : 0.0; /* the right-hand side left over, relative */
+    s->stats.residual_abs_inf = 0;
+    s->stats.pcg_iterations = iters; /* the reference returns the iteration count, :10 */
+    s->stats.num_fluid_cells = nf;
+    return iters;
+}
+
 static int fo_solve(fo_sim* s, double dt) {
+    if (s->prm.precond == FSIM_SOLVER_BASIC_SOR) return fo_basic_sor(s);
     const int64_t nf = s->nfluid;
     size_t sz = (size_t)(nf ? nf : 1) * sizeof(double);
     double* pres = (double*)calloc(1, sz);
@@ -1010,6 +1147,7 @@ int fo_run_stage(fo_sim* s, int stage, double dt) {
     case FSIM_STAGE_SOLVE: fo_solve(s, dt); return 0;
     case FSIM_STAGE_EXTRAPOLATE: fo_extrapolate(s); return 0;
     case FSIM_STAGE_G2P: fo_g2p(s); return 0;
+    case FSIM_STAGE_PUSH_APART: fo_push_apart(s); return 0;
     default: return -1;
     }
 }
@@ -1022,6 +1160,9 @@ void fo_step(fo_sim* s, double dt, fsim_step_stats* out) {
     if (s->prm.spawning_enabled) fo_spawn(s, dt);
     fo_advect(s, dt);
     s->stats.timer_ms[FSIM_T_SIMULATE_PARTICLES] = (omp_get_wtime() - t) * 1e3;
+    t = omp_get_wtime();
+    if (s->prm.push_apart_enabled) fo_push_apart(s);
+    s->stats.timer_ms[FSIM_T_PUSH_APART] = (omp_get_wtime() - t) * 1e3;
     t = omp_get_wtime();
     fo_push_out(s);
     s->stats.timer_ms[FSIM_T_PUSH_OUT] = (omp_get_wtime() - t) * 1e3;

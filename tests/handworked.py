@@ -521,3 +521,100 @@ def gfx_snapshot_linear_density(make):
     expect_d = 2.0 + 0.5 * g[:, 0] - 0.25 * g[:, 1] + 0.125 * g[:, 2]
     return [("positions in particle order", snap[:, :3], pos), ("speeds", snap[:, 3], np.array([13.0, 5.0, 0.0, 13.0])),
             ("densities", snap[:, 4], expect_d)]
+
+
+OMEGA_SOR = 1.98   # basicMacGrid.cpp:13
+
+
+def basic_sor(make, iterations: int, pressure_enabled: int = 0):
+    """BasicMacGrid::solveIncompressibility, basicMacGrid.cpp:5-102, release
+    (parallel) branch: each iteration a pass over the WATER cells with
+    x + y + z odd, then one over the even ones; a cell with s non-SOLID
+    neighbours computes d = (-u - v - w + u(-x) + v(-y) + w(-z)
+    [+ (avgPNum - averagePressure) pressureK]) * 1.98 / s and adds d to its own
+    three faces and subtracts it from the three faces it shares with its -x,
+    -y, -z neighbours, each only where the neighbour across it is not SOLID.
+    8^3 grid, SOLID walls, faces 0 except as listed.
+
+    Cell A (3,3,3), isolated WATER (all 6 neighbours AIR), odd.  Its faces:
+    u(3,3,3) = 0.3, v(3,3,3) = -0.2, w(3,3,3) = 0.1, u(2,3,3) = -0.5,
+    v(3,2,3) = 0.25, w(3,3,2) = 0.05: the bracket D0 = -0.4 (+ the density
+    term T).  Each update changes the bracket by -6d = -1.98 D, so after k
+    iterations the own faces have gained 0.33 D0 (1 - (-0.98)^k) / 1.98 and
+    the -side faces lost as much.
+
+    Cells B (2,5,5), even, and C (3,5,5), odd, WATER side by side; their
+    shared face u(2,5,5) = 0.6 and u(3,5,5) = 0.2.  The odd pass goes first:
+    C sees -0.2 + 0.6 = 0.4, d_C = 0.132 (u(3,5,5) 0.332, v, w of C 0.132,
+    shared face 0.468, v(3,4,5) and w(3,5,4) -0.132); then B sees -0.468,
+    d_B = -0.15444 (shared face 0.31356, v, w of B -0.15444, u(1,5,5),
+    v(2,4,5), w(2,5,4) 0.15444).  (A lexicographic sweep would update B
+    first.)  One iteration for this pair.
+
+    Cell W (1,2,6), odd, WATER next to the x = 0 and z = 7 walls: s = 4,
+    u(1,2,6) = 0.5 gives d = -0.5 * 1.98 / 4 = -0.2475; its faces on the
+    walls, u(0,2,6) and w(1,2,6), stay 0."""
+    n = 8
+    grid = scenes.grid(n)
+    kw = dict(precond=capi.SOLVER_BASIC_SOR, max_iterations=iterations, pressure_enabled=pressure_enabled,
+              pressure_k=2.0, average_pressure=2.0)
+    b = make(grid, **kw)
+    lab = np.full((n, n, n), capi.AIR, np.uint8)
+    lab[0], lab[-1], lab[:, 0], lab[:, -1], lab[:, :, 0], lab[:, :, -1] = (capi.SOLID,) * 6
+    for c in ((3, 3, 3), (2, 5, 5), (3, 5, 5), (1, 2, 6)):
+        lab[c] = capi.WATER
+    b.set_grid(capi.FIELD_LABEL, lab.reshape(-1))
+    f = [np.zeros((n, n, n)) for _ in range(3)]
+    f[0][3, 3, 3], f[1][3, 3, 3], f[2][3, 3, 3] = 0.3, -0.2, 0.1
+    f[0][2, 3, 3], f[1][3, 2, 3], f[2][3, 3, 2] = -0.5, 0.25, 0.05
+    f[0][2, 5, 5], f[0][3, 5, 5] = 0.6, 0.2
+    f[0][1, 2, 6] = 0.5
+    for fld, v in zip(F2, f):
+        b.set_grid(fld, v.reshape(-1))
+    dens = np.zeros((n, n, n))
+    dens[3, 3, 3] = 2.5   # T = (2.5 - 2) * 2 = 1 at A when the density term is on; B, C, W: (0 - 2) * 2 = -4
+    b.set_grid(capi.FIELD_DENSITY, dens.reshape(-1))
+    b.run_stage(capi.STAGE_SOLVE, scenes.DT)
+    out = [b.get_grid(fld).reshape(n, n, n) for fld in F2]
+    res = [("iterations reported", b.stats().pcg_iterations, iterations)]
+    # A: closed form
+    d0 = -0.4 + (1.0 if pressure_enabled else 0.0)
+    gain = 0.33 * d0 * (1.0 - (-0.98) ** iterations) / 1.98
+    res += [("A own faces", [out[0][3, 3, 3], out[1][3, 3, 3], out[2][3, 3, 3]], [0.3 + gain, -0.2 + gain, 0.1 + gain]),
+            ("A -side faces", [out[0][2, 3, 3], out[1][3, 2, 3], out[2][3, 3, 2]],
+             [-0.5 - gain, 0.25 - gain, 0.05 - gain])]
+    if iterations == 1 and not pressure_enabled:
+        res += [("C own faces", [out[0][3, 5, 5], out[1][3, 5, 5], out[2][3, 5, 5]], [0.332, 0.132, 0.132]),
+                ("C -y, -z faces", [out[1][3, 4, 5], out[2][3, 5, 4]], [-0.132, -0.132]),
+                ("shared face", out[0][2, 5, 5], 0.468 - 0.15444),
+                ("B v, w", [out[1][2, 5, 5], out[2][2, 5, 5]], [-0.15444, -0.15444]),
+                ("B -side faces", [out[0][1, 5, 5], out[1][2, 4, 5], out[2][2, 5, 4]], [0.15444, 0.15444, 0.15444]),
+                ("W own faces", [out[0][1, 2, 6], out[1][1, 2, 6], out[2][1, 2, 6]], [0.2525, -0.2475, 0.0]),
+                ("W -side faces", [out[0][0, 2, 6], out[1][1, 1, 6], out[2][1, 2, 5]], [0.0, 0.2475, 0.2475])]
+    return [(what, np.asarray(got, np.float64), np.asarray(exp, np.float64)) for what, got, exp in res]
+
+
+def push_apart_pair(make):
+    """pushParticlesApart, hashedParticles.cpp:64-107, on a 16^3 grid
+    (dx = 0.0625, r = 0.015625, diameter D = 0.03125):
+      P0 (0.5, 0.5, 0.5) and P1 (0.52, 0.5, 0.5), 0.02 apart: each moves half
+        the overlap, 0.005625, away from the other (:91-95), so they end
+        D apart with the midpoint kept: x = 0.494375 and 0.525625;
+      P2 (0.3, 0.3, 0.3) and P3 (0.3, 0.34, 0.3), 0.04 > D apart: unchanged
+        (:88);
+      P4 and P5 at (0.7, 0.7, 0.7), 0.00005 apart (closer than 1e-4): left as
+        they are (:88)."""
+    grid = scenes.grid(16)
+    b = make(grid, push_apart_enabled=1)
+    p0 = np.array([[0.5, 0.5, 0.5], [0.52, 0.5, 0.5], [0.3, 0.3, 0.3], [0.3, 0.34, 0.3], [0.7, 0.7, 0.7],
+                   [0.70005, 0.7, 0.7]]).astype(np.float32).astype(np.float64)
+    b.set_particles(p0, np.zeros_like(p0))
+    b.run_stage(capi.STAGE_PUSH_APART)
+    p, _, _ = b.get_particles()
+    half = (2 * grid.particle_r - (p0[1, 0] - p0[0, 0])) / 2
+    expect = p0.copy()
+    expect[0, 0] -= half
+    expect[1, 0] += half
+    return [("pair pushed to one diameter", p[:2], expect[:2]), ("pair further than D", p[2:4], p0[2:4]),
+            ("coincident pair", p[4:], p0[4:]),
+            ("distance after", float(np.linalg.norm(p[1] - p[0])), 2 * grid.particle_r)]

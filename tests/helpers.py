@@ -67,3 +67,95 @@ def dam_state(n: int = 16, transfer=capi.FLIP, steps: int = 2, sphere: bool = Fa
     pos, vel, c = sim.get_particles()
     sim.close()
     return sc, prm, pos, vel, c
+
+
+F_FACES = (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W)
+F_FACES2 = (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2)
+
+
+def copy_grid_to_oracle(sim: capi.Simulation, ref, fields) -> None:
+    """The GPU's FP32 grid fields, exactly, into the oracle."""
+    for f in fields:
+        ref.set_grid(f, sim.get_grid(f).astype(np.float64))
+
+
+def staged_step(sim: capi.Simulation, ref, dt: float, spawn: bool = False, stop: bool = False) -> dict:
+    """One step of the reference's stage order (simulator.cpp:51-100) on both
+    backends, stage by stage, each stage started from identical inputs: after
+    every stage the oracle takes the GPU's FP32 result (particles or grid
+    fields).  So each stage's error is its own, not carried over from the
+    FP32 rounding of the stages before it.  Returns, per stage, the largest
+    relative error of its outputs (max |gpu - oracle| / max |oracle|), the
+    label and binning mismatches (counts of differing entries) and the
+    solver statistics; the GPU state is that of its own step."""
+    e = {}
+    sync_oracle_from_gpu(sim, ref)
+    if spawn:
+        sim.run_stage(capi.STAGE_SPAWN, dt)
+        ref.run_stage(capi.STAGE_SPAWN, dt)
+        e["spawn_count_diff"] = abs(sim.num_particles() - ref.num_particles())
+        p_g, v_g, _ = sim.get_particles()
+        p_r, v_r, _ = ref.get_particles()
+        e["spawn_pos_abs"] = float(np.abs(p_g - p_r).max()) if len(p_g) else 0.0
+        sync_oracle_from_gpu(sim, ref)
+    for stage, name in ((capi.STAGE_ADVECT, "advect"), (capi.STAGE_PUSH_OUT, "push_out")):
+        sim.run_stage(stage, dt)
+        ref.run_stage(stage, dt)
+        sim.run_stage(capi.STAGE_BIN, dt)   # drops the particles a sink removed, as the oracle's advection does
+        p_g, v_g, _ = sim.get_particles()
+        p_r, v_r, _ = ref.get_particles()
+        e[f"{name}_count_diff"] = abs(len(p_g) - len(p_r))
+        if len(p_g) == len(p_r) and len(p_g):
+            e[f"{name}_pos_abs"] = float(np.abs(p_g - p_r).max())   # domain [0, 1]: absolute = relative
+            e[f"{name}_vel"] = field_rel_err(v_g, v_r)
+        sync_oracle_from_gpu(sim, ref)
+    if stop:
+        sim.run_stage(capi.STAGE_STOP, dt)
+        ref.run_stage(capi.STAGE_STOP, dt)
+    # binning + P2G (+ density) + marking from the same FP32 positions
+    sim.run_stage(capi.STAGE_P2G, dt)
+    ref.run_stage(capi.STAGE_P2G, dt)
+    cnt_r, start_r, perm_r = ref.bin()
+    e["bin_count_mismatch"] = int(np.count_nonzero(sim.get_grid(capi.FIELD_CELL_COUNT) != cnt_r))
+    e["bin_start_mismatch"] = int(np.count_nonzero(sim.get_grid(capi.FIELD_CELL_START) != start_r))
+    e["bin_perm_mismatch"] = int(np.count_nonzero(sim.slot_ids() != perm_r))
+    e["p2g"] = max(field_rel_err(sim.get_grid(f), ref.get_grid(f)) for f in F_FACES)
+    e["density"] = field_rel_err(sim.get_grid(capi.FIELD_DENSITY), ref.get_grid(capi.FIELD_DENSITY))
+    sim.run_stage(capi.STAGE_MARK, dt)
+    ref.run_stage(capi.STAGE_MARK, dt)
+    e["mark_label_mismatch"] = int(np.count_nonzero(sim.get_grid(capi.FIELD_LABEL) != ref.get_grid(capi.FIELD_LABEL)))
+    copy_grid_to_oracle(sim, ref, F_FACES + (capi.FIELD_DENSITY,))
+    # obstacles, borders, v2 + gravity
+    for stage in (capi.STAGE_OBSTACLES, capi.STAGE_BORDERS, capi.STAGE_POST_P2G):
+        sim.run_stage(stage, dt)
+        ref.run_stage(stage, dt)
+    e["prep_label_mismatch"] = int(np.count_nonzero(sim.get_grid(capi.FIELD_LABEL) != ref.get_grid(capi.FIELD_LABEL)))
+    e["prep"] = max(field_rel_err(sim.get_grid(f), ref.get_grid(f)) for f in F_FACES + F_FACES2)
+    copy_grid_to_oracle(sim, ref, F_FACES + F_FACES2)
+    # pressure solve from the same faces
+    sim.run_stage(capi.STAGE_SOLVE, dt)
+    ref.run_stage(capi.STAGE_SOLVE, dt)
+    e["rhs"] = field_rel_err(sim.get_grid(capi.FIELD_RHS), ref.get_grid(capi.FIELD_RHS))
+    e["pressure"] = field_rel_err(sim.get_grid(capi.FIELD_PRESSURE), ref.get_grid(capi.FIELD_PRESSURE))
+    e["projected"] = max(field_rel_err(sim.get_grid(f), ref.get_grid(f)) for f in F_FACES2)
+    st_g, st_r = sim.stats(), ref.stats()
+    e["pcg_gpu"], e["pcg_oracle"] = st_g.pcg_iterations, st_r.pcg_iterations
+    e["residual_gpu"], e["residual_oracle"] = st_g.residual_rel_l2, st_r.residual_rel_l2
+    e["b_norm"] = st_g.b_norm_l2
+    copy_grid_to_oracle(sim, ref, F_FACES2)
+    sim.run_stage(capi.STAGE_EXTRAPOLATE, dt)
+    ref.run_stage(capi.STAGE_EXTRAPOLATE, dt)
+    e["extrapolate"] = max(field_rel_err(sim.get_grid(f), ref.get_grid(f)) for f in F_FACES2)
+    copy_grid_to_oracle(sim, ref, F_FACES2)
+    sim.run_stage(capi.STAGE_G2P, dt)
+    ref.run_stage(capi.STAGE_G2P, dt)
+    p_g, v_g, c_g = sim.get_particles()
+    p_r, v_r, c_r = ref.get_particles()
+    e["g2p_vel"] = field_rel_err(v_g, v_r)
+    e["g2p_c"] = field_rel_err(c_g, c_r) if np.abs(c_r).max() > 0 else 0.0
+    return e
+
+
+STAGE_FIELD_KEYS = ("p2g", "density", "prep", "rhs", "pressure", "projected", "extrapolate", "g2p_vel")
+STAGE_EXACT_KEYS = ("bin_count_mismatch", "bin_start_mismatch", "bin_perm_mismatch", "mark_label_mismatch",
+                    "prep_label_mismatch", "advect_count_diff", "push_out_count_diff")

@@ -109,3 +109,14 @@ def test_solve_nan_break_hand_worked(gpu):
 def test_gfx_snapshot_linear_density_hand_worked(gpu):
     say("case: gfx snapshot of a linear density field, 16^3")
     _check(handworked.gfx_snapshot_linear_density(make), 4e-6, 4e-6)
+
+
+@pytest.mark.parametrize("iterations,pressure_enabled", [(1, 0), (5, 0), (4, 1)])
+def test_basic_sor_hand_worked(gpu, iterations, pressure_enabled):
+    say(f"case: BasicMacGrid red-black SOR, {iterations} iterations, density term {pressure_enabled}, 8^3")
+    _check(handworked.basic_sor(make, iterations, pressure_enabled), 2.4e-7, 1e-7)
+
+
+def test_push_apart_pair_hand_worked(gpu):
+    say("case: push-apart of a pair, 16^3")
+    _check(handworked.push_apart_pair(make), 0.0, 1e-7)

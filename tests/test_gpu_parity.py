@@ -446,3 +446,79 @@ def test_gfx_snapshot_parity(gpu):
     assert field_rel_err(g["v"], r[:, 3]) < 1e-6
     assert field_rel_err(g["density"], r[:, 4]) < 1e-5
     assert np.all(r[:, 4] > 0)
+
+
+@pytest.mark.parametrize("iterations", [80, 400])
+def test_basic_sor_parity(gpu, iterations):
+    """BasicMacGrid's red-black SOR (basicMacGrid.cpp:15-102) from identical faces: the GPU sweeps in
+    FP64 in the reference's order, so the projected faces are the oracle's rounded to FP32."""
+    say(f"case: BasicMacGrid SOR 16^3 dam break with a sphere, {iterations} iterations")
+    sc, prm, sim, ref = _prepared_pair()
+    p = scenes.params(precond=capi.SOLVER_BASIC_SOR, max_iterations=iterations)
+    sim.set_params(p)
+    ref.set_params(p)
+    for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        v = ref.get_grid(f).astype(np.float32)
+        sim.set_grid(f, v)
+        ref.set_grid(f, v.astype(np.float64))
+    sim.run_stage(capi.STAGE_SOLVE, DT)
+    ref.run_stage(capi.STAGE_SOLVE, DT)
+    for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        g, r = sim.get_grid(f), ref.get_grid(f)
+        assert ulp_mismatch(g, r, 0.5) == 0, f
+    st_g, st_r = sim.stats(), ref.stats()
+    assert st_g.pcg_iterations == st_r.pcg_iterations == iterations
+    assert abs(st_g.residual_rel_l2 - st_r.residual_rel_l2) <= 1e-5 * max(st_r.residual_rel_l2, 1e-6)
+    assert st_g.residual_rel_l2 < 1.0
+
+
+def test_basic_sor_whole_steps(gpu):
+    say("case: 3 whole steps with the BasicMacGrid solver, 24^3 dam break, GPU vs oracle from the same state")
+    sc = scenes.dam_break(24, capi.FLIP, sphere=True)
+    prm = scenes.params(precond=capi.SOLVER_BASIC_SOR, max_iterations=80)
+    sim, ref = pair(sc.grid, prm, sc.particles(), obstacles=sc.obstacles)
+    for k in range(3):
+        sync_oracle_from_gpu(sim, ref)
+        sim.step(DT)
+        ref.step(DT)
+        assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL)), k
+        for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+            assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-4, (k, f)
+        p_gpu, v_gpu, _ = sim.get_particles()
+        p_ref, v_ref, _ = ref.get_particles()
+        assert np.abs(p_gpu - p_ref).max() < 1e-5, k
+
+
+@pytest.mark.parametrize("r_cells", [0.25, 0.8])
+def test_push_apart_parity(gpu, r_cells):
+    """The Jacobi push-apart pass from the same FP32 positions: the GPU visits the same pairs in the
+    same (cell, id) order with FP64 sums, so it gives the oracle's positions rounded to FP32."""
+    n = 16
+    grid = capi.Grid(n, n, n, 1.0 / n, r_cells / n)
+    say(f"case: push-apart 16^3, r = {r_cells} dx, 30000 random particles")
+    pos, vel, _ = random_particles(grid, 30000, seed=21)
+    sim, ref = pair(grid, scenes.params(push_apart_enabled=1), pos, vel)
+    sim.run_stage(capi.STAGE_PUSH_APART)
+    ref.run_stage(capi.STAGE_PUSH_APART)
+    p_gpu, v_gpu, _ = sim.get_particles()
+    p_ref, _, _ = ref.get_particles()
+    assert ulp_mismatch(p_gpu, p_ref, 0.5) == 0
+    assert np.abs(p_gpu - pos).max() > 1e-3   # the pass moved particles
+    assert np.array_equal(v_gpu, vel)
+
+
+def test_whole_step_push_apart_parity(gpu):
+    say("case: 3 whole steps with push-apart, 24^3 dam break, GPU vs oracle from the same state")
+    sc = scenes.dam_break(24, capi.FLIP, sphere=True)
+    prm = scenes.params(push_apart_enabled=1, tol=1e-10)
+    sim, ref = pair(sc.grid, prm, sc.particles(), obstacles=sc.obstacles)
+    for k in range(3):
+        sync_oracle_from_gpu(sim, ref)
+        st = sim.step(DT)
+        ref.step(DT)
+        assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL)), k
+        p_gpu, v_gpu, _ = sim.get_particles()
+        p_ref, v_ref, _ = ref.get_particles()
+        assert np.abs(p_gpu - p_ref).max() < 1e-5, k
+        assert field_rel_err(v_gpu, v_ref) < 1e-4, k
+        assert st.timer_ms[capi.TIMER_KEYS.index("PushParticlesApart")] > 0

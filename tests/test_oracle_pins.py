@@ -475,3 +475,57 @@ def test_solve_nan_break_hand_worked():
 def test_gfx_snapshot_linear_density_hand_worked():
     # the density is summed in FP32 as in the reference (float density += double)
     _check(handworked.gfx_snapshot_linear_density(mk), 4e-6)
+
+
+@pytest.mark.parametrize("iterations,pressure_enabled", [(1, 0), (5, 0), (4, 1)])
+def test_basic_sor_hand_worked(iterations, pressure_enabled):
+    _check(handworked.basic_sor(mk, iterations, pressure_enabled), 1e-12)
+
+
+def test_basic_sor_converges_to_a_divergence_free_field():
+    """Red-black SOR is a convergent iteration on the SPD pressure system (omega < 2): after enough
+    iterations the right-hand side left on the faces is zero, and the faces equal the PCG projection's
+    up to the solve tolerance (both remove the same gradient component)."""
+    grid, o = _dam_prepared(n=12)
+    _, o2 = _dam_prepared(n=12)
+    o.set_params(scenes.params(precond=capi.SOLVER_BASIC_SOR, max_iterations=3000))
+    o.run_stage(capi.STAGE_SOLVE, DT)
+    st = o.stats()
+    assert st.pcg_iterations == 3000 and st.b_norm_l2 > 1.0
+    assert st.residual_rel_l2 < 1e-10
+    o2.set_params(scenes.params(tol=1e-12))
+    o2.run_stage(capi.STAGE_SOLVE, DT)
+    for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
+        a, b = o.get_grid(f), o2.get_grid(f)
+        assert np.abs(a - b).max() < 1e-8 * np.abs(b).max()
+
+
+def test_push_apart_pair_hand_worked():
+    _check(handworked.push_apart_pair(mk), 1e-15)
+
+
+@pytest.mark.parametrize("r_cells", [0.25, 0.8])
+def test_push_apart_equals_all_pairs_brute_force(r_cells):
+    """The binned pass finds every pair closer than D (also when D spans more than one cell,
+    r = 0.8 dx: two cells of reach) and applies each particle's offsets from the positions before
+    the pass: the same as an all-pairs O(n^2) sum, then the clamp."""
+    n = 16
+    grid = capi.Grid(n, n, n, 1.0 / n, r_cells / n)
+    rng = np.random.default_rng(7)
+    pos = rng.uniform(0.3, 0.45, (1500, 3))
+    pos[:20] = rng.uniform(0.0, 1.0, (20, 3)) * 0.05 + grid.cell_d   # some near the clamp box edge
+    o = oracle.Oracle(grid, scenes.params(push_apart_enabled=1))
+    o.set_particles(pos, np.zeros_like(pos))
+    o.run_stage(capi.STAGE_PUSH_APART)
+    got, _, _ = o.get_particles()
+    D = 2 * grid.particle_r
+    d = pos[:, None, :] - pos[None, :, :]
+    d2 = (d ** 2).sum(-1)
+    near = (d2 <= D * D) & (d2 >= 1e-8)
+    np.fill_diagonal(near, False)
+    dist = np.sqrt(np.where(near, d2, 1.0))
+    off = np.where(near[..., None], d * ((D - dist) / dist)[..., None] * 0.5, 0.0).sum(1)
+    lo = grid.cell_d + 1.01 * grid.particle_r
+    expect = np.clip(pos + off, lo, 1.0 - lo)
+    assert near.sum() > 1000
+    np.testing.assert_allclose(got, expect, rtol=0, atol=1e-13)   # summation order only
