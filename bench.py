@@ -73,11 +73,35 @@ def divergence_l2(sim, grid) -> float:
     return float(np.sqrt(np.sum(np.where(lab == capi.WATER, d, 0.0) ** 2)))
 
 
+def host_info() -> dict:
+    """nproc, the CPUs this process may run on, and the CPU model (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith("Model name")), None)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            model = next((ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")),
+                         None)
+        except OSError:
+            pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def cpu_threads_default() -> int:
+    """All CPUs this process may run on, at most 16: the host CPU share that goes with one GPU."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 def cpu_baseline(scene, threads: int) -> tuple[dict, dict]:
     """The FP64 oracle (the reference's cost structure) runs one full step of the scene from its initial
-    state, PCG to the same 1e-6 relative residual, timed on `threads` host threads.  The GPU runs the
-    same step on a fresh context, and the two are compared: labels, pressure, post-projection divergence,
-    particle positions and velocities."""
+    state, PCG to the same 1e-6 relative residual, timed on `threads` host threads: one step, not a
+    median (a 256^3 step is about 25 s of CPU work on 16 threads, the bound of the sample).  The GPU runs
+    the same step on a fresh context, and the two are compared: labels, pressure, post-projection
+    divergence, particle positions and velocities."""
     import oracle
 
     pos = scene.particles()
@@ -112,11 +136,53 @@ def cpu_baseline(scene, threads: int) -> tuple[dict, dict]:
            "position_drift_max": float(np.abs(g["pos"] - r["pos"]).max()),
            "position_drift_max_cells": float(np.abs(g["pos"] - r["pos"]).max() / scene.grid.cell_d),
            "velocity_rel_l2": float(np.linalg.norm(g["vel"] - r["vel"]) / max(np.linalg.norm(r["vel"]), 1e-300))}
+    host = host_info()
     base = {"value": 1.0 / wall, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle (clean-room restatement with the reference's cost structure), one full step "
-                       f"of the {scene.n}^3 scene from its initial state: {wall:.1f} s wall with "
-                       f"{st_o.pcg_iterations} PCG iterations to 1e-6 on {threads} threads")}
+            "sample": (f"FP64 oracle (clean-room restatement with the reference's cost structure), ONE full step "
+                       f"of the {scene.n}^3 scene from its initial state (not a median): {wall:.1f} s wall with "
+                       f"{st_o.pcg_iterations} PCG iterations to 1e-6 on {threads} OpenMP threads"),
+            **host}
     return base, acc
+
+
+def error_trace(steps: int, threads: int) -> dict:
+    """The 128^3 APIC dam break with a static sphere run `steps` steps on the GPU and on the FP64 oracle from
+    the same initial particles, with no re-synchronisation: per step, each side's relative pressure
+    residual ||r||_2/||b||_2 and post-projection divergence L2 / ||b||_2, and the particle-position drift
+    between the two (max and RMS, in cells)."""
+    import oracle
+
+    sc = scenes.CONFIG_128_APIC_SPHERE()
+    prm = scenes.params(transfer=sc.transfer)
+    pos = sc.particles()
+    sim = capi.Simulation(sc.grid, prm)
+    o = oracle.Oracle(sc.grid, prm, threads=threads)
+    for side in (sim, o):
+        if sc.obstacles:
+            side.set_obstacles(sc.obstacles)
+    sim.set_particles(pos)
+    o.set_particles(pos.astype(np.float64))
+    del pos
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st_g, st_o = sim.step(scenes.DT), o.step(scenes.DT)
+        pg, _, _ = sim.get_particles()
+        po, _, _ = o.get_particles()
+        d = np.abs(pg.astype(np.float64) - po) / sc.grid.cell_d
+        rows.append({"step": k + 1,
+                     "residual_rel_l2": [st_g.residual_rel_l2, st_o.residual_rel_l2],
+                     "divergence_l2_over_b": [divergence_l2(sim, sc.grid) / max(st_g.b_norm_l2, 1e-300),
+                                              divergence_l2(o, sc.grid) / max(st_o.b_norm_l2, 1e-300)],
+                     "drift_max_cells": float(d.max()), "drift_rms_cells": float(np.sqrt(np.mean(d ** 2)))})
+        print(f"# error trace step {k + 1}: {rows[-1]}", file=sys.stderr, flush=True)
+    sim.close()
+    o.close()
+    return {"workload": "dam break 128^3 APIC with a static sphere, PCG to 1e-6, GPU and FP64 oracle from the "
+                        "same initial particles, never re-synchronised",
+            "steps": steps, "oracle_threads": threads, "wall_s": round(time.perf_counter() - t0, 1),
+            "columns": "residual_rel_l2 and divergence_l2_over_b are [gpu, oracle]; drift is GPU vs oracle",
+            "trace": rows}
 
 
 COPY_BYTES = 1 << 30   # bytes read (and written) by each calibration copy
@@ -171,13 +237,16 @@ def _state_after(sc, steps: int):
     return state
 
 
-def _fpcg_iterations(sc, state, params, budget: int):
-    """FPCG iterations of one step from `state` on a fresh context (None: not converged within `budget`)."""
+def _fpcg_iterations(sc, state, params, budget: int, coarse: int = capi.MG_COARSE_NETWORK):
+    """FPCG iterations of one step from `state` on a fresh context (None: not converged within `budget`);
+    coarse: how the multigrid's coarsest level is solved (FSIM_OPT_MG_COARSE)."""
     sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED,
                                                  max_iterations=budget))
     if sc.obstacles:
         sim.set_obstacles(sc.obstacles)
-    sim.learned_set_params(params)
+    if params is not None:
+        sim.learned_set_params(params)
+    sim.set_option(capi.OPT_MG_COARSE, coarse)
     sim.set_option(capi.OPT_LEARNED_MIC_COUNT, max(budget // 2, 1))   # FPCG budget 2 x this count
     sim.set_particles(*state)
     st = sim.step(scenes.DT)
@@ -195,69 +264,70 @@ def _compare_scene(n: int):
     return scenes.dam_break(n, capi.APIC, seed=0, sphere=True), f"dam break {n}^3 APIC with a static sphere"
 
 
-def learned_solve_compare(sizes, train_n: int, train_scenes: int, epochs: int, seeds: int = 3) -> dict:
-    """Pressure-solve ms to 1e-6 relative residual, MIC(0)-PCG vs the learned FPCG, on the same right-hand side.
+def learned_solve_compare(sizes, train_n: int, train_scenes: int, epochs: int, seeds: int = 2) -> dict:
+    """Pressure-solve ms to 1e-6 relative residual, MIC(0)-PCG vs the learned multigrid FPCG, on the same
+    right-hand side.
 
-    The network is trained here on the GPU with the energy-norm loss (generate_samples + train: real
-    MIC(0)-PCG solves of seeded random dam breaks at train_n^3 and residual pairs) from `seeds` initial
-    seeds; each is validated by its FPCG iteration count on a 64^3 double dam break (another size and
-    scene than the tests), and the best is set on a context of each size n in `sizes` (model
-    selection: a trained network can stall FPCG, and the validation solve shows it).  Each n^3 scene
-    (_compare_scene) is stepped 3 times with MIC(0)-PCG, its particle state is saved, and the next
-    step is solved from that same state with each preconditioner (the second of two runs is timed, so
-    nothing is allocated in it).  `learned` is the production path: FPCG with the MIC(0)-PCG fallback
-    after twice the last MIC(0) iteration count.  `fpcg_iterations` compares the FPCG iteration
-    counts, on fresh contexts with a budget of 10x the MIC(0) count, of the selected network, an
-    untrained (initial) network and an all-zero network (the same solver with its damped-Jacobi steps
-    alone)."""
+    The learned preconditioner is a multigrid V-cycle whose coarsest level (16^3 here) is solved by the
+    network.  The network is trained here, on the GPU: learned.harvest_samples steps seeded random dam
+    breaks at train_n^3 and records the coarsest-level problems of accurate multigrid solves;
+    learned.train fits the network to them from `seeds` initial seeds.  Each candidate is validated by its
+    FPCG iteration count on a 64^3 FLIP double dam break (another scene than the training's) and the best
+    is set on a context of each size n in `sizes`.  Each n^3 scene (_compare_scene) is stepped 3 times
+    with MIC(0)-PCG, its particle state is saved, and the next step is solved from that same state with
+    each preconditioner (the second of two runs is timed, so nothing is allocated in it).  `learned` is
+    the production path: FPCG with the MIC(0)-PCG fallback after twice the last MIC(0) iteration count;
+    `mg_smooth` is the same V-cycle with the coarsest level smoothed instead (no network).
+    `fpcg_iterations` compares FPCG iteration counts on fresh contexts (budget 10x the MIC(0) count) for
+    the coarsest level solved by the trained network, by an untrained network, by smoothing alone and
+    accurately (600 sweeps: the bound a perfect network would reach)."""
     t0 = time.perf_counter()
-    tgrid, B, L, P = learned.generate_samples(train_n, train_scenes, 4, seed=0, residuals_per_solve=4)
+    B, OP, X, dims = learned.harvest_samples(train_n, train_scenes, 3, per_solve=12, seed=0)
     t_gen = time.perf_counter() - t0
+    print(f"# learned: {len(B)} coarse samples {dims} from {train_scenes} scenes at {train_n}^3 in {t_gen:.1f}s",
+          file=sys.stderr, flush=True)
     val_sc = scenes.double_dam_break(64, capi.FLIP, seed=2)
     val_state = _state_after(val_sc, 3)
     cands = []
     t0 = time.perf_counter()
     for seed in range(1, seeds + 1):
-        init = capi.Simulation(tgrid, scenes.params())
-        init.learned_init(seed)
-        p0 = init.learned_get_params()
-        init.close()
         t1 = time.perf_counter()
-        prm, hist = learned.train(tgrid, B, L, P, epochs=epochs, batch=4, lr=1e-3, seed=seed, params=p0,
-                                  loss=learned.LOSS_ENERGY)
+        prm, hist = learned.train(dims, B, OP, X, epochs=epochs, batch=8, lr=1e-3, seed=seed)
         t_tr = time.perf_counter() - t1
-        its = _fpcg_iterations(val_sc, val_state, prm, 400)
-        cands.append({"seed": seed, "params": prm, "init": p0, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+        its = _fpcg_iterations(val_sc, val_state, prm, 200)
+        cands.append({"seed": seed, "params": prm, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
                       "train_s": round(t_tr, 2), "validation_iterations": its})
         print(f"# learned seed {seed}: loss {hist[0]:.3e} -> {hist[-1]:.3e} in {t_tr:.1f}s, "
               f"validation FPCG {its} it", file=sys.stderr, flush=True)
     t_train = time.perf_counter() - t0
     ok = [c for c in cands if c["validation_iterations"] is not None]
     best = min(ok, key=lambda c: c["validation_iterations"]) if ok else cands[0]
-    params, untrained = best["params"], cands[0]["init"]
-    print(f"# learned: {len(B)} samples at {train_n}^3 in {t_gen:.1f}s, {seeds} x {epochs} epochs in {t_train:.1f}s, "
-          f"selected seed {best['seed']}", file=sys.stderr, flush=True)
-    training = {"samples": len(B), "grid": [train_n] * 3, "epochs": epochs, "loss": "energy norm",
-                "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
-                "validation": "FPCG iterations (budget 400) of the 4th step of a 64^3 FLIP double dam break",
+    init = capi.Simulation(capi.Grid(*dims, 1.0 / dims[0], 0.25 / dims[0]), scenes.params())
+    init.learned_init(best["seed"])
+    untrained = init.learned_get_params()
+    init.close()
+    training = {"samples": len(B), "coarse_grid": list(dims), "scene_grid": [train_n] * 3, "scenes": train_scenes,
+                "epochs": epochs, "loss": "masked MSE against the accurate coarse solution",
+                "harvest_s": round(t_gen, 2), "train_s": round(t_train, 2),
+                "validation": "FPCG iterations (budget 200) of the 4th step of a 64^3 FLIP double dam break",
                 "selected_seed": best["seed"],
-                "candidates": [{k: v for k, v in c.items() if k not in ("params", "init")} for c in cands]}
-    return {"training": training, "grids": [_compare_at(n, params, untrained) for n in sizes]}
+                "candidates": [{k: v for k, v in c.items() if k != "params"} for c in cands]}
+    return {"training": training, "grids": [_compare_at(n, best["params"], untrained) for n in sizes]}
 
 
 def _compare_at(n: int, params, untrained) -> dict:
-    """MIC(0)-PCG vs learned FPCG on the n^3 comparison scene (learned_solve_compare)."""
+    """MIC(0)-PCG vs learned multigrid FPCG on the n^3 comparison scene (learned_solve_compare)."""
     sc, what = _compare_scene(n)
     state = _state_after(sc, 3)
     sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
     if sc.obstacles:
         sim.set_obstacles(sc.obstacles)
+    sim.learned_set_params(params)
     res = {}
 
-    def solve(name, precond, prm=None, profile=False):
+    def solve(name, precond, coarse=capi.MG_COARSE_NETWORK, profile=False):
         sim.set_params(scenes.params(transfer=sc.transfer, precond=precond))
-        if prm is not None:
-            sim.learned_set_params(prm)
+        sim.set_option(capi.OPT_MG_COARSE, coarse)
         for _ in range(2):
             sim.set_particles(*state)
             st = sim.step(scenes.DT)
@@ -267,7 +337,7 @@ def _compare_at(n: int, params, untrained) -> dict:
         print(f"# {n}^3 {name}: {res[name]}", file=sys.stderr, flush=True)
         if not profile:
             return None
-        # a third run with per-kernel HIP events, for the network's time per application
+        # a third run with per-kernel HIP events: the V-cycle's and the network's time per application
         sim.set_particles(*state)
         sim.set_option(capi.OPT_PROFILE, 1)
         sim.reset_kernel_times()
@@ -276,26 +346,35 @@ def _compare_at(n: int, params, untrained) -> dict:
         return sim.kernel_times()
 
     solve("mic0_pcg", capi.PRECOND_MIC0)
-    kt = solve("learned", capi.PRECOND_LEARNED, params, profile=True)
+    kt = solve("learned", capi.PRECOND_LEARNED, profile=True)
+    solve("mg_smooth", capi.PRECOND_LEARNED, capi.MG_COARSE_SMOOTH)
     k_mic = res["mic0_pcg"]["pcg_iterations"]
+    levels, cdims = sim.mg_info()
     sim.close()
-    iters = {name: _fpcg_iterations(sc, state, prm, 10 * k_mic)
-             for name, prm in (("trained", params), ("untrained", untrained), ("zero_network", np.zeros_like(params)))}
-    iters["budget"] = 10 * k_mic
-    iters["mic0_pcg"] = k_mic
+    budget = 10 * k_mic
+    iters = {"network_trained": _fpcg_iterations(sc, state, params, budget),
+             "network_untrained": _fpcg_iterations(sc, state, untrained, budget),
+             "smooth": _fpcg_iterations(sc, state, None, budget, capi.MG_COARSE_SMOOTH),
+             "accurate": _fpcg_iterations(sc, state, None, budget, capi.MG_COARSE_ACCURATE),
+             "budget": budget, "mic0_pcg": k_mic}
     print(f"# {n}^3 FPCG iterations (None: not converged within the budget): {iters}", file=sys.stderr, flush=True)
     net_ms, net_calls = kt.get("learned", (0.0, 0))
+    mg_ms, mg_calls = kt.get("mg", (0.0, 0))
     network = None
     if net_calls > 0 and net_ms > 0:
-        fl = learned.flops_per_apply(sc.grid.shape)
+        fl = learned.flops_per_apply(cdims)
         tf = fl / (net_ms / net_calls * 1e-3) / 1e12
-        network = {"bound": "mfma", "achieved": round(tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                   "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4), "flops_per_apply": fl,
+        network = {"bound": "mfma", "achieved": round(tf, 3), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 5), "grid": list(cdims), "flops_per_apply": fl,
                    "ms_per_apply": round(net_ms / net_calls, 4), "applies": net_calls,
-                   "note": "ms_per_apply covers the network and its damped-Jacobi steps"}
+                   "note": "the network on the coarsest level: launch-bound at this size, not MFMA-bound"}
     mic_ms, l_ms = res["mic0_pcg"]["solve_ms"], res["learned"]["solve_ms"]
-    return {"grid": [n] * 3, "workload": f"{what}, 4th step, same right-hand side for both solvers",
-            "mic0_pcg": res["mic0_pcg"], "learned": res["learned"], "fpcg_iterations": iters,
+    return {"grid": [n] * 3, "workload": f"{what}, 4th step, same right-hand side for every solver",
+            "multigrid": {"levels": levels, "coarsest": list(cdims),
+                          "ms_per_vcycle": round(mg_ms / mg_calls, 4) if mg_calls else None,
+                          "note": "ms_per_vcycle covers the whole V-cycle, network included; setup counted once"},
+            "mic0_pcg": res["mic0_pcg"], "learned": res["learned"], "mg_smooth": res["mg_smooth"],
+            "fpcg_iterations": iters,
             "learned_over_mic0_time": round(l_ms / mic_ms, 3) if mic_ms > 0 else None,
             "network": network}
 
@@ -307,7 +386,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256, help="grid cells per axis (default 256)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=cpu_threads_default())
+    ap.add_argument("--error-trace-steps", type=int, default=20,
+                    help="steps of the 128^3 GPU-vs-oracle error trace (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=None,
                     help="steps timed per kernel with HIP events after the timed region (default: --steps)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -317,9 +398,9 @@ def main():
                     help="skip the pressure-solve comparison MIC(0)-PCG vs learned FPCG")
     ap.add_argument("--learned-sizes", default="128,256",
                     help="comma-separated grid sizes of the MIC(0)-PCG vs learned FPCG comparison")
-    ap.add_argument("--learned-train-size", type=int, default=32)
-    ap.add_argument("--learned-scenes", type=int, default=60)
-    ap.add_argument("--learned-epochs", type=int, default=15)
+    ap.add_argument("--learned-train-size", type=int, default=64)
+    ap.add_argument("--learned-scenes", type=int, default=24)
+    ap.add_argument("--learned-epochs", type=int, default=60)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -391,6 +472,10 @@ def main():
         cpu, accuracy = cpu_baseline(scene, args.cpu_threads)
         print(f"# accuracy vs the FP64 oracle: {json.dumps(accuracy)}", file=sys.stderr, flush=True)
 
+    trace = None
+    if args.error_trace_steps > 0 and not args.no_cpu_baseline:
+        trace = error_trace(args.error_trace_steps, args.cpu_threads)
+
     out = {
         "metric": "steps/s @256^3 FLIP",
         "value": round(steps_per_s, 4),
@@ -411,6 +496,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "error_trace": trace,
         "pressure_solve": solve_cmp,
         "kernel_ms_per_step": breakdown,
     }

@@ -39,7 +39,7 @@ FIELD_DTYPES = {FIELD_U: np.float32, FIELD_V: np.float32, FIELD_W: np.float32, F
                 FIELD_V2: np.float32, FIELD_W2: np.float32, FIELD_LABEL: np.uint8, FIELD_DENSITY: np.float32,
                 FIELD_PRESSURE: np.float64, FIELD_RHS: np.float32, FIELD_PRECON: np.float64,
                 FIELD_CELL_COUNT: np.int32, FIELD_CELL_START: np.int32}
-OP_APPLY_A, OP_APPLY_MIC0, OP_FACTOR_MIC0 = 0, 1, 2
+OP_APPLY_A, OP_APPLY_MIC0, OP_FACTOR_MIC0, OP_APPLY_MG = 0, 1, 2, 3
 
 
 class GridDesc(C.Structure):
@@ -113,6 +113,11 @@ PROTOTYPES = {
     "fsim_learned_forward": (C.c_int, [_P, _P, _P, _P]),
     "fsim_learned_grad": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(C.c_double)]),
     "fsim_learned_train_batch": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_float, C.POINTER(C.c_double)]),
+    "fsim_learned_forward_op": (C.c_int, [_P, _P, _P, _P]),
+    "fsim_learned_grad_op": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(C.c_double)]),
+    "fsim_learned_train_batch_op": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_float, C.POINTER(C.c_double)]),
+    "fsim_mg_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fsim_learned_harvest": (C.c_int, [_P, C.c_double, C.c_int, _P, _P, _P, C.POINTER(C.c_int)]),
     "fsim_scene_dam_break": (C.c_int, [C.POINTER(GridDesc), C.c_int, _P, C.c_int, C.POINTER(Obstacle), C.c_uint64,
                                        C.c_int64, _P, C.POINTER(C.c_int64)]),
     "fsim_gfx_snapshot": (C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_int64)]),
@@ -123,7 +128,8 @@ PROTOTYPES = {
     "fsim_reset_kernel_times": (C.c_int, [_P]),
 }
 (OPT_SWEEP_MODE, OPT_PROFILE, OPT_LEARNED_SMOOTHING, OPT_LEARNED_LOSS, OPT_LEARNED_CONV, OPT_LEARNED_REFRESH,
- OPT_LEARNED_MIC_COUNT) = range(7)
+ OPT_LEARNED_MIC_COUNT, OPT_MG_COARSE) = range(8)
+MG_COARSE_NETWORK, MG_COARSE_SMOOTH, MG_COARSE_ACCURATE = 0, 1, 2
 
 _lib = None
 
@@ -383,6 +389,58 @@ class Simulation:
         loss = C.c_double(0.0)
         self._check(self.lib.fsim_learned_grad(self._ctx, _ptr(b), _ptr(lab), _ptr(pv), _ptr(g), C.byref(loss)))
         return loss.value, g
+
+    def _op(self, op):
+        o = np.ascontiguousarray(op, dtype=np.float32).reshape(-1)
+        if o.size % (4 * self.grid.nc):
+            raise ValueError("an operator has 4 values per cell: diag, cx, cy, cz")
+        return o
+
+    def learned_forward_op(self, b: np.ndarray, op: np.ndarray) -> np.ndarray:
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+        out = np.empty(self.grid.nc, np.float32)
+        self._check(self.lib.fsim_learned_forward_op(self._ctx, _ptr(bb), _ptr(self._op(op)), _ptr(out)))
+        return out
+
+    def learned_grad_op(self, b: np.ndarray, op: np.ndarray, p: np.ndarray):
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+        pv = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        g = np.empty(learned_num_params(), np.float32)
+        loss = C.c_double(0.0)
+        self._check(self.lib.fsim_learned_grad_op(self._ctx, _ptr(bb), _ptr(self._op(op)), _ptr(pv), _ptr(g),
+                                                  C.byref(loss)))
+        return loss.value, g
+
+    def learned_train_batch_op(self, b: np.ndarray, op: np.ndarray, p: np.ndarray, lr: float) -> float:
+        n = len(b)
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(n, -1)
+        pv = np.ascontiguousarray(p, dtype=np.float64).reshape(n, -1)
+        o = self._op(op)
+        if o.size != 4 * bb.size:
+            raise ValueError("one operator per sample")
+        loss = C.c_double(0.0)
+        self._check(self.lib.fsim_learned_train_batch_op(self._ctx, n, _ptr(bb), _ptr(o), _ptr(pv), C.c_float(lr),
+                                                         C.byref(loss)))
+        return loss.value
+
+    def mg_info(self):
+        """(number of multigrid levels, coarsest level's dims)"""
+        lv = C.c_int()
+        d = (C.c_int * 3)()
+        self._check(self.lib.fsim_mg_info(self._ctx, C.byref(lv), d))
+        return lv.value, tuple(d)
+
+    def learned_harvest(self, dt: float, capacity: int):
+        """Coarsest-level problems of one learned-multigrid solve of the current state (after the
+        preparation stages): (b[n, Nc'], op[4, Nc'], x[n, Nc']) on the coarse level's grid."""
+        _, dims = self.mg_info()
+        nc = int(np.prod(dims))
+        b = np.empty((max(capacity, 1), nc), np.float32)
+        x = np.empty((max(capacity, 1), nc), np.float32)
+        op = np.empty((4, nc), np.float32)
+        n = C.c_int()
+        self._check(self.lib.fsim_learned_harvest(self._ctx, dt, capacity, _ptr(b), _ptr(op), _ptr(x), C.byref(n)))
+        return b[: n.value], op, x[: n.value], dims
 
     def learned_train_batch(self, b: np.ndarray, label: np.ndarray, p: np.ndarray, lr: float) -> float:
         n = len(b)

@@ -4,16 +4,53 @@
 //
 // usage: fsim_facade_demo <n> <steps> <transfer 0|1|2> [out.bin]
 //   out.bin receives the final particle positions (float32, 3 per particle)
+//        fsim_facade_demo particles <num> <grown> <shrunk>
+//   builds HashedParticles(num, ...) on a 32^3 unit grid, grows it to
+//   <grown> and shrinks it to <shrunk> with setParticleNum (host only, no
+//   device needed) and prints the ranges it saw
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
+#include <string>
 #include <vector>
 
 #include "fsim_simulator.hpp"
 
 using namespace fsim;
 
+static int particles_demo(int num, int grown, int shrunk) {
+    const double dx = 1.0 / 32, r = dx / 4;
+    particles::HashedParticles hp(num, r, dvec3(1, 1, 1), dvec3(dx, dx, dx), false, 0.0, 7);
+    double lo0 = 1e9, hi0 = -1e9, lo1 = 1e9, hi1 = -1e9, vmax = 0;
+    const size_t n0 = hp.getParticleNum();
+    std::vector<double> first;
+    for (size_t i = 0; i < n0; i++) {
+        const auto& q = hp.getParticleAt(i);
+        for (double v : {q.pos.x, q.pos.y, q.pos.z}) lo0 = v < lo0 ? v : lo0, hi0 = v > hi0 ? v : hi0;
+        first.push_back(q.pos.x);
+    }
+    hp.setParticleNum((size_t)grown);
+    const size_t n1 = hp.getParticleNum();
+    double mean = 0;
+    for (size_t i = n0; i < n1; i++) {
+        const auto& q = hp.getParticleAt(i);
+        for (double v : {q.pos.x, q.pos.y, q.pos.z}) lo1 = v < lo1 ? v : lo1, hi1 = v > hi1 ? v : hi1, mean += v;
+        for (double v : {q.v.x, q.v.y, q.v.z}) vmax = v * v > vmax ? v * v : vmax;
+    }
+    mean /= 3.0 * (double)(n1 - n0 > 0 ? n1 - n0 : 1);
+    hp.setParticleNum((size_t)shrunk);
+    const size_t n2 = hp.getParticleNum();
+    bool kept = true;
+    for (size_t i = 0; i < n2 && i < first.size(); i++) kept = kept && hp.getParticleAt(i).pos.x == first[i];
+    std::printf("{\"n0\": %zu, \"lo0\": %.9f, \"hi0\": %.9f, \"n1\": %zu, \"lo1\": %.9f, \"hi1\": %.9f, "
+                "\"mean1\": %.6f, \"vmax1\": %.3g, \"n2\": %zu, \"kept\": %s}\n",
+                n0, lo0, hi0, n1, lo1, hi1, mean, vmax, n2, kept ? "true" : "false");
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 4 && std::string(argv[1]) == "particles")
+        return particles_demo(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]));
     const int n = argc > 1 ? std::atoi(argv[1]) : 32;
     const int steps = argc > 2 ? std::atoi(argv[2]) : 3;
     const int transfer = argc > 3 ? std::atoi(argv[3]) : 1;

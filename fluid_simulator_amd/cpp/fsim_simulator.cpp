@@ -1,6 +1,7 @@
 // fsim_simulator.cpp — the C++ facade's forwarding to the C ABI.
 #include "fsim_simulator.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace fsim {
@@ -61,6 +62,37 @@ namespace particles {
 HashedParticles::HashedParticles(std::vector<Particle> initial, double particleR)
     : host_(std::move(initial)), r_(particleR) {}
 
+HashedParticles::HashedParticles(int num, double r, dvec3 dimensions, dvec3 cellD, bool zConst, double z,
+                                 uint64_t seed)
+    : r_(r), dims_(dimensions), cellD_(cellD), grid_known_(true), zConst_(zConst), z_(z), seed_(seed) {
+    host_.reserve((size_t)std::max(num, 0));
+    for (int p = 0; p < num; p++) {
+        Particle q;
+        q.pos.x = uniform(dims_.x * 0.5 + cellD_.x, dims_.x - 1.1 * r_ - cellD_.x);
+        q.pos.y = uniform(dims_.y * 0.5 + cellD_.y, dims_.y - 1.1 * r_ - cellD_.y);
+        q.pos.z = zConst_ ? z_ : uniform(dims_.z * 0.5 + cellD_.z, dims_.z - 1.1 * r_ - cellD_.z);
+        host_.push_back(q);
+    }
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014), the n-th output for the seed; the
+// top 53 bits as a double in [0, 1)
+static double splitmix_unit(uint64_t seed, uint64_t n) {
+    uint64_t z = seed + (n + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+double HashedParticles::uniform(double lo, double hi) { return lo + (hi - lo) * splitmix_unit(seed_, draws_++); }
+
+void HashedParticles::setGrid(const dvec3& cellD, const dvec3& dimensions) {
+    cellD_ = cellD;
+    dims_ = dimensions;
+    grid_known_ = true;
+}
+
 void HashedParticles::ensureHost() {
     if (host_stale_ && refresh_) refresh_();
     host_stale_ = false;
@@ -85,7 +117,28 @@ size_t HashedParticles::getParticleNum() {
 }
 void HashedParticles::setParticleNum(size_t n) {
     ensureHost();
-    host_.resize(n);
+    if (n > host_.size() && !grid_known_)
+        throw std::runtime_error("setParticleNum: the grid of the particles is not known yet");
+    if (n < host_.size()) host_.resize(n);
+    while (host_.size() < n) {
+        Particle q;
+        q.pos.x = uniform(1.1 * r_ + cellD_.x, dims_.x - 1.1 * r_ - cellD_.x);
+        q.pos.y = uniform(1.1 * r_ + cellD_.y, dims_.y - 1.1 * r_ - cellD_.y);
+        q.pos.z = zConst_ ? z_ : uniform(1.1 * r_ + cellD_.z, dims_.z - 1.1 * r_ - cellD_.z);
+        host_.push_back(q);
+    }
+    device_stale_ = true;
+}
+
+void HashedParticles::updateGridParams(const dvec3& cellD, const dvec3& dimensions) {
+    ensureHost();
+    setGrid(cellD, dimensions);
+    auto clampv = [](double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    for (Particle& q : host_) {
+        q.pos.x = clampv(q.pos.x, 1.1 * r_ + cellD.x, dimensions.x - 1.1 * r_ - cellD.x);
+        q.pos.y = clampv(q.pos.y, 1.1 * r_ + cellD.y, dimensions.y - 1.1 * r_ - cellD.y);
+        q.pos.z = zConst_ ? z_ : clampv(q.pos.z, 1.1 * r_ + cellD.z, dimensions.z - 1.1 * r_ - cellD.z);
+    }
     device_stale_ = true;
 }
 void HashedParticles::setParticleR(double r) {
@@ -150,6 +203,7 @@ Simulator::Simulator(SimulatorConfig cfg, std::shared_ptr<particles::HashedParti
                      std::shared_ptr<macgrid::MacGrid> mg)
     : config(cfg), hashedParticles(std::move(hp)), macGrid(std::move(mg)) {
     for (const char* k : kTimerKeys) stepDuration[k] = 0;
+    if (!hashedParticles->grid_known_) hashedParticles->setGrid(macGrid->cellD, macGrid->dimensions);
     createContext();
 }
 
@@ -202,7 +256,7 @@ fsim_params Simulator::makeParams() const {
     p.fluid_density = macGrid->fluidDensity;
     p.residual_tolerance = macGrid->residualTolerance;
     p.stop_rule = stopRule;
-    p.precond = precond;
+    p.precond = dynamic_cast<const macgrid::BasicMacGrid*>(macGrid.get()) ? FSIM_SOLVER_BASIC_SOR : precond;
     p.seed = seed;
     return p;
 }
@@ -261,6 +315,7 @@ void Simulator::setNewMacGrid(std::shared_ptr<macgrid::MacGrid> mg) {
 void Simulator::setNewHashedParticles(std::shared_ptr<particles::HashedParticles> hp) {
     hashedParticles->refresh_ = nullptr;
     hashedParticles = std::move(hp);
+    if (!hashedParticles->grid_known_) hashedParticles->setGrid(macGrid->cellD, macGrid->dimensions);
     hashedParticles->device_stale_ = true;
     hashedParticles->r_dirty_ = true;
     hashedParticles->refresh_ = [this]() { pullParticles(); };

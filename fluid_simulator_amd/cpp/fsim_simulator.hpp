@@ -104,6 +104,11 @@ struct Particle {
 class HashedParticles {
 public:
     HashedParticles(std::vector<Particle> initial, double particleR);
+    // hashedParticles.cpp:23-42: num particles at uniform positions in
+    // [dimensions/2 + cellD, dimensions - 1.1 r - cellD] per axis (z = the
+    // given z when zConst), at rest.  The positions come from a SplitMix64
+    // counter RNG keyed by `seed` in place of std::rand (random.h:13-28).
+    HashedParticles(int num, double r, dvec3 dimensions, dvec3 cellD, bool zConst, double z, uint64_t seed = 0);
 
     // forEach may write (the next step uploads the particles again);
     // forEachConst only reads (extension for snapshot readers).
@@ -111,7 +116,13 @@ public:
     void forEachConst(const std::function<void(const Particle&)>& fn);
     Particle& getParticleAt(size_t idx);
     size_t getParticleNum();
-    void setParticleNum(size_t n);   // truncates, or appends zero particles
+    // hashedParticles.cpp:132-151: drops particles from the end, or adds
+    // particles at rest at uniform positions in [1.1 r + cellD,
+    // dimensions - 1.1 r - cellD] per axis (z = the fixed z when zConst)
+    void setParticleNum(size_t n);
+    // hashedParticles.cpp:340-353: new grid spacing and size; every particle
+    // is clamped into [1.1 r + cellD, dimensions - 1.1 r - cellD]
+    void updateGridParams(const dvec3& cellD, const dvec3& dimensions);
     double getParticleR() const { return r_; }
     void setParticleR(double r);
     void addParticles(const std::vector<Particle>& ps);
@@ -120,6 +131,13 @@ private:
     friend class fsim::simulator::Simulator;
     std::vector<Particle> host_;
     double r_;
+    dvec3 dims_{1, 1, 1}, cellD_{0, 0, 0};   // set by the constructor above or by the Simulator's grid
+    bool grid_known_ = false;
+    bool zConst_ = false;
+    double z_ = 0;
+    uint64_t seed_ = 0, draws_ = 0;
+    double uniform(double lo, double hi);   // getDoubleInRange (random.h:24-26)
+    void setGrid(const dvec3& cellD, const dvec3& dimensions);
     bool host_stale_ = false;      // the device holds newer particles
     bool device_stale_ = true;     // the host was written after the last upload
     bool r_dirty_ = true;
@@ -137,6 +155,7 @@ class MacGrid {
 public:
     // MacGrid(targetDimensions, resolution) with equal cubic cells (macGrid.cpp:9-13)
     MacGrid(const dvec3& targetDimensions, int resolution, bool twoD = false);
+    virtual ~MacGrid() = default;
 
     const dvec3 cellD;
     const dvec3 cellDInv;
@@ -163,6 +182,22 @@ private:
     std::vector<float> v2_[3];
     std::vector<double> p_;
     bool stale_[5] = {true, true, true, true, true};
+};
+
+// The two solver plugins the manager chooses between by
+// SimulationConfig::GridSolverType (simulationManager.cpp:16-19):
+// BridsonSolverGrid solves with MIC(0)-PCG (bridsonSolverGrid.h:10-43),
+// BasicMacGrid with red-black SOR sweeps (basicMacGrid.h:7-16).
+class BridsonSolverGrid : public MacGrid {
+public:
+    BridsonSolverGrid(const dvec3& targetDimensions, float resolution, bool twoD, double fluidDensity_ = 1.0)
+        : MacGrid(targetDimensions, (int)resolution, twoD) {
+        fluidDensity = fluidDensity_;
+    }
+};
+class BasicMacGrid : public MacGrid {
+public:
+    using MacGrid::MacGrid;
 };
 
 }  // namespace macgrid
@@ -206,7 +241,8 @@ public:
     // Extensions: the last step's solver statistics and the raw context.
     const fsim_step_stats& lastStats() const { return stats_; }
     fsim_ctx* context() { return ctx_; }
-    // Pressure-solve options with no reference counterpart.
+    // Pressure-solve options with no reference counterpart.  A BasicMacGrid
+    // grid always solves with FSIM_SOLVER_BASIC_SOR.
     int stopRule = FSIM_STOP_ABS_INF;
     int precond = FSIM_PRECOND_MIC0;
     uint64_t seed = 0;

@@ -30,7 +30,7 @@ int stage_p2g_fused(fsim_ctx* ctx);
 static const char* const kKernelNames[KID_COUNT] = {
     "spawn", "move", "bin_keys", "bin_scan", "bin_scatter", "bin_segsort", "bin_gather", "p2g", "mark",
     "obstacles", "borders", "post_p2g", "rhs", "mic_factor", "sweep_fwd", "sweep_bwd", "spmv", "update", "dot",
-    "true_residual", "pressure", "extrapolate", "g2p", "learned", "basic_sor", "push_apart"};
+    "true_residual", "pressure", "extrapolate", "g2p", "learned", "basic_sor", "push_apart", "mg"};
 
 int set_profiling(fsim_ctx* ctx, bool on) {
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -103,6 +103,7 @@ static int dalloc(fsim_ctx* ctx, T** p, int64_t n) {
 
 static void free_all(fsim_ctx* ctx) {
     learned_free(ctx);
+    mg_free(ctx);
     gfx_free(ctx);
     void* ptrs[] = {ctx->P.base, ctx->Palt.base, ctx->key, ctx->rank, ctx->perm, ctx->removed, ctx->u, ctx->v, ctx->w,
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
@@ -346,7 +347,15 @@ int fsim_get_stats(const fsim_ctx* ctx, fsim_step_stats* out) {
 static int run_stage(fsim_ctx* ctx, int stage, double dt) {
     switch (stage) {
     case FSIM_STAGE_SPAWN: return stage_spawn(ctx, dt);
-    case FSIM_STAGE_ADVECT: FSIM_TRY(upload_obstacles(ctx)); return stage_move(ctx, dt, 1);
+    case FSIM_STAGE_ADVECT: {
+        FSIM_TRY(upload_obstacles(ctx));
+        FSIM_TRY(stage_move(ctx, dt, 1));
+        // removeParticles runs inside advectParticles (simulator.cpp:250): drop
+        // the particles a sink took now (the binning compacts them)
+        bool sink = false;
+        for (const fsim_obstacle& o : ctx->obs) sink = sink || o.kind == FSIM_OBS_SINK;
+        return (sink && ctx->prm.despawning_enabled) ? stage_bin(ctx) : FSIM_OK;
+    }
     case FSIM_STAGE_PUSH_OUT: FSIM_TRY(upload_obstacles(ctx)); return stage_move(ctx, dt, 2);
     case FSIM_STAGE_STOP: return stage_move(ctx, dt, 4);
     case FSIM_STAGE_BIN: return stage_bin(ctx);
@@ -557,6 +566,11 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
         ctx->last_mic_iterations = value;
         ctx->learned_since_mic = 0;
         return FSIM_OK;
+    case FSIM_OPT_MG_COARSE:
+        if (value < FSIM_MG_COARSE_NETWORK || value > FSIM_MG_COARSE_ACCURATE)
+            return set_err(ctx, FSIM_ERR_INVALID, "unknown coarse-level mode");
+        ctx->mg_coarse = value;
+        return FSIM_OK;
     default: return set_err(ctx, FSIM_ERR_INVALID, "unknown option");
     }
 }
@@ -585,21 +599,68 @@ int fsim_learned_get_params(fsim_ctx* ctx, float* params) {
     return learned_get_params(ctx, params);
 }
 
+// the operator of a label field, or of op, as flat float arrays [4][Nc]
+static std::vector<float> label_op(const fsim_ctx* ctx, const uint8_t* label, int n = 1) {
+    const int64_t nc = ctx->g.nc;
+    std::vector<float> op((size_t)(4 * nc * n));
+    for (int k = 0; k < n; k++) label_operator(ctx->g.nx, ctx->g.ny, ctx->g.nz, label + k * nc, op.data() + 4 * k * nc);
+    return op;
+}
+
 int fsim_learned_forward(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out) {
     if (!ctx || !b || !label || !out) return FSIM_ERR_INVALID;
-    return learned_forward_host(ctx, b, label, out);
+    return learned_forward_host(ctx, b, label_op(ctx, label).data(), out);
+}
+
+int fsim_learned_forward_op(fsim_ctx* ctx, const float* b, const float* op, float* out) {
+    if (!ctx || !b || !op || !out) return FSIM_ERR_INVALID;
+    return learned_forward_host(ctx, b, op, out);
 }
 
 int fsim_learned_grad(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad,
                       double* loss) {
     if (!ctx || !b || !label || !p || !grad || !loss) return FSIM_ERR_INVALID;
-    return learned_grad_host(ctx, b, label, p, grad, loss);
+    return learned_grad_host(ctx, b, label_op(ctx, label).data(), p, grad, loss);
+}
+
+int fsim_learned_grad_op(fsim_ctx* ctx, const float* b, const float* op, const double* p, float* grad, double* loss) {
+    if (!ctx || !b || !op || !p || !grad || !loss) return FSIM_ERR_INVALID;
+    return learned_grad_host(ctx, b, op, p, grad, loss);
 }
 
 int fsim_learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
                              double* mean_loss) {
     if (!ctx || !b || !label || !p || !mean_loss || n <= 0 || !(lr > 0.f)) return FSIM_ERR_INVALID;
-    return learned_train_batch(ctx, n, b, label, p, lr, mean_loss);
+    return learned_train_batch(ctx, n, b, label_op(ctx, label, n).data(), p, lr, mean_loss);
+}
+
+int fsim_learned_train_batch_op(fsim_ctx* ctx, int n, const float* b, const float* op, const double* p, float lr,
+                                double* mean_loss) {
+    if (!ctx || !b || !op || !p || !mean_loss || n <= 0 || !(lr > 0.f)) return FSIM_ERR_INVALID;
+    return learned_train_batch(ctx, n, b, op, p, lr, mean_loss);
+}
+
+int fsim_mg_info(fsim_ctx* ctx, int* levels, int dims[3]) {
+    if (!ctx) return FSIM_ERR_INVALID;
+    return mg_info(ctx, levels, dims);
+}
+
+int fsim_learned_harvest(fsim_ctx* ctx, double dt, int capacity, float* b, float* op, float* x, int* n_out) {
+    if (!ctx || capacity < 0 || !n_out || (capacity > 0 && (!b || !op || !x)) || !(dt > 0.0)) return FSIM_ERR_INVALID;
+    *n_out = 0;
+    FSIM_TRY(mg_harvest_begin(ctx, std::max(capacity, 1)));
+    const int precond = ctx->prm.precond, coarse = ctx->mg_coarse;
+    ctx->prm.precond = FSIM_PRECOND_LEARNED;
+    ctx->mg_coarse = FSIM_MG_COARSE_ACCURATE;
+    ctx->mg_harvesting = true;
+    int rc = stage_solve(ctx, dt);
+    if (rc == FSIM_OK) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? FSIM_OK : FSIM_ERR_HIP;
+    ctx->prm.precond = precond;
+    ctx->mg_coarse = coarse;
+    ctx->mg_harvesting = false;
+    if (rc != FSIM_OK) return rc;
+    if (capacity == 0) return FSIM_OK;
+    return mg_harvest_end(ctx, capacity, b, op, x, n_out);
 }
 
 }  // extern "C"

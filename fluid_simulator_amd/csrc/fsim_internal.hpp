@@ -90,6 +90,50 @@ __host__ __device__ inline int64_t ridx(int x, int y, int z, int nx, int ny, int
     return ((int64_t)rot_plane(x, z, nx) * ny + y) * nz + z;
 }
 
+// Dimensions and scale of a pressure solve; stencil neighbours in the rotated layout.
+struct SolveArgs {
+    int nx, ny, nz;
+    int off63;          // 63 mod nx: the plane shift of a z step across a 64-wide z-tile edge
+    int64_t nc;
+    float scale_f;      // dt / (rho dx^2)
+    double scale;
+};
+
+// flat index of the grid fields (labels, velocities, density)
+__device__ inline int64_t sidx(const SolveArgs& S, int x, int y, int z) { return ((int64_t)x * S.ny + y) * S.nz + z; }
+// index of the solver vectors, nbinfo and W (rotated layout)
+__device__ inline int64_t rsidx(const SolveArgs& S, int x, int y, int z) { return ridx(x, y, z, S.nx, S.ny, S.nz); }
+// Rotated indices of the six neighbours of the interior cell stored at
+// plane p, row y, column z (rotated index i): +-y are +-nz; +-x are the
+// next/previous plane; +-z move one column and one plane, except across a
+// 64-wide z-tile edge where rot(z) wraps 63 <-> 0 and the plane moves by 63.
+struct RNeighbours {
+    int64_t px, mx, py, my, pz, mz;
+};
+__device__ inline RNeighbours rneighbours(const SolveArgs& S, int64_t i, int p, int y, int z) {
+    const int64_t plane = (int64_t)S.ny * S.nz;
+    const int64_t yz = (int64_t)y * S.nz + z;
+    const int rz = rot_of(z);
+    const int pn = p + 1 == S.nx ? 0 : p + 1;
+    const int pp = p == 0 ? S.nx - 1 : p - 1;
+    RNeighbours n;
+    n.px = pn * plane + yz;
+    n.mx = pp * plane + yz;
+    n.py = i + S.nz;
+    n.my = i - S.nz;
+    const int o = S.off63;
+    n.pz = (rz == WF_TZ - 1 ? (p >= o ? p - o : p - o + S.nx) : pn) * plane + yz + 1;
+    n.mz = (rz == 0 ? (p + o >= S.nx ? p + o - S.nx : p + o) : pp) * plane + yz - 1;
+    return n;
+}
+// rotated indices of the +a and -a neighbours of interior cell (x, y, z)
+__device__ inline int64_t rnb_plus(const SolveArgs& S, int x, int y, int z, int a) {
+    return rsidx(S, x + (a == 0), y + (a == 1), z + (a == 2));
+}
+__device__ inline int64_t rnb_minus(const SolveArgs& S, int x, int y, int z, int a) {
+    return rsidx(S, x - (a == 0), y - (a == 1), z - (a == 2));
+}
+
 struct WaveArgs {
     int nx, ny, nz;
     int TJ, TK;                   // tiles along y and z
@@ -134,7 +178,7 @@ enum KernelId {
     KID_SPAWN = 0, KID_MOVE, KID_BIN_KEYS, KID_BIN_SCAN, KID_BIN_SCATTER, KID_BIN_SEGSORT, KID_BIN_GATHER,
     KID_P2G, KID_MARK, KID_OBSTACLES, KID_BORDERS, KID_POST_P2G, KID_RHS, KID_MIC_FACTOR, KID_SWEEP_FWD,
     KID_SWEEP_BWD, KID_SPMV, KID_UPDATE, KID_DOT, KID_TRUE_RES, KID_PRESSURE, KID_EXTRAPOLATE, KID_G2P,
-    KID_LEARNED, KID_SOR, KID_PUSH_APART, KID_COUNT
+    KID_LEARNED, KID_SOR, KID_PUSH_APART, KID_MG, KID_COUNT
 };
 
 struct Particles {
@@ -150,6 +194,9 @@ struct Particles {
 
 namespace fsim {
 struct LearnedNet;
+struct MgState;
+constexpr int kMgMaxLevels = 8;
+constexpr int kMgCoarseMaxSide = 16;   // the network's level: the first whose largest side is <= this
 }
 
 struct fsim_ctx {
@@ -198,6 +245,9 @@ struct fsim_ctx {
     int learned_since_mic = 0;        // learned solves since that MIC(0) solve
     int learned_refresh = 50;         // learned solves between MIC(0) solves that renew the count (0: never)
     fsim::LearnedNet* learned = nullptr;   // learned preconditioner (network parameters and buffers)
+    fsim::MgState* mg = nullptr;          // multigrid hierarchy of the learned preconditioner
+    int mg_coarse = 0;                    // coarsest-level solve: FSIM_MG_COARSE_*
+    bool mg_harvesting = false;           // a solve run by fsim_learned_harvest
     int partials_cap = 0;
     fsim::SolveScalars* scal = nullptr;   // device
     fsim::SolveScalars* scal_host = nullptr;  // pinned host mirror
@@ -211,7 +261,7 @@ struct fsim_ctx {
     uint32_t wave_epoch = 0;
     int* wave_err = nullptr;
     int sweep_mode = 0;                          // 0 wavefront, 1 one launch per plane
-    int learned_smoothing = 4;                   // damped-Jacobi steps after the network
+    int learned_smoothing = 2;                   // red-black sweeps after the network on the coarsest level
     int learned_loss = 0;                        // training loss: 0 masked MSE, 1 energy norm
     int learned_conv_lds = 1;                    // 1: LDS-staged convolutions where they fit
 
@@ -307,16 +357,30 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
 int gfx_snapshot(fsim_ctx* ctx, fsim_gfx_particle* host_out);
 void gfx_free(fsim_ctx* ctx);
 
+// ---- multigrid of the learned preconditioner (fsim_mg.hip) ------------------
+int mg_levels_for(int nx, int ny, int nz);
+int mg_setup(fsim_ctx* ctx);                                        // coarse operators of the current labels
+int mg_apply(fsim_ctx* ctx, const float* r, float* z, double scale);  // z = V-cycle(r), rotated layout
+int mg_info(fsim_ctx* ctx, int* levels, int* dims);
+int mg_harvest_begin(fsim_ctx* ctx, int cap);
+int mg_harvest_end(fsim_ctx* ctx, int cap, float* b, float* op, float* x, int* n_out);
+void mg_free(fsim_ctx* ctx);
+
 // ---- learned preconditioner (fsim_learned.hip) ------------------------------
+// the network on the coarsest multigrid level (flat, operator arrays): x = out * max|b| * norm
+int learned_apply_level(fsim_ctx* ctx, int nx, int ny, int nz, const float* b, const float* diag, const float* ox,
+                        const float* oy, const float* oz, double norm, float* x);
+// the network on the solve's own grid (rotated layout): z = out * max|r| * binv
+int learned_apply_level0(fsim_ctx* ctx, const float* r, float* z, float binv);
 int64_t learned_num_params();
 int learned_init(fsim_ctx* ctx, uint64_t seed);
 int learned_set_params(fsim_ctx* ctx, const float* params);
 int learned_get_params(fsim_ctx* ctx, float* params);
 bool learned_ready(const fsim_ctx* ctx);
-int learned_apply(fsim_ctx* ctx, const float* r, float* z);   // z = M(r), solver layout
-int learned_forward_host(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out);
-int learned_grad_host(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad, double* loss);
-int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
+void label_operator(int nx, int ny, int nz, const uint8_t* label, float* op);   // op[4][nc] of a label field
+int learned_forward_host(fsim_ctx* ctx, const float* b, const float* op, float* out);
+int learned_grad_host(fsim_ctx* ctx, const float* b, const float* op, const double* p, float* grad, double* loss);
+int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const float* op, const double* p, float lr,
                         double* mean_loss);
 void learned_free(fsim_ctx* ctx);
 

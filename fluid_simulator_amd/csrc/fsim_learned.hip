@@ -3,7 +3,9 @@
 //
 // Network (fully convolutional, so weights trained on small grids apply to
 // any grid whose padded size is a multiple of 8):
-//   in  = (r / s, WATER mask, 0 x 6)                         level 0, 8 ch
+//   in  = (b / s, unknown mask, diag / 6, cx, cy, cz, 0, 0)   level 0, 8 ch
+//         (the operator L of the grid it runs on: diagonal diag, couplings
+//         -cx, -cy, -cz to the +x, +y, +z neighbour; out ~ L^-1 b / s)
 //   e0  = relu(conv3(in, 8 -> 16))                           level 0
 //   e1  = relu(conv3(pool(e0), 16 -> 32))                    level 1
 //   e2  = relu(conv3(pool(e1), 32 -> 64))                    level 2
@@ -11,7 +13,7 @@
 //   d2  = relu(conv3(cat(up(e3), e2), 192 -> 64))            level 2
 //   d1  = relu(conv3(cat(up(d2), e1), 96 -> 32))             level 1
 //   d0  = relu(conv3(cat(up(d1), e0), 48 -> 16))             level 0
-//   out = conv1(d0, 16 -> 1);  z = WATER ? out * s : 0
+//   out = conv1(d0, 16 -> 1)
 // with s = max|r|, pool = 2x2x2 average, up = nearest, conv3 = 3x3x3, stride 1,
 // zero padding.  Activations and weights are BF16, every sum is accumulated in
 // FP32 (MFMA), biases and the output are FP32.
@@ -382,30 +384,54 @@ __global__ void k_upcat(const __bf16* __restrict__ coarse, const __bf16* __restr
     out[i] = coarse[pv * Cu + c];
 }
 
-// network input from the solver's r (rotated layout) or from a flat host-order vector
-__global__ void k_net_input(const float* __restrict__ r, const uint16_t* __restrict__ nbinfo, int rotated,
-                            const uint8_t* __restrict__ label, const float* __restrict__ scale, __bf16* __restrict__ x0,
-                            Dims nd, int nx, int ny, int nz) {
+// Network input: (b / s, unknown, diag / 6, cx, cy, cz, 0, 0) per voxel, from
+// the solver's r and nbinfo (rotated layout: the 7-point operator of the
+// WATER cells, couplings 1 towards WATER neighbours) ...
+__global__ void k_net_input(const float* __restrict__ r, const uint16_t* __restrict__ nbinfo,
+                            const float* __restrict__ scale, __bf16* __restrict__ x0, Dims nd, int nx, int ny, int nz) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nd.V()) return;
     const int z = (int)(v % nd.W), y = (int)((v / nd.W) % nd.H), x = (int)(v / ((int64_t)nd.W * nd.H));
-    float val = 0.f, fl = 0.f;
+    float val = 0.f, fl = 0.f, dg = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     if (x < nx && y < ny && z < nz) {
-        if (rotated) {
-            const int64_t i = ridx(x, y, z, nx, ny, nz);
-            const bool f = (nbinfo[i] & NB_FLUID) != 0;
-            fl = f ? 1.f : 0.f;
-            val = f ? r[i] : 0.f;
-        } else {
-            const int64_t i = ((int64_t)x * ny + y) * nz + z;
-            const bool f = label[i] == FSIM_WATER;
-            fl = f ? 1.f : 0.f;
-            val = f ? r[i] : 0.f;
+        const int64_t i = ridx(x, y, z, nx, ny, nz);
+        const uint16_t f = nbinfo[i];
+        if (f & NB_FLUID) {
+            fl = 1.f;
+            val = r[i];
+            dg = (float)((f >> NB_NONSOLID_SHIFT) & 7);
+            cx = (f & NB_PX) ? 1.f : 0.f;
+            cy = (f & NB_PY) ? 1.f : 0.f;
+            cz = (f & NB_PZ) ? 1.f : 0.f;
         }
     }
     const float s = *scale;
     const float inv = s > 0.f ? 1.f / s : 0.f;
-    bf16x8 o = {(__bf16)(val * inv), (__bf16)fl, 0, 0, 0, 0, 0, 0};
+    bf16x8 o = {(__bf16)(val * inv), (__bf16)fl, (__bf16)(dg * (1.f / 6.f)), (__bf16)cx, (__bf16)cy, (__bf16)cz, 0, 0};
+    *reinterpret_cast<bf16x8*>(x0 + v * kInCh) = o;
+}
+// ... or from flat arrays of b and an operator (diag, cx, cy, cz) times norm
+__global__ void k_net_input_op(const float* __restrict__ b, const float* __restrict__ diag, const float* __restrict__ ox,
+                               const float* __restrict__ oy, const float* __restrict__ oz, float norm,
+                               const float* __restrict__ scale, __bf16* __restrict__ x0, Dims nd, int nx, int ny, int nz) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nd.V()) return;
+    const int z = (int)(v % nd.W), y = (int)((v / nd.W) % nd.H), x = (int)(v / ((int64_t)nd.W * nd.H));
+    float val = 0.f, fl = 0.f, dg = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    if (x < nx && y < ny && z < nz) {
+        const int64_t i = ((int64_t)x * ny + y) * nz + z;
+        if (diag[i] > 0.f) {
+            fl = 1.f;
+            val = b[i];
+            dg = diag[i] * norm;
+            cx = ox[i] * norm;
+            cy = oy[i] * norm;
+            cz = oz[i] * norm;
+        }
+    }
+    const float s = *scale;
+    const float inv = s > 0.f ? 1.f / s : 0.f;
+    bf16x8 o = {(__bf16)(val * inv), (__bf16)fl, (__bf16)(dg * (1.f / 6.f)), (__bf16)cx, (__bf16)cy, (__bf16)cz, 0, 0};
     *reinterpret_cast<bf16x8*>(x0 + v * kInCh) = o;
 }
 
@@ -415,6 +441,14 @@ __global__ void k_absmax(const float* __restrict__ r, const uint16_t* __restrict
     unsigned m = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         if (nbinfo[i] & NB_FLUID) m = max(m, __float_as_uint(fabsf(r[i])));
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (unsigned)__shfl_down((int)m, d, 64));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+__global__ void k_absmax_op(const float* __restrict__ b, const float* __restrict__ diag, int64_t n,
+                            unsigned* __restrict__ out) {
+    unsigned m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (diag[i] > 0.f) m = max(m, __float_as_uint(fabsf(b[i])));
     for (int d = 32; d >= 1; d >>= 1) m = max(m, (unsigned)__shfl_down((int)m, d, 64));
     if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
 }
@@ -429,9 +463,10 @@ __global__ void k_conv1_fwd(const __bf16* __restrict__ d0, const float* __restri
     out[v] = s;
 }
 
-// z (solver layout) = WATER ? out * s : 0
+// z (solver layout) = WATER ? out * s * mult : 0
 __global__ void k_net_output(const float* __restrict__ out, const uint16_t* __restrict__ nbinfo,
-                             const float* __restrict__ scale, float* __restrict__ z, Dims nd, int nx, int ny, int nz) {
+                             const float* __restrict__ scale, float mult, float* __restrict__ z, Dims nd, int nx, int ny,
+                             int nz) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nc = (int64_t)nx * ny * nz;
     if (i >= nc) return;
@@ -439,7 +474,17 @@ __global__ void k_net_output(const float* __restrict__ out, const uint16_t* __re
     const int zc = (int)(i % nz), yc = (int)((i / nz) % ny), xc = (int)(i / ((int64_t)nz * ny));
     const int64_t ri = ridx(xc, yc, zc, nx, ny, nz);
     const int64_t v = ((int64_t)xc * nd.H + yc) * nd.W + zc;
-    z[ri] = (nbinfo[ri] & NB_FLUID) ? out[v] * *scale : 0.f;
+    if (nbinfo[ri] & NB_FLUID) z[ri] = out[v] * *scale * mult;
+}
+// x (flat level) = unknown ? out * s * mult : 0
+__global__ void k_net_output_op(const float* __restrict__ out, const float* __restrict__ diag,
+                                const float* __restrict__ scale, float mult, float* __restrict__ x, Dims nd, int nx,
+                                int ny, int nz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nx * ny * nz) return;
+    const int zc = (int)(i % nz), yc = (int)((i / nz) % ny), xc = (int)(i / ((int64_t)nz * ny));
+    const int64_t v = ((int64_t)xc * nd.H + yc) * nd.W + zc;
+    x[i] = diag[i] > 0.f ? out[v] * *scale * mult : 0.f;
 }
 
 // ---- backward pieces (training) ----------------------------------------------------------
@@ -639,35 +684,34 @@ __global__ void k_mse(const float* __restrict__ out, const float* __restrict__ t
     if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0] * inv_n;
 }
 
-// Energy-norm loss: loss = e^T A e / t^T A t with e = (out - t) on WATER cells,
-// dout = 2 A e / t^T A t.  A is the unscaled 7-point pressure matrix on WATER
-// cells (diagonal = cnt, the number of non-SOLID neighbours; -1 towards WATER
-// neighbours; AIR neighbours are Dirichlet zeros).  Step 1: e into a buffer.
+// Energy-norm loss: loss = e^T A e / t^T A t with e = (out - t) on the
+// sample's unknowns, dout = 2 A e / t^T A t.  A is the sample's operator in
+// network layout: diagonal aop[0], couplings -aop[1..3] to the +x, +y, +z
+// neighbour (symmetric).  Step 1: e into a buffer.
 __global__ void k_energy_err(const float* __restrict__ out, const float* __restrict__ target,
                              const uint8_t* __restrict__ fluid, int64_t V, float* __restrict__ e) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v < V) e[v] = fluid[v] ? out[v] - target[v] : 0.f;
 }
-// Step 2: A e (e is zero off WATER, so all six neighbours may be summed).
-__global__ void k_energy(const float* __restrict__ e, const uint8_t* __restrict__ cnt, Dims d, float inv_et,
+// Step 2: A e (e is zero off the unknowns, whose couplings are zero).
+__global__ void k_energy(const float* __restrict__ e, const float* __restrict__ aop, Dims d, float inv_et,
                          float* __restrict__ dout, float* __restrict__ loss_parts) {
     __shared__ float red[256];
     const int64_t V = d.V();
+    const float *dg = aop, *cx = aop + V, *cy = aop + 2 * V, *cz = aop + 3 * V;
     float s = 0.f;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
-        const int c = cnt[v];
         float ae = 0.f;
-        if (c) {
+        if (dg[v] > 0.f) {
             const int z = (int)(v % d.W), y = (int)((v / d.W) % d.H), x = (int)(v / ((int64_t)d.W * d.H));
             const int64_t sx = (int64_t)d.H * d.W;
-            float nb = 0.f;
-            if (x > 0) nb += e[v - sx];
-            if (x < d.D - 1) nb += e[v + sx];
-            if (y > 0) nb += e[v - d.W];
-            if (y < d.H - 1) nb += e[v + d.W];
-            if (z > 0) nb += e[v - 1];
-            if (z < d.W - 1) nb += e[v + 1];
-            ae = (float)c * e[v] - nb;
+            ae = dg[v] * e[v];
+            if (x > 0) ae -= cx[v - sx] * e[v - sx];
+            if (x < d.D - 1) ae -= cx[v] * e[v + sx];
+            if (y > 0) ae -= cy[v - d.W] * e[v - d.W];
+            if (y < d.H - 1) ae -= cy[v] * e[v + d.W];
+            if (z > 0) ae -= cz[v - 1] * e[v - 1];
+            if (z < d.W - 1) ae -= cz[v] * e[v + 1];
         }
         dout[v] = 2.f * ae * inv_et;
         s += e[v] * ae;
@@ -699,110 +743,148 @@ __global__ void k_to_bf16(const float* __restrict__ a, __bf16* __restrict__ b, i
 }
 
 // ---- host side ----------------------------------------------------------------------------
+// Parameter-sized buffers are allocated once per context; the activation and
+// training buffers are sized for the grid the network runs on (the coarsest
+// multigrid level of a solve, or the context's own grid for training
+// samples) and are reallocated when that grid changes.
 struct LearnedNet {
     int64_t woff[kNetLayers], boff[kNetLayers], out_w = 0, out_b = 0, nparams = 0;
     bool has_weights = false;
     int adam_t = 0;
+    int gx = -1, gy = -1, gz = -1;   // the grid the per-size buffers are for
     Dims lv[4];
     float *w32 = nullptr, *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr, *grad_acc = nullptr;
     __bf16* w16 = nullptr;
-    // forward
-    __bf16 *x0 = nullptr, *e[4] = {}, *pl[3] = {}, *cat[3] = {}, *dd[3] = {};   // cat/dd index = level
-    float *out = nullptr, *scale = nullptr;
-    unsigned* scale_bits = nullptr;
-    // training
-    bool train_ready = false;
-    float *in_flat = nullptr, *target = nullptr, *dout = nullptr, *dd0 = nullptr, *loss_parts = nullptr;
-    uint8_t *label_flat = nullptr, *fluid = nullptr;
-    uint8_t* cnt = nullptr;          // energy loss: non-SOLID neighbour count of WATER voxels, 0 elsewhere
-    float* eres = nullptr;           // energy loss: out - target on WATER
-    double energy_t = 0.0;           // energy loss: t^T A t of the uploaded sample
-    float* g32[kNetLayers] = {};
-    __bf16* g16[kNetLayers] = {};
-    float *dcat[3] = {}, *dcoarse[3] = {}, *dskip[3] = {}, *dpool[3] = {};
     __bf16* wflip = nullptr;
     float* wparts = nullptr;
     int64_t wparts_cap = 0;
-    std::vector<void*> allocs;
+    float* loss_parts = nullptr;
+    bool train_params_ready = false;
+    // forward (per size)
+    __bf16 *x0 = nullptr, *e[4] = {}, *pl[3] = {}, *cat[3] = {}, *dd[3] = {};   // cat/dd index = level
+    float *out = nullptr, *scale = nullptr;
+    unsigned* scale_bits = nullptr;
+    // training (per size)
+    bool train_ready = false;
+    float *in_b = nullptr, *in_op = nullptr, *target = nullptr, *dout = nullptr, *dd0 = nullptr;
+    uint8_t* fluid = nullptr;
+    float* aop = nullptr;             // the sample's operator in network layout (energy loss)
+    float* eres = nullptr;            // energy loss: out - target on the unknowns
+    double energy_t = 0.0;            // energy loss: t^T A t of the uploaded sample
+    float* g32[kNetLayers] = {};
+    __bf16* g16[kNetLayers] = {};
+    float *dcat[3] = {}, *dcoarse[3] = {}, *dskip[3] = {}, *dpool[3] = {};
+    std::vector<void*> pallocs, allocs;   // parameter-sized, per-size
 };
 
 template <typename T>
-static int nalloc(fsim_ctx* ctx, LearnedNet* N, T** p, int64_t n) {
+static int nalloc(fsim_ctx* ctx, std::vector<void*>& list, T** p, int64_t n) {
     if (hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess)
         return set_err(ctx, FSIM_ERR_OOM, "learned preconditioner: device allocation failed");
-    N->allocs.push_back(*p);
+    list.push_back(*p);
     return FSIM_OK;
+}
+
+static void free_sized(LearnedNet* N) {
+    for (void* p : N->allocs) (void)hipFree(p);
+    N->allocs.clear();
+    N->gx = N->gy = N->gz = -1;
+    N->train_ready = false;
 }
 
 void learned_free(fsim_ctx* ctx) {
     LearnedNet* N = ctx->learned;
     if (!N) return;
-    for (void* p : N->allocs) (void)hipFree(p);
+    free_sized(N);
+    for (void* p : N->pallocs) (void)hipFree(p);
     delete N;
     ctx->learned = nullptr;
 }
 
 static unsigned gb(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-static int learned_alloc(fsim_ctx* ctx) {
+static int learned_alloc_params(fsim_ctx* ctx) {
     if (ctx->learned) return FSIM_OK;
     LearnedNet* N = new LearnedNet();
     ctx->learned = N;
     param_offsets(N->woff, N->boff, &N->out_w, &N->out_b, &N->nparams);
-    auto r8 = [](int n) { return (n + 7) / 8 * 8; };
-    N->lv[0] = Dims{r8(ctx->g.nx), r8(ctx->g.ny), r8(ctx->g.nz)};
-    for (int l = 1; l < 4; l++) N->lv[l] = Dims{N->lv[l - 1].D / 2, N->lv[l - 1].H / 2, N->lv[l - 1].W / 2};
-    const int64_t V0 = N->lv[0].V(), V1 = N->lv[1].V(), V2 = N->lv[2].V(), V3 = N->lv[3].V();
     int rc;
-    if ((rc = nalloc(ctx, N, &N->w32, N->nparams)) || (rc = nalloc(ctx, N, &N->w16, N->nparams)) ||
-        (rc = nalloc(ctx, N, &N->x0, V0 * kInCh)) || (rc = nalloc(ctx, N, &N->e[0], V0 * 16)) ||
-        (rc = nalloc(ctx, N, &N->e[1], V1 * 32)) || (rc = nalloc(ctx, N, &N->e[2], V2 * 64)) ||
-        (rc = nalloc(ctx, N, &N->e[3], V3 * 128)) || (rc = nalloc(ctx, N, &N->pl[0], V1 * 16)) ||
-        (rc = nalloc(ctx, N, &N->pl[1], V2 * 32)) || (rc = nalloc(ctx, N, &N->pl[2], V3 * 64)) ||
-        (rc = nalloc(ctx, N, &N->dd[2], V2 * 64)) ||
-        (rc = nalloc(ctx, N, &N->dd[1], V1 * 32)) || (rc = nalloc(ctx, N, &N->dd[0], V0 * 16)) ||
-        (rc = nalloc(ctx, N, &N->out, V0)) || (rc = nalloc(ctx, N, &N->scale, 1)) ||
-        (rc = nalloc(ctx, N, &N->scale_bits, 1)))
+    if ((rc = nalloc(ctx, N->pallocs, &N->w32, N->nparams)) || (rc = nalloc(ctx, N->pallocs, &N->w16, N->nparams)))
         return rc;
     return FSIM_OK;
 }
 
-static int learned_alloc_train(fsim_ctx* ctx) {
+// forward buffers for a network running on an nx x ny x nz grid (padded to multiples of 8)
+static int learned_alloc(fsim_ctx* ctx, int nx, int ny, int nz) {
+    FSIM_TRY(learned_alloc_params(ctx));
     LearnedNet* N = ctx->learned;
+    if (N->gx == nx && N->gy == ny && N->gz == nz) return FSIM_OK;
+    FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_sized(N);
+    auto r8 = [](int n) { return (n + 7) / 8 * 8; };
+    N->lv[0] = Dims{r8(nx), r8(ny), r8(nz)};
+    for (int l = 1; l < 4; l++) N->lv[l] = Dims{N->lv[l - 1].D / 2, N->lv[l - 1].H / 2, N->lv[l - 1].W / 2};
+    const int64_t V0 = N->lv[0].V(), V1 = N->lv[1].V(), V2 = N->lv[2].V(), V3 = N->lv[3].V();
+    std::vector<void*>& A = N->allocs;
+    int rc;
+    if ((rc = nalloc(ctx, A, &N->x0, V0 * kInCh)) || (rc = nalloc(ctx, A, &N->e[0], V0 * 16)) ||
+        (rc = nalloc(ctx, A, &N->e[1], V1 * 32)) || (rc = nalloc(ctx, A, &N->e[2], V2 * 64)) ||
+        (rc = nalloc(ctx, A, &N->e[3], V3 * 128)) || (rc = nalloc(ctx, A, &N->pl[0], V1 * 16)) ||
+        (rc = nalloc(ctx, A, &N->pl[1], V2 * 32)) || (rc = nalloc(ctx, A, &N->pl[2], V3 * 64)) ||
+        (rc = nalloc(ctx, A, &N->dd[2], V2 * 64)) || (rc = nalloc(ctx, A, &N->dd[1], V1 * 32)) ||
+        (rc = nalloc(ctx, A, &N->dd[0], V0 * 16)) || (rc = nalloc(ctx, A, &N->out, V0)) ||
+        (rc = nalloc(ctx, A, &N->scale, 1)) || (rc = nalloc(ctx, A, &N->scale_bits, 1)))
+        return rc;
+    N->gx = nx;
+    N->gy = ny;
+    N->gz = nz;
+    return FSIM_OK;
+}
+
+// training buffers for samples on the context's own grid
+static int learned_alloc_train(fsim_ctx* ctx) {
+    FSIM_TRY(learned_alloc(ctx, ctx->g.nx, ctx->g.ny, ctx->g.nz));
+    LearnedNet* N = ctx->learned;
+    int rc;
+    if (!N->train_params_ready) {
+        std::vector<void*>& P = N->pallocs;
+        if ((rc = nalloc(ctx, P, &N->adam_m, N->nparams)) || (rc = nalloc(ctx, P, &N->adam_v, N->nparams)) ||
+            (rc = nalloc(ctx, P, &N->grad, N->nparams)) || (rc = nalloc(ctx, P, &N->grad_acc, N->nparams)) ||
+            (rc = nalloc(ctx, P, &N->loss_parts, 1024)) || (rc = nalloc(ctx, P, &N->wflip, 64 * 27 * 192)))
+            return rc;
+        N->wparts_cap = 64LL * 64 * 27 * 192;
+        if ((rc = nalloc(ctx, P, &N->wparts, N->wparts_cap))) return rc;
+        FSIM_HIP(ctx, hipMemsetAsync(N->adam_m, 0, sizeof(float) * N->nparams, ctx->stream));
+        FSIM_HIP(ctx, hipMemsetAsync(N->adam_v, 0, sizeof(float) * N->nparams, ctx->stream));
+        N->adam_t = 0;
+        N->train_params_ready = true;
+    }
     if (N->train_ready) return FSIM_OK;
     const int64_t V0 = N->lv[0].V(), V1 = N->lv[1].V(), V2 = N->lv[2].V(), V3 = N->lv[3].V();
     const int64_t Vl[4] = {V0, V1, V2, V3};
-    int rc;
-    if ((rc = nalloc(ctx, N, &N->adam_m, N->nparams)) || (rc = nalloc(ctx, N, &N->adam_v, N->nparams)) ||
-        (rc = nalloc(ctx, N, &N->grad, N->nparams)) || (rc = nalloc(ctx, N, &N->grad_acc, N->nparams)) ||
-        (rc = nalloc(ctx, N, &N->in_flat, ctx->g.nc)) || (rc = nalloc(ctx, N, &N->label_flat, ctx->g.nc)) ||
-        (rc = nalloc(ctx, N, &N->target, V0)) || (rc = nalloc(ctx, N, &N->fluid, V0)) ||
-        (rc = nalloc(ctx, N, &N->dout, V0)) || (rc = nalloc(ctx, N, &N->dd0, V0 * 16)) ||
-        (rc = nalloc(ctx, N, &N->loss_parts, 1024)) || (rc = nalloc(ctx, N, &N->wflip, 64 * 27 * 192)) ||
-        (rc = nalloc(ctx, N, &N->cat[2], V2 * 192)) || (rc = nalloc(ctx, N, &N->cat[1], V1 * 96)) ||
-        (rc = nalloc(ctx, N, &N->cat[0], V0 * 48)) || (rc = nalloc(ctx, N, &N->cnt, V0)) ||
-        (rc = nalloc(ctx, N, &N->eres, V0)))
+    std::vector<void*>& A = N->allocs;
+    if ((rc = nalloc(ctx, A, &N->in_b, ctx->g.nc)) || (rc = nalloc(ctx, A, &N->in_op, 4 * ctx->g.nc)) ||
+        (rc = nalloc(ctx, A, &N->target, V0)) || (rc = nalloc(ctx, A, &N->fluid, V0)) ||
+        (rc = nalloc(ctx, A, &N->dout, V0)) || (rc = nalloc(ctx, A, &N->dd0, V0 * 16)) ||
+        (rc = nalloc(ctx, A, &N->cat[2], V2 * 192)) || (rc = nalloc(ctx, A, &N->cat[1], V1 * 96)) ||
+        (rc = nalloc(ctx, A, &N->cat[0], V0 * 48)) || (rc = nalloc(ctx, A, &N->aop, 4 * V0)) ||
+        (rc = nalloc(ctx, A, &N->eres, V0)))
         return rc;
     for (int l = 0; l < kNetLayers; l++) {
         const int64_t n = Vl[kLayers[l].level] * kLayers[l].cout;
-        if ((rc = nalloc(ctx, N, &N->g32[l], n)) || (rc = nalloc(ctx, N, &N->g16[l], n))) return rc;
+        if ((rc = nalloc(ctx, A, &N->g32[l], n)) || (rc = nalloc(ctx, A, &N->g16[l], n))) return rc;
     }
     // concat data gradients at levels 2, 1, 0 and their splits
     const int cu[3] = {32, 64, 128}, cs[3] = {16, 32, 64};   // index = level
     for (int l = 0; l < 3; l++) {
-        if ((rc = nalloc(ctx, N, &N->dcat[l], Vl[l] * (cu[l] + cs[l]))) ||
-            (rc = nalloc(ctx, N, &N->dcoarse[l], Vl[l + 1] * cu[l])) || (rc = nalloc(ctx, N, &N->dskip[l], Vl[l] * cs[l])))
+        if ((rc = nalloc(ctx, A, &N->dcat[l], Vl[l] * (cu[l] + cs[l]))) ||
+            (rc = nalloc(ctx, A, &N->dcoarse[l], Vl[l + 1] * cu[l])) || (rc = nalloc(ctx, A, &N->dskip[l], Vl[l] * cs[l])))
             return rc;
     }
     // data gradients of the pooled inputs: dpool[l] at level l + 1 with the channels of e[l]
     const int pc[3] = {16, 32, 64};
     for (int l = 0; l < 3; l++)
-        if ((rc = nalloc(ctx, N, &N->dpool[l], Vl[l + 1] * pc[l]))) return rc;
-    N->wparts_cap = 64LL * 64 * 27 * 192;
-    if ((rc = nalloc(ctx, N, &N->wparts, N->wparts_cap))) return rc;
-    FSIM_HIP(ctx, hipMemsetAsync(N->adam_m, 0, sizeof(float) * N->nparams, ctx->stream));
-    FSIM_HIP(ctx, hipMemsetAsync(N->adam_v, 0, sizeof(float) * N->nparams, ctx->stream));
-    N->adam_t = 0;
+        if ((rc = nalloc(ctx, A, &N->dpool[l], Vl[l + 1] * pc[l]))) return rc;
     N->train_ready = true;
     return FSIM_OK;
 }
@@ -856,14 +938,14 @@ static int sync_bf16(fsim_ctx* ctx) {
 bool learned_ready(const fsim_ctx* ctx) { return ctx->learned && ctx->learned->has_weights; }
 
 int learned_set_params(fsim_ctx* ctx, const float* params) {
-    FSIM_TRY(learned_alloc(ctx));
+    FSIM_TRY(learned_alloc_params(ctx));
     LearnedNet* N = ctx->learned;
     FSIM_HIP(ctx, hipMemcpyAsync(N->w32, params, sizeof(float) * N->nparams, hipMemcpyHostToDevice, ctx->stream));
     FSIM_TRY(sync_bf16(ctx));
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     N->has_weights = true;
     N->adam_t = 0;
-    if (N->train_ready) {
+    if (N->train_params_ready) {
         FSIM_HIP(ctx, hipMemsetAsync(N->adam_m, 0, sizeof(float) * N->nparams, ctx->stream));
         FSIM_HIP(ctx, hipMemsetAsync(N->adam_v, 0, sizeof(float) * N->nparams, ctx->stream));
     }
@@ -900,8 +982,30 @@ int learned_init(fsim_ctx* ctx, uint64_t seed) {
     return learned_set_params(ctx, p.data());
 }
 
-// z = M(r) in the solver layout (r, z: rotated FP32 vectors of the context)
-int learned_apply(fsim_ctx* ctx, const float* r, float* z) {
+// The network on a coarse multigrid level (flat arrays): x = unknown ? out * max|b| * norm : 0
+int learned_apply_level(fsim_ctx* ctx, int nx, int ny, int nz, const float* b, const float* diag, const float* ox,
+                        const float* oy, const float* oz, double norm, float* x) {
+    FSIM_PROF(ctx, KID_LEARNED);
+    FSIM_TRY(learned_alloc(ctx, nx, ny, nz));
+    LearnedNet* N = ctx->learned;
+    hipStream_t st = ctx->stream;
+    const int64_t nc = (int64_t)nx * ny * nz;
+    FSIM_HIP(ctx, hipMemsetAsync(N->scale_bits, 0, sizeof(unsigned), st));
+    hipLaunchKernelGGL(k_absmax_op, dim3(std::min<unsigned>(gb(nc), 2048u)), dim3(256), 0, st, b, diag, nc, N->scale_bits);
+    const float* sc = reinterpret_cast<const float*>(N->scale_bits);   // the max as a float
+    hipLaunchKernelGGL(k_net_input_op, dim3(gb(N->lv[0].V())), dim3(256), 0, st, b, diag, ox, oy, oz, (float)norm, sc,
+                       N->x0, N->lv[0], nx, ny, nz);
+    FSIM_TRY(net_forward(ctx, false));
+    hipLaunchKernelGGL(k_net_output_op, dim3(gb(nc)), dim3(256), 0, st, N->out, diag, sc, (float)norm, x, N->lv[0], nx, ny,
+                       nz);
+    FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// The network on the solve's own grid (rotated r and z): z = WATER ? out * max|r| * binv : 0
+int learned_apply_level0(fsim_ctx* ctx, const float* r, float* z, float binv) {
+    FSIM_PROF(ctx, KID_LEARNED);
+    FSIM_TRY(learned_alloc(ctx, ctx->g.nx, ctx->g.ny, ctx->g.nz));
     LearnedNet* N = ctx->learned;
     hipStream_t st = ctx->stream;
     const int64_t nc = ctx->g.nc;
@@ -909,72 +1013,97 @@ int learned_apply(fsim_ctx* ctx, const float* r, float* z) {
     hipLaunchKernelGGL(k_absmax, dim3(std::min<unsigned>(gb(nc), 2048u)), dim3(256), 0, st, r, ctx->nbinfo, nc,
                        N->scale_bits);
     static_assert(sizeof(unsigned) == sizeof(float), "scale bits");
-    const float* sc = reinterpret_cast<const float*>(N->scale_bits);   // the max as a float
-    hipLaunchKernelGGL(k_net_input, dim3(gb(N->lv[0].V())), dim3(256), 0, st, r, ctx->nbinfo, 1, nullptr, sc, N->x0,
-                       N->lv[0], ctx->g.nx, ctx->g.ny, ctx->g.nz);
+    const float* sc = reinterpret_cast<const float*>(N->scale_bits);
+    hipLaunchKernelGGL(k_net_input, dim3(gb(N->lv[0].V())), dim3(256), 0, st, r, ctx->nbinfo, sc, N->x0, N->lv[0],
+                       ctx->g.nx, ctx->g.ny, ctx->g.nz);
     FSIM_TRY(net_forward(ctx, false));
-    hipLaunchKernelGGL(k_net_output, dim3(gb(nc)), dim3(256), 0, st, N->out, ctx->nbinfo, sc, z, N->lv[0], ctx->g.nx,
+    hipLaunchKernelGGL(k_net_output, dim3(gb(nc)), dim3(256), 0, st, N->out, ctx->nbinfo, sc, binv, z, N->lv[0], ctx->g.nx,
                        ctx->g.ny, ctx->g.nz);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
 
-// Uploads one sample given in flat grid order: input (b / s, WATER mask), and for training the
-// target p / s and the WATER mask in network order.  s = max |b| over WATER cells.
-static int upload_sample(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* s_out) {
+// The 7-point operator of a label field (flat order): diag = non-SOLID
+// neighbours of a WATER cell (cells outside the grid count as SOLID),
+// couplings 1 towards a WATER +x / +y / +z neighbour.
+void label_operator(int nx, int ny, int nz, const uint8_t* label, float* op) {
+    const int64_t nc = (int64_t)nx * ny * nz;
+    auto lab = [&](int x, int y, int z) -> int {
+        if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return FSIM_SOLID;
+        return label[((int64_t)x * ny + y) * nz + z];
+    };
+    static const int nb6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                const int64_t i = ((int64_t)x * ny + y) * nz + z;
+                float d = 0.f, c[3] = {0.f, 0.f, 0.f};
+                if (label[i] == FSIM_WATER) {
+                    for (const auto& o : nb6) d += lab(x + o[0], y + o[1], z + o[2]) != FSIM_SOLID ? 1.f : 0.f;
+                    c[0] = lab(x + 1, y, z) == FSIM_WATER ? 1.f : 0.f;
+                    c[1] = lab(x, y + 1, z) == FSIM_WATER ? 1.f : 0.f;
+                    c[2] = lab(x, y, z + 1) == FSIM_WATER ? 1.f : 0.f;
+                }
+                op[i] = d;
+                op[nc + i] = c[0];
+                op[2 * nc + i] = c[1];
+                op[3 * nc + i] = c[2];
+            }
+}
+
+// Uploads one sample in flat grid order: input (b / s, the operator's
+// channels), and for training the target p / s, the unknown mask and the
+// operator in network order.  s = max |b| over the unknowns (diag > 0).
+static int upload_sample(fsim_ctx* ctx, const float* b, const float* op, const double* p, float* s_out) {
     LearnedNet* N = ctx->learned;
     const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
     const int64_t nc = ctx->g.nc;
     float s = 0.f;
     for (int64_t i = 0; i < nc; i++)
-        if (label[i] == FSIM_WATER) s = std::max(s, std::fabs(b[i]));
+        if (op[i] > 0.f) s = std::max(s, std::fabs(b[i]));
     *s_out = s;
-    FSIM_HIP(ctx, hipMemcpyAsync(N->in_flat, b, sizeof(float) * nc, hipMemcpyHostToDevice, ctx->stream));
-    FSIM_HIP(ctx, hipMemcpyAsync(N->label_flat, label, nc, hipMemcpyHostToDevice, ctx->stream));
+    FSIM_HIP(ctx, hipMemcpyAsync(N->in_b, b, sizeof(float) * nc, hipMemcpyHostToDevice, ctx->stream));
+    FSIM_HIP(ctx, hipMemcpyAsync(N->in_op, op, sizeof(float) * 4 * nc, hipMemcpyHostToDevice, ctx->stream));
     FSIM_HIP(ctx, hipMemcpyAsync(N->scale, &s, sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_net_input, dim3(gb(N->lv[0].V())), dim3(256), 0, ctx->stream, N->in_flat, nullptr, 0,
-                       N->label_flat, N->scale, N->x0, N->lv[0], nx, ny, nz);
+    hipLaunchKernelGGL(k_net_input_op, dim3(gb(N->lv[0].V())), dim3(256), 0, ctx->stream, N->in_b, N->in_op,
+                       N->in_op + nc, N->in_op + 2 * nc, N->in_op + 3 * nc, 1.0f, N->scale, N->x0, N->lv[0], nx, ny, nz);
     FSIM_HIP(ctx, hipGetLastError());
     if (p) {
         const Dims d = N->lv[0];
-        std::vector<float> t((size_t)d.V(), 0.f);
-        std::vector<uint8_t> f((size_t)d.V(), 0), cn((size_t)d.V(), 0);
+        const int64_t V = d.V();
+        std::vector<float> t((size_t)V, 0.f), a((size_t)(4 * V), 0.f);
+        std::vector<uint8_t> f((size_t)V, 0);
         const float inv = s > 0.f ? 1.f / s : 0.f;
-        auto lab = [&](int x, int y, int z) -> int {   // cells outside the grid count as SOLID
-            if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return FSIM_SOLID;
-            return label[((int64_t)x * ny + y) * nz + z];
-        };
-        static const int nb6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
         for (int x = 0; x < nx; x++)
             for (int y = 0; y < ny; y++)
                 for (int z = 0; z < nz; z++) {
                     const int64_t i = ((int64_t)x * ny + y) * nz + z, v = ((int64_t)x * d.H + y) * d.W + z;
-                    if (label[i] != FSIM_WATER) continue;
+                    if (!(op[i] > 0.f)) continue;
                     f[(size_t)v] = 1;
                     t[(size_t)v] = (float)p[i] * inv;
-                    int c = 0;
-                    for (const auto& o : nb6) c += lab(x + o[0], y + o[1], z + o[2]) != FSIM_SOLID;
-                    cn[(size_t)v] = (uint8_t)c;
+                    for (int k = 0; k < 4; k++) a[(size_t)(k * V + v)] = op[k * nc + i];
                 }
-        // t^T A t in FP64 over the WATER cells
+        // t^T A t in FP64 over the unknowns
         double et = 0.0;
+        const int64_t sx = (int64_t)d.H * d.W, sy = d.W;
         for (int x = 0; x < nx; x++)
             for (int y = 0; y < ny; y++)
                 for (int z = 0; z < nz; z++) {
                     const int64_t v = ((int64_t)x * d.H + y) * d.W + z;
                     if (!f[(size_t)v]) continue;
-                    double at = (double)cn[(size_t)v] * t[(size_t)v];
-                    for (const auto& o : nb6) {
-                        const int xx = x + o[0], yy = y + o[1], zz = z + o[2];
-                        if (lab(xx, yy, zz) == FSIM_WATER) at -= t[(size_t)(((int64_t)xx * d.H + yy) * d.W + zz)];
-                    }
+                    double at = (double)a[(size_t)v] * t[(size_t)v];
+                    if (x > 0) at -= (double)a[(size_t)(V + v - sx)] * t[(size_t)(v - sx)];
+                    if (x < d.D - 1) at -= (double)a[(size_t)(V + v)] * t[(size_t)(v + sx)];
+                    if (y > 0) at -= (double)a[(size_t)(2 * V + v - sy)] * t[(size_t)(v - sy)];
+                    if (y < d.H - 1) at -= (double)a[(size_t)(2 * V + v)] * t[(size_t)(v + sy)];
+                    if (z > 0) at -= (double)a[(size_t)(3 * V + v - 1)] * t[(size_t)(v - 1)];
+                    if (z < d.W - 1) at -= (double)a[(size_t)(3 * V + v)] * t[(size_t)(v + 1)];
                     et += (double)t[(size_t)v] * at;
                 }
         N->energy_t = et;
         FSIM_HIP(ctx, hipMemcpyAsync(N->target, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, ctx->stream));
         FSIM_HIP(ctx, hipMemcpyAsync(N->fluid, f.data(), f.size(), hipMemcpyHostToDevice, ctx->stream));
-        if (N->cnt)
-            FSIM_HIP(ctx, hipMemcpyAsync(N->cnt, cn.data(), cn.size(), hipMemcpyHostToDevice, ctx->stream));
+        FSIM_HIP(ctx, hipMemcpyAsync(N->aop, a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, ctx->stream));
         // the host vectors go out of scope: wait for the copies
         FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
@@ -982,12 +1111,12 @@ static int upload_sample(fsim_ctx* ctx, const float* b, const uint8_t* label, co
 }
 
 // network output for one flat-order sample (raw network output, flat grid order)
-int learned_forward_host(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out) {
+int learned_forward_host(fsim_ctx* ctx, const float* b, const float* op, float* out) {
     if (!learned_ready(ctx)) return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner has no parameters");
-    LearnedNet* N = ctx->learned;
     FSIM_TRY(learned_alloc_train(ctx));
+    LearnedNet* N = ctx->learned;
     float s;
-    FSIM_TRY(upload_sample(ctx, b, label, nullptr, &s));
+    FSIM_TRY(upload_sample(ctx, b, op, nullptr, &s));
     FSIM_TRY(net_forward(ctx, false));
     const Dims d = N->lv[0];
     std::vector<float> o((size_t)d.V());
@@ -1066,7 +1195,7 @@ static int net_backward(fsim_ctx* ctx, double* loss) {
     if (ctx->learned_loss == 1) {
         const float inv_et = N->energy_t > 0.0 ? (float)(1.0 / N->energy_t) : 0.f;
         hipLaunchKernelGGL(k_energy_err, dim3(gb(V0)), dim3(256), 0, st, N->out, N->target, N->fluid, V0, N->eres);
-        hipLaunchKernelGGL(k_energy, dim3(nlb), dim3(256), 0, st, N->eres, N->cnt, L[0], inv_et, N->dout, N->loss_parts);
+        hipLaunchKernelGGL(k_energy, dim3(nlb), dim3(256), 0, st, N->eres, N->aop, L[0], inv_et, N->dout, N->loss_parts);
     } else {
         hipLaunchKernelGGL(k_mse, dim3(nlb), dim3(256), 0, st, N->out, N->target, N->fluid, V0, inv_n, N->dout,
                            N->loss_parts);
@@ -1114,12 +1243,12 @@ static int net_backward(fsim_ctx* ctx, double* loss) {
     return FSIM_OK;
 }
 
-int learned_grad_host(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad, double* loss) {
+int learned_grad_host(fsim_ctx* ctx, const float* b, const float* op, const double* p, float* grad, double* loss) {
     if (!learned_ready(ctx)) return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner has no parameters");
     FSIM_TRY(learned_alloc_train(ctx));
     LearnedNet* N = ctx->learned;
     float s;
-    FSIM_TRY(upload_sample(ctx, b, label, p, &s));
+    FSIM_TRY(upload_sample(ctx, b, op, p, &s));
     FSIM_TRY(net_forward(ctx, true));
     FSIM_TRY(net_backward(ctx, loss));
     FSIM_HIP(ctx, hipMemcpyAsync(grad, N->grad, sizeof(float) * N->nparams, hipMemcpyDeviceToHost, ctx->stream));
@@ -1132,8 +1261,8 @@ __global__ void k_axpy_f(float* __restrict__ y, const float* __restrict__ x, int
     if (i < n) y[i] += x[i];
 }
 
-// One Adam step on the mean gradient of n samples (flat grid order, stride nc each).
-int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
+// One Adam step on the mean gradient of n samples (flat grid order, stride nc each, operators 4 nc each).
+int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const float* op, const double* p, float lr,
                         double* mean_loss) {
     if (!learned_ready(ctx)) return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner has no parameters");
     if (n <= 0) return set_err(ctx, FSIM_ERR_INVALID, "empty training batch");
@@ -1145,7 +1274,7 @@ int learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* lab
     for (int k = 0; k < n; k++) {
         float s;
         double l;
-        FSIM_TRY(upload_sample(ctx, b + k * nc, label + k * nc, p + k * nc, &s));
+        FSIM_TRY(upload_sample(ctx, b + k * nc, op + 4 * k * nc, p + k * nc, &s));
         FSIM_TRY(net_forward(ctx, true));
         FSIM_TRY(net_backward(ctx, &l));
         lsum += l;

@@ -27,49 +27,6 @@
 
 namespace fsim {
 
-struct SolveArgs {
-    int nx, ny, nz;
-    int off63;          // 63 mod nx: the plane shift of a z step across a 64-wide z-tile edge
-    int64_t nc;
-    float scale_f;      // dt / (rho dx^2)
-    double scale;
-};
-
-// flat index of the grid fields (labels, velocities, density)
-__device__ inline int64_t sidx(const SolveArgs& S, int x, int y, int z) { return ((int64_t)x * S.ny + y) * S.nz + z; }
-// index of the solver vectors, nbinfo and W (rotated layout)
-__device__ inline int64_t rsidx(const SolveArgs& S, int x, int y, int z) { return ridx(x, y, z, S.nx, S.ny, S.nz); }
-// Rotated indices of the six neighbours of the interior cell stored at
-// plane p, row y, column z (rotated index i): +-y are +-nz; +-x are the
-// next/previous plane; +-z move one column and one plane, except across a
-// 64-wide z-tile edge where rot(z) wraps 63 <-> 0 and the plane moves by 63.
-struct RNeighbours {
-    int64_t px, mx, py, my, pz, mz;
-};
-__device__ inline RNeighbours rneighbours(const SolveArgs& S, int64_t i, int p, int y, int z) {
-    const int64_t plane = (int64_t)S.ny * S.nz;
-    const int64_t yz = (int64_t)y * S.nz + z;
-    const int rz = rot_of(z);
-    const int pn = p + 1 == S.nx ? 0 : p + 1;
-    const int pp = p == 0 ? S.nx - 1 : p - 1;
-    RNeighbours n;
-    n.px = pn * plane + yz;
-    n.mx = pp * plane + yz;
-    n.py = i + S.nz;
-    n.my = i - S.nz;
-    const int o = S.off63;
-    n.pz = (rz == WF_TZ - 1 ? (p >= o ? p - o : p - o + S.nx) : pn) * plane + yz + 1;
-    n.mz = (rz == 0 ? (p + o >= S.nx ? p + o - S.nx : p + o) : pp) * plane + yz - 1;
-    return n;
-}
-// rotated indices of the +a and -a neighbours of interior cell (x, y, z)
-__device__ inline int64_t rnb_plus(const SolveArgs& S, int x, int y, int z, int a) {
-    return rsidx(S, x + (a == 0), y + (a == 1), z + (a == 2));
-}
-__device__ inline int64_t rnb_minus(const SolveArgs& S, int x, int y, int z, int a) {
-    return rsidx(S, x - (a == 0), y - (a == 1), z - (a == 2));
-}
-
 // ---- block reductions (FP64, fixed order) ------------------------------------
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -635,42 +592,6 @@ __global__ void __launch_bounds__(kBlock) k_update_x(int64_t n, const uint16_t* 
 }
 
 
-// z = y + omega (r - A y) / diag(A) on WATER cells (0 elsewhere), same stencil and item split as k_spmv
-constexpr float kLearnedOmega = 2.0f / 3.0f;
-__global__ void __launch_bounds__(kBlock) k_jacobi_correct(SolveArgs S, const uint16_t* __restrict__ nbinfo,
-                                                           const float* __restrict__ r, const float* __restrict__ y,
-                                                           float* __restrict__ z, float omega) {
-    const float ms = -S.scale_f;
-    const RowItems it = row_items(S);
-    const int lane = threadIdx.x & 63;
-    for (int k = it.first; k < it.end; k += it.step) {
-        int p, yy, z0;
-        item_cells(it, k, p, yy, z0);
-        const int64_t rowbase = ((int64_t)p * S.ny + yy) * S.nz;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int zc = z0 + lane + 64 * c;
-            if (zc >= S.nz - 1) continue;
-            const int64_t i = rowbase + zc;
-            const uint16_t f = nbinfo[i];
-            if (!(f & NB_FLUID)) {
-                z[i] = 0.f;
-                continue;
-            }
-            const RNeighbours n = rneighbours(S, i, p, yy, zc);
-            const float diag = S.scale_f * (float)((f >> NB_NONSOLID_SHIFT) & 7);
-            float ay = diag * y[i];
-            if (f & NB_PX) ay += ms * y[n.px];
-            if (f & NB_PY) ay += ms * y[n.py];
-            if (f & NB_PZ) ay += ms * y[n.pz];
-            if (f & NB_MX) ay += ms * y[n.mx];
-            if (f & NB_MY) ay += ms * y[n.my];
-            if (f & NB_MZ) ay += ms * y[n.mz];
-            z[i] = diag > 0.f ? y[i] + omega * (r[i] - ay) / diag : y[i];   // no non-SOLID neighbour: no correction
-        }
-    }
-}
-
 // FP64 true residual t = b - A x; optionally r = (float)t.
 __global__ void __launch_bounds__(kBlock) k_true_residual(SolveArgs S, const uint16_t* __restrict__ nbinfo,
                                                           const float* __restrict__ b, const double* __restrict__ x,
@@ -936,26 +857,12 @@ static bool stop_met(const fsim_ctx* ctx, double ss, double mx, double bnorm) {
     return mx < ctx->prm.residual_tolerance;
 }
 
-// z = M^-1 r with the solve's preconditioner: MIC(0) sweeps, or the learned
-// one: z0 = net(r), then nu damped-Jacobi steps z_{k+1} = z_k + omega D^-1
-// (r - A z_k) (nu = learned_smoothing).  The network carries the smooth part
-// of A^-1; the steps remove the high-frequency residual it leaves, which a
-// convolutional map trained on right-hand sides handles poorly.  The steps
-// alternate between y and z, starting in whichever makes the last land in z.
+// z = M^-1 r with the solve's preconditioner: the MIC(0) sweeps, or the
+// learned multigrid V-cycle (fsim_mg.hip), whose coarsest level the network
+// solves.
 static int precondition(fsim_ctx* ctx, const SolveArgs& S, bool flexible) {
     if (!flexible) return mic_apply(ctx, S, ctx->r, ctx->y, ctx->z);
-    FSIM_PROF(ctx, KID_LEARNED);
-    const int nu = ctx->learned_smoothing;
-    float* a = (nu & 1) ? ctx->y : ctx->z;
-    float* b = (nu & 1) ? ctx->z : ctx->y;
-    FSIM_TRY(learned_apply(ctx, ctx->r, a));
-    for (int k = 0; k < nu; k++) {
-        hipLaunchKernelGGL(k_jacobi_correct, dim3(red_blocks(S.nc)), dim3(kBlock), 0, ctx->stream, S, ctx->nbinfo,
-                           ctx->r, a, b, kLearnedOmega);
-        std::swap(a, b);
-    }
-    FSIM_HIP(ctx, hipGetLastError());
-    return FSIM_OK;
+    return mg_apply(ctx, ctx->r, ctx->z, S.scale);
 }
 
 // After a new z: z.r (and, flexible, z.q first), giving beta.  Plain PCG:
@@ -1192,7 +1099,7 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     st.learned_budget = 0;
     if (ctx->prm.precond == FSIM_SOLVER_BASIC_SOR) return stage_basic_sor(ctx, S);
     const bool learned = ctx->prm.precond == FSIM_PRECOND_LEARNED;
-    if (learned && !learned_ready(ctx))
+    if (learned && ctx->mg_coarse == FSIM_MG_COARSE_NETWORK && !learned_ready(ctx))
         return set_err(ctx, FSIM_ERR_INVALID, "learned preconditioner selected but it has no parameters "
                                               "(fsim_learned_init or fsim_learned_set_params)");
     FSIM_HIP(ctx, hipMemsetAsync(ctx->scal, 0, sizeof(SolveScalars), ctx->stream));
@@ -1214,8 +1121,8 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     const int max_it = ctx->prm.max_iterations;
     PcgResult res;
     bool use_mic = !learned;
-    if (learned && (ctx->last_mic_iterations <= 0 ||
-                    (ctx->learned_refresh > 0 && ctx->learned_since_mic >= ctx->learned_refresh))) {
+    if (learned && !ctx->mg_harvesting &&
+        (ctx->last_mic_iterations <= 0 || (ctx->learned_refresh > 0 && ctx->learned_since_mic >= ctx->learned_refresh))) {
         // no MIC(0) count to budget against yet (or it is old): this solve is
         // MIC(0)-PCG, which measures it
         use_mic = true;
@@ -1223,9 +1130,10 @@ int stage_solve(fsim_ctx* ctx, double dt) {
     } else if (learned) {
         // flexible PCG with the network; MIC(0)-PCG from scratch when it does not
         // reach the tolerance within twice the last MIC(0) iteration count
-        const int budget = std::min(max_it, 2 * ctx->last_mic_iterations);
+        const int budget = ctx->mg_harvesting ? max_it : std::min(max_it, 2 * ctx->last_mic_iterations);
         st.learned_budget = budget;
-        ctx->learned_since_mic++;
+        if (!ctx->mg_harvesting) ctx->learned_since_mic++;
+        FSIM_TRY(mg_setup(ctx));
         FSIM_TRY(pcg_run(ctx, S, bnorm, true, budget, &res));
         st.learned_iterations = res.iterations;
         st.learned_stop = res.converged ? 0 : res.nan ? 2 : 1;
@@ -1326,6 +1234,13 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
         FSIM_TRY(fill_f32(ctx, ctx->z, 0.0f, nc));
         FSIM_TRY(mic_apply(ctx, S, ctx->r, ctx->y, ctx->z));
         FSIM_TRY(check_wave_err(ctx));
+        FSIM_HIP(ctx, hipMemcpyAsync(hout.data(), ctx->z, sizeof(float) * nc, hipMemcpyDeviceToHost, ctx->stream));
+    } else if (op == FSIM_OP_APPLY_MG) {
+        if (ctx->mg_coarse == FSIM_MG_COARSE_NETWORK && !learned_ready(ctx))
+            return set_err(ctx, FSIM_ERR_INVALID, "the multigrid's coarse network has no parameters");
+        FSIM_TRY(mg_setup(ctx));
+        FSIM_TRY(fill_f32(ctx, ctx->z, 0.0f, nc));
+        FSIM_TRY(mg_apply(ctx, ctx->r, ctx->z, S.scale));
         FSIM_HIP(ctx, hipMemcpyAsync(hout.data(), ctx->z, sizeof(float) * nc, hipMemcpyDeviceToHost, ctx->stream));
     } else {
         return set_err(ctx, FSIM_ERR_INVALID, "unknown solver op");

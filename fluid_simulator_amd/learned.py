@@ -1,11 +1,16 @@
 """Training data and training loop for the learned pressure preconditioner.
 
-Samples are (b, label, p) triples of real pressure solves: seeded random
-dam-break scenes (1-3 boxes) are stepped on the GPU with MIC(0)-PCG and each
-step's right-hand side, cell labels and converged pressure are recorded.  The
-network (include/fsim.h, fsim_learned_*) is fully convolutional, so it is
-trained on a small grid and its parameters are then set on a context of any
-size with Simulation.learned_set_params.
+The preconditioner is a multigrid V-cycle whose coarsest level (at most 16
+cells a side) is solved by the network (include/fsim.h, FSIM_PRECOND_LEARNED).
+Its training samples are that level's problems as the GPU solver meets them:
+seeded random dam-break and sphere-drop scenes are stepped on the GPU with
+MIC(0)-PCG, and at chosen steps the pressure solve is run once more by
+fsim_learned_harvest (flexible PCG with the V-cycle and an accurate
+coarsest-level solve), which returns each coarsest-level right-hand side b,
+its solution x and the level's operator (Galerkin coefficients on the
+network's scale).  The network is fully convolutional: it is trained on a
+context whose grid is the coarsest level's and its parameters are then set
+on a context of any size with Simulation.learned_set_params.
 """
 from __future__ import annotations
 
@@ -15,6 +20,8 @@ from . import capi, scenes
 
 DT = 1.0 / 240.0
 LOSS_MSE, LOSS_ENERGY = 0, 1
+PREP_STAGES = (capi.STAGE_ADVECT, capi.STAGE_PUSH_OUT, capi.STAGE_P2G, capi.STAGE_MARK, capi.STAGE_OBSTACLES,
+               capi.STAGE_BORDERS, capi.STAGE_POST_P2G)
 
 # (input channels, output channels, level) of the 3x3x3 layers, in order
 # (fsim_learned.hip kLayers); the 1x1 output layer maps 16 channels to 1.
@@ -32,76 +39,69 @@ def flops_per_apply(shape) -> float:
     return total + 2.0 * 16 * float(np.prod(d0))
 
 
-def random_blocks(rng: np.random.Generator, n_max: int = 3):
-    blocks = []
-    for _ in range(int(rng.integers(1, n_max + 1))):
-        lo = rng.uniform(0.0, 0.6, 3)
-        hi = np.minimum(lo + rng.uniform(0.2, 0.6, 3), 1.0)
-        lo[1] = 0.0   # boxes stand on the floor
-        blocks.append([*lo, *hi])
-    return blocks
+def random_scene(rng: np.random.Generator, n: int, seed: int) -> scenes.Scene:
+    """A seeded random dam break (one block of random size, or two opposing ones) or sphere drop (a ball
+    of water above a pool)."""
+    kind = int(rng.integers(0, 3))
+    if kind == 0:
+        blocks = [[0.0, 0.0, 0.0, rng.uniform(0.2, 0.6), rng.uniform(0.3, 0.8), 1.0]]
+    elif kind == 1:
+        w, h = rng.uniform(0.2, 0.45), rng.uniform(0.3, 0.7)
+        blocks = [[0.0, 0.0, 0.0, w, h, 1.0], [1.0 - w, 0.0, 0.0, 1.0, h, 1.0]]
+    else:
+        r = rng.uniform(0.08, 0.18)
+        c = (rng.uniform(0.3, 0.7), rng.uniform(0.55, 0.75), rng.uniform(0.3, 0.7))
+        blocks = [[0.0, 0.0, 0.0, 1.0, rng.uniform(0.15, 0.35), 1.0],
+                  [c[0] - r, c[1] - r, c[2] - r, c[0] + r, c[1] + r, c[2] + r]]
+    return scenes.Scene(f"random_{kind}_{n}", n, capi.FLIP, blocks, [], seed)
 
 
-def generate_samples(n: int, scenes_count: int, steps_per_scene: int, seed: int = 0, residuals_per_solve: int = 0):
-    """Arrays B [k, Nc] float32, L [k, Nc] uint8, P [k, Nc] float64 from the solves on an n^3 grid,
-    each followed by residuals_per_solve residual_samples pairs."""
+def harvest_samples(n: int, scenes_count: int, harvests_per_scene: int, per_solve: int = 12, seed: int = 0,
+                    max_steps: int = 30, log=None):
+    """Coarsest-level samples (B [k, Nc'], OP [k, 4, Nc'], X [k, Nc'], dims) from scenes_count random scenes
+    on an n^3 grid, harvests_per_scene solves each (at random steps up to max_steps), at most per_solve
+    samples per solve."""
     rng = np.random.default_rng(seed)
     grid = scenes.grid(n)
-    B, L, P = [], [], []
+    B, OP, X = [], [], []
+    dims = None
     for s in range(scenes_count):
-        pos = capi.scene_dam_break(grid, random_blocks(rng), seed=seed * 1000 + s)
+        sc = random_scene(rng, n, seed * 1000 + s)
+        pos = capi.scene_dam_break(grid, sc.blocks, seed=sc.seed)
         if len(pos) == 0:
             continue
         sim = capi.Simulation(grid, scenes.params())
         sim.set_particles(pos)
-        for _ in range(steps_per_scene):
-            st = sim.step(DT)
-            if st.early_exit:
-                continue
-            b, lab, p = sim.get_grid(capi.FIELD_RHS), sim.get_grid(capi.FIELD_LABEL), sim.get_grid(capi.FIELD_PRESSURE)
-            B.append(b)
-            L.append(lab)
-            P.append(p)
-            if residuals_per_solve:
-                rb, rl, rp = residual_samples(sim, lab, p, residuals_per_solve, seed=seed * 7919 + len(B))
-                B.extend(rb)
-                L.extend(rl)
-                P.extend(rp)
+        at = sorted(rng.choice(np.arange(1, max_steps), size=harvests_per_scene, replace=False))
+        step = 0
+        for a in at:
+            while step < a - 1:
+                sim.step(DT)
+                step += 1
+            for st in PREP_STAGES:
+                sim.run_stage(st, DT)
+            b, op, x, dims = sim.learned_harvest(DT, per_solve)
+            for k in range(len(b)):
+                B.append(b[k])
+                OP.append(op)
+                X.append(x[k])
+            # finish the step the harvest solved (its pressure is applied)
+            sim.run_stage(capi.STAGE_EXTRAPOLATE, DT)
+            sim.run_stage(capi.STAGE_G2P, DT)
+            step += 1
         sim.close()
-    return grid, np.stack(B), np.stack(L), np.stack(P)
+        if log:
+            log(f"scene {s}: {len(B)} samples")
+    return np.stack(B), np.stack(OP), np.stack(X), tuple(dims)
 
 
-def _smooth(f: np.ndarray, k: int) -> np.ndarray:
-    for _ in range(k):
-        f = (f + np.roll(f, 1, 0) + np.roll(f, -1, 0) + np.roll(f, 1, 1) + np.roll(f, -1, 1) + np.roll(f, 1, 2)
-             + np.roll(f, -1, 2)) / 7.0
-    return f
-
-
-def residual_samples(sim: capi.Simulation, label: np.ndarray, p: np.ndarray, count: int, seed: int = 0):
-    """Pairs (A e, e) for error fields e on the WATER cells of the simulation's last solve:
-    white noise smoothed 0-11 times, every third one mixed with the solve's pressure p.
-    Inside flexible PCG the network sees residuals, not only right-hand sides; A e comes from
-    the library's own SpMV (fsim_solver_apply), so every pair is exact."""
-    rng = np.random.default_rng(seed)
-    shape = sim.grid.shape
-    fl = (label == capi.WATER)
-    pmax = max(float(np.abs(p).max()), 1e-30)
-    B, P = [], []
-    for k in range(count):
-        e = _smooth((rng.standard_normal(sim.grid.nc) * fl).reshape(shape), int(rng.integers(0, 12))).reshape(-1) * fl
-        if k % 3 == 0:
-            e = e + rng.uniform(0.2, 1.0) * p / pmax * np.abs(e).max()
-        B.append(sim.solver_apply(capi.OP_APPLY_A, DT, e).astype(np.float32))
-        P.append(e)
-    return np.stack(B), np.stack([label] * count), np.stack(P)
-
-
-def train(grid: capi.Grid, B, L, P, epochs: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
+def train(dims, B, OP, X, epochs: int, batch: int = 8, lr: float = 1e-3, seed: int = 0,
           params: np.ndarray | None = None, log=None, loss: int = LOSS_MSE) -> tuple[np.ndarray, list[float]]:
-    """Adam over shuffled mini-batches; returns the parameters and the per-epoch mean loss.
-    loss: LOSS_MSE (masked MSE against p/s) or LOSS_ENERGY (e^T A e / t^T A t, the norm CG minimises)."""
+    """Adam over shuffled mini-batches on a context whose grid is the coarsest level's (dims); returns the
+    parameters and the per-epoch mean loss.  loss: LOSS_MSE (masked MSE against x / s) or LOSS_ENERGY
+    (e^T L e / t^T L t with the sample's operator)."""
     rng = np.random.default_rng(seed)
+    grid = capi.Grid(*dims, 1.0 / dims[0], 0.25 / dims[0])
     sim = capi.Simulation(grid, scenes.params())
     sim.set_option(capi.OPT_LEARNED_LOSS, loss)
     if params is None:
@@ -115,7 +115,7 @@ def train(grid: capi.Grid, B, L, P, epochs: int, batch: int = 4, lr: float = 1e-
         losses = []
         for i in range(0, k, batch):
             idx = order[i:i + batch]
-            losses.append(sim.learned_train_batch(B[idx], L[idx], P[idx], lr))
+            losses.append(sim.learned_train_batch_op(B[idx], OP[idx], X[idx].astype(np.float64), lr))
         history.append(float(np.mean(losses)))
         if log:
             log(f"epoch {ep}: loss {history[-1]:.4e}")

@@ -209,7 +209,9 @@ typedef enum fsim_field {
 typedef enum fsim_solver_op {
     FSIM_OP_APPLY_A = 0,       /* applyAMatrix, bridsonSolverGrid.cpp:165-198 */
     FSIM_OP_APPLY_MIC0 = 1,    /* applyPreconditioner, bridsonSolverGrid.cpp:126-163 */
-    FSIM_OP_FACTOR_MIC0 = 2    /* calculatePreconditioner; out = diagonal P */
+    FSIM_OP_FACTOR_MIC0 = 2,   /* calculatePreconditioner; out = diagonal P */
+    FSIM_OP_APPLY_MG = 3       /* one V-cycle of the learned multigrid preconditioner (no
+                                  reference counterpart; coarse level per FSIM_OPT_MG_COARSE) */
 } fsim_solver_op;
 
 /* ---- context ---------------------------------------------------------- */
@@ -267,13 +269,14 @@ typedef enum fsim_option {
     FSIM_OPT_SWEEP_MODE = 0,
     /* 1: time every kernel with HIP events (fsim_get_kernel_time) */
     FSIM_OPT_PROFILE = 1,
-    /* learned preconditioner: damped-Jacobi steps (omega 2/3) applied after
-     * the network, 0 .. 16 (default 4) */
+    /* learned preconditioner: red-black Gauss-Seidel sweeps on the coarsest
+     * multigrid level after the network (and the sweeps of the
+     * FSIM_MG_COARSE_SMOOTH mode), 0 .. 16 (default 2) */
     FSIM_OPT_LEARNED_SMOOTHING = 2,
-    /* learned preconditioner training loss (fsim_learned_grad,
-     * fsim_learned_train_batch): 0 (default) masked MSE of out against p/s
-     * over WATER cells; 1 energy norm e^T A e / t^T A t of e = out - p/s, with
-     * A the unscaled pressure matrix on WATER cells */
+    /* learned preconditioner training loss (fsim_learned_grad*,
+     * fsim_learned_train_batch*): 0 (default) masked MSE of out against p/s
+     * over the sample's unknowns; 1 energy norm e^T A e / t^T A t of
+     * e = out - p/s, with A the sample's operator */
     FSIM_OPT_LEARNED_LOSS = 3,
     /* learned preconditioner forward convolutions: 1 (default) the full- and
      * half-resolution layers stage their input tile in LDS when the level's
@@ -288,9 +291,23 @@ typedef enum fsim_option {
     /* learned preconditioner: sets the MIC(0) iteration count the fallback
      * budget is twice of, as if a MIC(0)-PCG solve had just taken that many
      * (> 0); the next solves then run FPCG */
-    FSIM_OPT_LEARNED_MIC_COUNT = 6
+    FSIM_OPT_LEARNED_MIC_COUNT = 6,
+    /* learned preconditioner: how the coarsest multigrid level is solved,
+     * fsim_mg_coarse (default FSIM_MG_COARSE_NETWORK) */
+    FSIM_OPT_MG_COARSE = 7
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
+
+/* Coarsest-level solve of the learned multigrid preconditioner.  NETWORK:
+ * the learned network, then FSIM_OPT_LEARNED_SMOOTHING sweeps; SMOOTH: no
+ * network, only those sweeps from zero (the same preconditioner without the
+ * network); ACCURATE: 600 sweeps (a converged coarse solve, for comparisons
+ * and for harvesting training samples). */
+typedef enum fsim_mg_coarse {
+    FSIM_MG_COARSE_NETWORK = 0,
+    FSIM_MG_COARSE_SMOOTH = 1,
+    FSIM_MG_COARSE_ACCURATE = 2
+} fsim_mg_coarse;
 
 /* Per-kernel GPU time (HIP events on the context's stream) accumulated while
  * FSIM_OPT_PROFILE is on.  Kernel ids run 0 .. fsim_num_kernels()-1. */
@@ -300,36 +317,67 @@ int fsim_get_kernel_time(fsim_ctx* ctx, int kernel, double* total_ms, int64_t* l
 int fsim_reset_kernel_times(fsim_ctx* ctx);
 
 /* ---- learned preconditioner (no counterpart in the reference) -------------
- * A 4-level 3D convolutional encoder-decoder (3x3x3 kernels, 16 base
- * channels, ReLU, BF16 weights and activations, FP32 accumulation) that maps
- * (r / max|r|, WATER mask) to an approximation of A^-1 r.  With
- * fsim_params.precond = FSIM_PRECOND_LEARNED the pressure solve runs flexible
- * PCG with it and falls back to MIC(0)-PCG (fsim_step_stats.fallback = 1) when
- * the tolerance is not reached within twice the last MIC(0) iteration count.
- * That count comes from a MIC(0)-PCG solve of the context's own: the first
- * solve with FSIM_PRECOND_LEARNED, and every FSIM_OPT_LEARNED_REFRESH solves
- * after it, run MIC(0)-PCG (fsim_step_stats.learned_stop = 3).  The network is fully
- * convolutional: parameters trained on a small grid apply to any grid.
- * Parameter vector order: for the 7 conv layers (8->16, 16->32, 32->64,
- * 64->128, 192->64, 96->32, 48->16) the weights [cout][27 taps][cin] (tap =
- * ((dx+1)*3 + dy+1)*3 + dz+1) then the biases [cout]; then the 1x1 output
- * layer's 16 weights and its bias. */
+ * With fsim_params.precond = FSIM_PRECOND_LEARNED the pressure solve runs
+ * flexible PCG preconditioned by one multigrid V-cycle per iteration:
+ * level 0 is the solve's 7-point operator, each coarser level aggregates
+ * 2x2x2 cells (piecewise-constant prolongation, Galerkin operator
+ * P^T A P), two red-black Gauss-Seidel sweeps before and after each coarse
+ * correction, and the coarsest level (the first whose largest side is at most
+ * 16 cells; fsim_mg_info) is solved by the learned network and
+ * FSIM_OPT_LEARNED_SMOOTHING sweeps.  It falls back to MIC(0)-PCG
+ * (fsim_step_stats.fallback = 1) when the tolerance is not reached within
+ * twice the last MIC(0) iteration count.  That count comes from a MIC(0)-PCG
+ * solve of the context's own: the first solve with FSIM_PRECOND_LEARNED, and
+ * every FSIM_OPT_LEARNED_REFRESH solves after it, run MIC(0)-PCG
+ * (fsim_step_stats.learned_stop = 3).
+ *
+ * The network: a 4-level 3D convolutional encoder-decoder (3x3x3 kernels,
+ * 16 base channels, ReLU, BF16 weights and activations, FP32 accumulation)
+ * mapping (b / s, unknown mask, diag / 6, cx, cy, cz, 0, 0) to L^-1 b / s,
+ * where L is the level's operator on the network's scale (Galerkin
+ * coefficients times 4^-level: diagonal diag, couplings -cx, -cy, -cz to the
+ * +x, +y, +z neighbour) and s = max|b|.  Fully convolutional: parameters
+ * trained on one size apply to any.  Parameter vector order: for the 7 conv
+ * layers (8->16, 16->32, 32->64, 64->128, 192->64, 96->32, 48->16) the weights
+ * [cout][27 taps][cin] (tap = ((dx+1)*3 + dy+1)*3 + dz+1) then the biases
+ * [cout]; then the 1x1 output layer's 16 weights and its bias.
+ *
+ * Training samples live on the context's own grid, in flat grid order:
+ * either a label field (the operator is then the 7-point matrix of the
+ * WATER cells: diag = non-SOLID neighbours, couplings 1 between WATER
+ * neighbours) or an explicit operator op[4][Nc] = (diag, cx, cy, cz). */
 int64_t fsim_learned_num_params(void);
 /* He-normal weights from a counter RNG keyed by seed, zero biases */
 int fsim_learned_init(fsim_ctx* ctx, uint64_t seed);
 int fsim_learned_set_params(fsim_ctx* ctx, const float* params);
 int fsim_learned_get_params(fsim_ctx* ctx, float* params);
-/* Network output (before the max|b| scale and the WATER mask) for one
- * right-hand side b and label field, both in flat grid order. */
+/* Network output (raw, before the scale s and the mask) for one right-hand
+ * side b with the operator of a label field or of op. */
 int fsim_learned_forward(fsim_ctx* ctx, const float* b, const uint8_t* label, float* out);
-/* Loss (mean squared error on WATER cells against p / max|b|) and its gradient
- * with respect to every parameter, for one sample (flat grid order). */
+int fsim_learned_forward_op(fsim_ctx* ctx, const float* b, const float* op, float* out);
+/* Loss (FSIM_OPT_LEARNED_LOSS) against p / s and its gradient with respect
+ * to every parameter, for one sample. */
 int fsim_learned_grad(fsim_ctx* ctx, const float* b, const uint8_t* label, const double* p, float* grad,
                       double* loss);
+int fsim_learned_grad_op(fsim_ctx* ctx, const float* b, const float* op, const double* p, float* grad, double* loss);
 /* One Adam step (lr, beta 0.9 / 0.999, eps 1e-8) on the mean gradient of n
- * samples stored back to back (n * Nc values each). */
+ * samples stored back to back (Nc values each; op: 4 Nc each). */
 int fsim_learned_train_batch(fsim_ctx* ctx, int n, const float* b, const uint8_t* label, const double* p, float lr,
                              double* mean_loss);
+int fsim_learned_train_batch_op(fsim_ctx* ctx, int n, const float* b, const float* op, const double* p, float lr,
+                                double* mean_loss);
+/* Multigrid hierarchy of the context: number of levels and the coarsest
+ * level's dimensions. */
+int fsim_mg_info(fsim_ctx* ctx, int* levels, int dims[3]);
+/* Training samples generated by the GPU solver itself: runs the pressure
+ * solve of the current state (call after FSIM_STAGE_POST_P2G) as flexible
+ * PCG with the multigrid preconditioner and an accurate coarsest-level solve,
+ * and returns up to capacity of the coarsest-level problems it met: the
+ * right-hand sides b[n][Nc'], their solutions x[n][Nc'] (L x = b on the
+ * network's scale) and the level's operator op[4][Nc'] (Nc' = cells of the
+ * coarsest level, fsim_mg_info).  The solve's pressure is applied as in
+ * FSIM_STAGE_SOLVE. */
+int fsim_learned_harvest(fsim_ctx* ctx, double dt, int capacity, float* b, float* op, float* x, int* n_out);
 
 /* ---- synthetic scenes (host-only, no GPU needed) -------------------------- */
 /* Dam-break generator: a cell (not on the domain border, not inside any

@@ -5,7 +5,9 @@ Only tests/ may import this module, as the checker of the HIP network
 The network has no counterpart in the reference; this module restates the
 architecture documented there:
 
-    in  = (b / s, WATER mask, 0 x 6), s = max |b| over WATER cells   level 0, 8 ch
+    in  = (b / s, unknown, diag / 6, cx, cy, cz, 0, 0), s = max |b| over the unknowns   level 0, 8 ch
+          (the operator: diagonal diag, couplings -cx, -cy, -cz to the +x, +y, +z neighbour; from a
+          label field, the 7-point matrix of its WATER cells)
     e0  = relu(conv3(in, 8->16)); e1 = relu(conv3(pool(e0), 16->32))
     e2  = relu(conv3(pool(e1), 32->64)); e3 = relu(conv3(pool(e2), 64->128))
     d2  = relu(conv3(cat(up(e3), e2), 192->64))
@@ -19,17 +21,19 @@ emulate_bf16 the weights, the input and every stored activation are rounded
 to bfloat16 (round to nearest even) where the GPU rounds them; sums stay in
 FP64 here (FP32 on the GPU).
 
-Training losses: masked MSE of out against p/s over WATER cells, or the
-energy norm e^T A e / t^T A t of e = out - p/s (A the unscaled 7-point
-pressure matrix on WATER cells: diagonal = number of non-SOLID neighbours,
--1 towards WATER neighbours, AIR neighbours Dirichlet zeros).
+Training losses: masked MSE of out against p/s over the unknowns, or the
+energy norm e^T A e / t^T A t of e = out - p/s (A the sample's operator; for
+a label field the unscaled 7-point pressure matrix on WATER cells: diagonal =
+number of non-SOLID neighbours, -1 towards WATER neighbours, AIR neighbours
+Dirichlet zeros).
 
 Parity pins (tests/test_learned_oracle.py): central finite differences of
 both FP64 losses against backward() on every layer, conv3 of a unit impulse
 equals the mirrored kernel, pool(up(x)) == x, the 1x1 layer is a dot
 product; the energy form equals its edge sum
 sum_(WATER-WATER edges) (e_i - e_j)^2 + sum_i (AIR neighbours of i) e_i^2
-(loops over cells) and A is symmetric.
+(loops over cells) and A is symmetric; the operator form apply_op equals that
+label form for a label field and is symmetric for any coefficients.
 """
 from __future__ import annotations
 
@@ -138,17 +142,50 @@ def pool2_bwd(g):
     return up2(g) / 8.0
 
 
-def make_input(b: np.ndarray, label: np.ndarray, shape, water: int = 0):   # FSIM_WATER = 0
-    """Flat grid-order b and labels -> network input [D,H,W,8] (padded to multiples of 8), scale s."""
+def label_operator(label: np.ndarray, shape, water: int = 0, solid: int = 2) -> np.ndarray:
+    """The 7-point operator of a label field as op[4][x][y][z] = (diag, cx, cy, cz): diag = non-SOLID
+    neighbours of a WATER cell (cells outside the grid count as SOLID), couplings 1 between WATER cells
+    and their WATER +x / +y / +z neighbour; 0 elsewhere."""
+    lab = label.reshape(shape)
+    pad = np.full(tuple(n + 2 for n in shape), solid, lab.dtype)
+    pad[1:-1, 1:-1, 1:-1] = lab
+    w = lab == water
+    op = np.zeros((4,) + tuple(shape))
+    for ax in range(3):
+        for d in (-1, 1):
+            op[0] += np.roll(pad, -d, axis=ax)[1:-1, 1:-1, 1:-1] != solid
+        op[1 + ax] = w & (np.roll(pad, -1, axis=ax)[1:-1, 1:-1, 1:-1] == water)
+    op[0] = np.where(w, op[0], 0.0)
+    return op
+
+
+def make_input_op(b: np.ndarray, op: np.ndarray, shape):
+    """Flat grid-order b and an operator op[4][...] (diag, cx, cy, cz) -> network input [D,H,W,8] (padded to
+    multiples of 8): (b / s, unknown, diag / 6, cx, cy, cz, 0, 0), s = max |b| over the unknowns (diag > 0)."""
     nx, ny, nz = shape
-    fl = (label.reshape(shape) == water)
+    op = np.asarray(op, np.float64).reshape((4,) + tuple(shape))
+    fl = op[0] > 0
     bb = np.where(fl, b.reshape(shape).astype(np.float64), 0.0)
     s = float(np.max(np.abs(bb))) if fl.any() else 0.0
     pad = [(0, (-n) % 8) for n in shape]
     x = np.zeros(tuple(n + p[1] for n, p in zip(shape, pad)) + (IN_CH,))
     x[:nx, :ny, :nz, 0] = bb / s if s > 0 else 0.0
     x[:nx, :ny, :nz, 1] = fl
+    x[:nx, :ny, :nz, 2] = np.where(fl, op[0] / 6.0, 0.0)
+    for a in range(3):
+        x[:nx, :ny, :nz, 3 + a] = np.where(fl, op[1 + a], 0.0)
     return x, s, fl
+
+
+def make_input(b: np.ndarray, label: np.ndarray, shape, water: int = 0):   # FSIM_WATER = 0
+    """Flat grid-order b and labels -> network input (make_input_op with the labels' operator)."""
+    return make_input_op(b, label_operator(label, shape, water), shape)
+
+
+def operator_padded(op: np.ndarray, shape, padded) -> np.ndarray:
+    out = np.zeros((4,) + tuple(padded))
+    out[:, :shape[0], :shape[1], :shape[2]] = np.asarray(op).reshape((4,) + tuple(shape))
+    return out
 
 
 def forward(P: dict, x: np.ndarray, emulate_bf16: bool = False):
@@ -251,13 +288,30 @@ def apply_a(e: np.ndarray, fluid: np.ndarray, cnt: np.ndarray) -> np.ndarray:
     return np.where(fluid, cnt * ez - nb, 0.0)
 
 
-def energy_loss_and_grad(P: dict, x: np.ndarray, target: np.ndarray, fluid: np.ndarray, cnt: np.ndarray,
+def apply_op(e: np.ndarray, op: np.ndarray) -> np.ndarray:
+    """A e for the operator op[4][...] (diagonal op[0], couplings -op[1..3] between a cell and its +x, +y,
+    +z neighbour, symmetric) on its unknowns (op[0] > 0; e is taken as zero elsewhere)."""
+    fl = op[0] > 0
+    ez = np.where(fl, e, 0.0)
+    ae = op[0] * ez
+    for ax in range(3):
+        c = op[1 + ax]
+        up = [slice(None)] * 3
+        lo = [slice(None)] * 3
+        up[ax], lo[ax] = slice(1, None), slice(None, -1)
+        ae[tuple(lo)] -= c[tuple(lo)] * ez[tuple(up)]    # the +ax neighbour
+        ae[tuple(up)] -= c[tuple(lo)] * ez[tuple(lo)]    # the -ax neighbour (the coupling stored below it)
+    return np.where(fl, ae, 0.0)
+
+
+def energy_loss_and_grad(P: dict, x: np.ndarray, target: np.ndarray, fluid: np.ndarray, op: np.ndarray,
                          emulate_bf16: bool = False):
-    """Energy norm e^T A e / t^T A t of e = out - target on WATER cells, and its parameter gradient."""
+    """Energy norm e^T A e / t^T A t of e = out - target on the unknowns, A the operator op[4][...] on the
+    network's (padded) grid, and its parameter gradient."""
     out, c = forward(P, x, emulate_bf16)
     e = np.where(fluid, out - target, 0.0)
-    ae = apply_a(e, fluid, cnt)
-    et = float(np.sum(np.where(fluid, target, 0.0) * apply_a(target, fluid, cnt)))
+    ae = apply_op(e, op)
+    et = float(np.sum(np.where(fluid, target, 0.0) * apply_op(target, op)))
     loss = float(np.sum(e * ae) / et)
     g = backward(c, 2.0 * ae / et)
     return loss, g, out

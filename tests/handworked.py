@@ -295,7 +295,8 @@ def _face_triples(what, got: dict, expect: dict):
     """The set of changed faces must match exactly; their values to the bar."""
     keys = sorted(expect)
     return [(f"faces changed ({what})", sorted(got), keys),
-            (f"values of the changed faces ({what})", [got.get(k, np.nan) for k in keys], [expect[k] for k in keys])]
+            (f"values of the changed faces ({what})", np.array([got.get(k, np.nan) for k in keys]),
+             np.array([expect[k] for k in keys], np.float64))]
 
 
 def obstacle_face_speeds(make, sphere: bool):

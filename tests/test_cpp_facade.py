@@ -63,3 +63,21 @@ def test_facade_matches_c_abi(gpu, tmp_path, transfer):
     assert info["pcg_iterations"] == info["it_count"] == st.pcg_iterations
     assert info["water_cells"] == int(np.count_nonzero(labels == capi.WATER))
     assert np.array_equal(facade_pos, pos.astype(np.float32))
+
+
+def test_facade_particle_count_edits():
+    """HashedParticles(num, r, dimensions, cellD, ...) places num particles in the upper half box
+    [dims/2 + cellD, dims - 1.1 r - cellD] (hashedParticles.cpp:26-35); setParticleNum adds particles at
+    rest in [1.1 r + cellD, dims - 1.1 r - cellD] (:137-146) and drops from the end (:134-136).  The
+    positions come from a counter RNG instead of std::rand: the ranges and the (uniform) mean are
+    checked, not the values."""
+    _need_demo()
+    res = subprocess.run([DEMO, "particles", "2000", "50000", "1500"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0, res.stderr
+    info = json.loads(res.stdout.strip().splitlines()[-1])
+    dx, r = 1.0 / 32, 1.0 / 128
+    assert info["n0"] == 2000 and info["n1"] == 50000 and info["n2"] == 1500 and info["kept"]
+    assert 0.5 + dx <= info["lo0"] < 0.5 + dx + 0.01 and 1 - 1.1 * r - dx - 0.01 < info["hi0"] <= 1 - 1.1 * r - dx
+    lo, hi = 1.1 * r + dx, 1 - 1.1 * r - dx
+    assert lo <= info["lo1"] < lo + 0.001 and hi - 0.001 < info["hi1"] <= hi
+    assert abs(info["mean1"] - 0.5) < 0.01 and info["vmax1"] == 0

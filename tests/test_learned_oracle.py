@@ -140,13 +140,13 @@ def test_energy_loss_matches_finite_differences():
     lab[1:7, 1:5, 1:7] = 0
     b = rng.standard_normal(shape).astype(np.float32)
     x, s, fl = net.make_input(b.reshape(-1), lab.reshape(-1), shape)
-    cnt = net.neighbour_counts(lab.reshape(-1), shape, x.shape[:3])
+    op = net.operator_padded(net.label_operator(lab.reshape(-1), shape), shape, x.shape[:3])
     target = np.where(fl, rng.standard_normal(shape), 0.0)
     P = {}
     for name, _, shp in net.param_layout()[0]:
         fan = shp[1] * shp[2] if len(shp) == 3 else net.OUT_IN
         P[name] = rnd(rng, *shp, scale=np.sqrt(2.0 / fan)) + (0.05 if name.startswith("b") else 0.0)
-    loss, g, _ = net.energy_loss_and_grad(P, x, target, fl, cnt)
+    loss, g, _ = net.energy_loss_and_grad(P, x, target, fl, op)
     assert loss > 0
     h = 1e-6
     for name in ("wo", "bo", "w6", "b5", "w3", "w0"):
@@ -154,8 +154,36 @@ def test_energy_loss_matches_finite_differences():
         idx = tuple(int(rng.integers(0, n)) for n in shp)
         old = P[name][idx]
         P[name][idx] = old + h
-        lp = net.energy_loss_and_grad(P, x, target, fl, cnt)[0]
+        lp = net.energy_loss_and_grad(P, x, target, fl, op)[0]
         P[name][idx] = old - h
-        lm = net.energy_loss_and_grad(P, x, target, fl, cnt)[0]
+        lm = net.energy_loss_and_grad(P, x, target, fl, op)[0]
         P[name][idx] = old
         assert (lp - lm) / (2 * h) == pytest.approx(g[name][idx], rel=1e-4, abs=1e-8), (name, idx)
+
+
+def test_operator_form_equals_label_form_and_is_symmetric():
+    """apply_op with the labels' operator equals apply_a (the form the edge sum pins); for arbitrary
+    non-negative couplings (a Galerkin coarse operator) apply_op is symmetric, and the input channels
+    carry the operator."""
+    rng = np.random.default_rng(13)
+    lab = small_labels(rng)
+    shape = lab.shape
+    fl = lab == 0
+    cnt = net.neighbour_counts(lab.reshape(-1), shape, shape)
+    op = net.label_operator(lab.reshape(-1), shape)
+    assert np.array_equal(op[0], np.where(fl, cnt, 0))
+    e = rng.standard_normal(shape)
+    assert np.allclose(net.apply_op(e, op), net.apply_a(e, fl, cnt), rtol=0, atol=1e-12)
+    gop = rng.uniform(0, 4, (4,) + shape)
+    gop[0] = np.where(fl, gop[0] + 20, 0.0)
+    for a in range(3):
+        gop[1 + a] *= fl & np.roll(fl, -1, axis=a)
+        last = [slice(None)] * 3
+        last[a] = -1
+        gop[1 + a][tuple(last)] = 0.0
+    u, v = rng.standard_normal(shape), rng.standard_normal(shape)
+    assert float(np.sum(np.where(fl, u, 0) * net.apply_op(v, gop))) == pytest.approx(
+        float(np.sum(np.where(fl, v, 0) * net.apply_op(u, gop))), rel=1e-12)
+    x, s, f = net.make_input_op(e.reshape(-1), gop, shape)
+    assert np.allclose(x[:shape[0], :shape[1], :shape[2], 2], gop[0] / 6.0)
+    assert np.allclose(x[:shape[0], :shape[1], :shape[2], 3], np.where(fl, gop[1], 0.0))
