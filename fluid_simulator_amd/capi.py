@@ -121,6 +121,8 @@ PROTOTYPES = {
     "fsim_scene_dam_break": (C.c_int, [C.POINTER(GridDesc), C.c_int, _P, C.c_int, C.POINTER(Obstacle), C.c_uint64,
                                        C.c_int64, _P, C.POINTER(C.c_int64)]),
     "fsim_gfx_snapshot": (C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_int64)]),
+    "fsim_gfx_snapshot_noalloc": (C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_int64)]),
+    "fsim_gfx_snapshot_noalloc_max": (C.c_int64, []),
     "fsim_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
     "fsim_num_kernels": (C.c_int, []),
     "fsim_kernel_name": (C.c_char_p, [C.c_int]),
@@ -325,12 +327,14 @@ class Simulation:
         a = np.ascontiguousarray(values, dtype=FIELD_DTYPES[field]).reshape(-1)
         self._check(self.lib.fsim_set_grid(self._ctx, field, _ptr(a), a.nbytes))
 
-    def gfx_snapshot(self) -> np.ndarray:
-        """SimulationManager's ParticleGfxData of every particle, in particle order (GFX_DTYPE records)."""
+    def gfx_snapshot(self, noalloc: bool = False) -> np.ndarray:
+        """SimulationManager's ParticleGfxData of every particle, in particle order (GFX_DTYPE records).
+        noalloc: through fsim_gfx_snapshot_noalloc (fixed per-GPU staging, no device allocation)."""
         n = self.num_particles()
         out = np.zeros(max(n, 1), GFX_DTYPE)
         got = C.c_int64()
-        self._check(self.lib.fsim_gfx_snapshot(self._ctx, n, _ptr(out), C.byref(got)))
+        fn = self.lib.fsim_gfx_snapshot_noalloc if noalloc else self.lib.fsim_gfx_snapshot
+        self._check(fn(self._ctx, n, _ptr(out), C.byref(got)))
         return out[:n]
 
     def set_option(self, option: int, value: int):

@@ -212,6 +212,7 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
         return fail(FSIM_ERR_HIP);
     if (particles_reserve(ctx, 1024) != FSIM_OK) return fail(FSIM_ERR_OOM);
     if ((rc = wave_init(ctx)) != FSIM_OK) return fail(rc);
+    if ((rc = scratch_init(ctx)) != FSIM_OK) return fail(rc);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(FSIM_ERR_HIP);
     *out = ctx;
     return FSIM_OK;
@@ -511,6 +512,16 @@ int fsim_gfx_snapshot(fsim_ctx* ctx, int64_t capacity, fsim_gfx_particle* out, i
     if (capacity < ctx->P.n) return set_err(ctx, FSIM_ERR_INVALID, "capacity smaller than the particle count");
     return gfx_snapshot(ctx, out);
 }
+
+int fsim_gfx_snapshot_noalloc(fsim_ctx* ctx, int64_t capacity, fsim_gfx_particle* out, int64_t* n_out) {
+    if (!ctx) return FSIM_ERR_INVALID;
+    if (n_out) *n_out = ctx->P.n;
+    if (!out) return FSIM_OK;
+    if (capacity < ctx->P.n) return set_err(ctx, FSIM_ERR_INVALID, "capacity smaller than the particle count");
+    return gfx_snapshot_noalloc(ctx, out);
+}
+
+int64_t fsim_gfx_snapshot_noalloc_max(void) { return scratch_particles(); }
 
 int fsim_num_kernels(void) { return KID_COUNT; }
 

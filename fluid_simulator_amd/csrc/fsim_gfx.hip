@@ -13,10 +13,41 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "fsim_internal.hpp"
 
 namespace fsim {
+
+// Staging of the allocation-free snapshot (fsim_gfx_snapshot_noalloc): a
+// fixed array in every GPU's copy of this code object, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel or a
+// copy (on the host it is only a stand-in for the symbol): it uses the
+// device address hipGetSymbolAddress returns for the current device, kept
+// per device in g_scratch.  Calls that overlap on one GPU share the array
+// and race; each call overwrites the records it reads back.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int64_t kScratchParticles = (int64_t)(kScratchBytes / sizeof(fsim_gfx_particle));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static void* g_scratch[kMaxScratchDevices];
+static std::mutex g_scratch_mu;
+
+// Resolves this device's scratch address once (with the device current).
+int scratch_init(fsim_ctx* ctx) {
+    const int dev = ctx->device;
+    if (dev < 0 || dev >= kMaxScratchDevices) return set_err(ctx, FSIM_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[dev]) return FSIM_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_err(ctx, FSIM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+    return FSIM_OK;
+}
+
+int64_t scratch_particles() { return kScratchParticles; }
 
 struct GfxArgs {
     const float* pos[3];
@@ -107,6 +138,22 @@ int gfx_snapshot(fsim_ctx* ctx, fsim_gfx_particle* host_out) {
         ctx->gfx_cap = cap;
     }
     return gfx_run(ctx, ctx->gfx_staging, host_out);
+}
+
+// The same kernel and copy with the staging in this GPU's scratch array:
+// no allocation, at most kScratchParticles particles.
+int gfx_snapshot_noalloc(fsim_ctx* ctx, fsim_gfx_particle* host_out) {
+    if (ctx->P.n > kScratchParticles)
+        return set_err(ctx, FSIM_ERR_INVALID, "fsim_gfx_snapshot_noalloc: more than " +
+                                                  std::to_string(kScratchParticles) +
+                                                  " particles (64 MiB scratch); use fsim_gfx_snapshot");
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        p = g_scratch[ctx->device];
+    }
+    if (!p) return set_err(ctx, FSIM_ERR_INVALID, "scratch address of this device was not resolved");
+    return gfx_run(ctx, static_cast<fsim_gfx_particle*>(p), host_out);
 }
 
 void gfx_free(fsim_ctx* ctx) {

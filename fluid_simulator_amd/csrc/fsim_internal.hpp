@@ -355,6 +355,9 @@ int solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double* out
 
 // ---- renderer snapshot (fsim_gfx.hip) --------------------------------------
 int gfx_snapshot(fsim_ctx* ctx, fsim_gfx_particle* host_out);
+int gfx_snapshot_noalloc(fsim_ctx* ctx, fsim_gfx_particle* host_out);
+int scratch_init(fsim_ctx* ctx);
+int64_t scratch_particles();
 void gfx_free(fsim_ctx* ctx);
 
 // ---- multigrid of the learned preconditioner (fsim_mg.hip) ------------------

@@ -260,6 +260,16 @@ int fsim_solver_apply(fsim_ctx* ctx, int op, double dt, const double* in, double
  * trilinearInterpoll(pos, cell centre) * avgPNum}.  Computed on the device
  * from the current state (call it between steps), one device-to-host copy. */
 int fsim_gfx_snapshot(fsim_ctx* ctx, int64_t capacity, fsim_gfx_particle* out, int64_t* n_out);
+/* No counterpart in the reference: fsim_gfx_snapshot for callers that cannot
+ * allocate device memory (a render loop under HIP graph capture, a serving
+ * loop that forbids hipMalloc).  Same kernel and output; the device staging
+ * is a fixed 64 MiB array per GPU (held until the process ends) instead of
+ * the context's buffer, so the call allocates nothing.  At most
+ * fsim_gfx_snapshot_noalloc_max() particles (3,355,443), else
+ * FSIM_ERR_INVALID.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (two threads, two contexts) race. */
+int fsim_gfx_snapshot_noalloc(fsim_ctx* ctx, int64_t capacity, fsim_gfx_particle* out, int64_t* n_out);
+int64_t fsim_gfx_snapshot_noalloc_max(void);
 
 /* ---- implementation options (no reference counterpart) ------------------- */
 typedef enum fsim_option {
