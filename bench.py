@@ -9,9 +9,11 @@ and grid state resident in HBM.
 
 Prints one JSON line: steps/s (value), plus the roofline of the dominant
 kernel measured with HIP events on the library's stream, the FP64 CPU
-oracle timed on a bounded sample of the same workload, and the second
-metric, pressure-solve ms to 1e-6 with MIC(0)-PCG vs the learned FPCG on the
-same right-hand side at 128^3 (network trained in this run, on the GPU).
+oracle timed on a bounded sample of the same workload, a 20-step 128^3
+GPU-vs-oracle error trace, the second metric, pressure-solve ms to 1e-6 with
+MIC(0)-PCG vs the learned multigrid FPCG on the same right-hand side at 128^3
+and 256^3 (network trained in this run, on the GPU), and the 256^3 scene with
+solid boxes and a particle source run with both solvers.
 """
 from __future__ import annotations
 
@@ -312,7 +314,7 @@ def learned_solve_compare(sizes, train_n: int, train_scenes: int, epochs: int, s
                 "validation": "FPCG iterations (budget 200) of the 4th step of a 64^3 FLIP double dam break",
                 "selected_seed": best["seed"],
                 "candidates": [{k: v for k, v in c.items() if k != "params"} for c in cands]}
-    return {"training": training, "grids": [_compare_at(n, best["params"], untrained) for n in sizes]}
+    return {"training": training, "grids": [_compare_at(n, best["params"], untrained) for n in sizes]}, best["params"]
 
 
 def _compare_at(n: int, params, untrained) -> dict:
@@ -379,6 +381,44 @@ def _compare_at(n: int, params, untrained) -> dict:
             "network": network}
 
 
+def boxes_source_run(params, steps: int) -> dict:
+    """The 256^3 FLIP scene with 3-8 solid boxes and a particle source (scenes.CONFIG_256_BOXES_SOURCE) run
+    `steps` steps with MIC(0)-PCG and, on a second context from the same start, with the learned multigrid
+    FPCG (the network of the solver comparison) and its MIC(0)-PCG fallback after twice the last MIC(0)
+    count; the first learned solve is MIC(0)-PCG, which sets that count, and so is every 50th after.
+    Per solver: mean step and solve ms, mean iterations, fallbacks, final particle count."""
+    sc = scenes.CONFIG_256_BOXES_SOURCE()
+    pos = sc.particles()
+    out = {"workload": f"256^3 FLIP double dam break with {len(sc.obstacles) - 1} solid boxes and a particle source "
+                       f"(2000 particles per step), {steps} steps from the start, PCG to 1e-6",
+           "steps": steps, "initial_particles": len(pos)}
+    for name, precond in (("mic0_pcg", capi.PRECOND_MIC0), ("learned", capi.PRECOND_LEARNED)):
+        sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer, precond=precond, spawning_enabled=1))
+        sim.set_obstacles(sc.obstacles)
+        if precond == capi.PRECOND_LEARNED:
+            sim.learned_set_params(params)
+        sim.set_particles(pos)
+        stats = []
+        t0 = time.perf_counter()
+        for k in range(steps):
+            stats.append(sim.step(scenes.DT))
+            if k % 50 == 0:
+                print(f"# boxes+source {name} step {k}: {stats[-1].step_ms:.1f} ms", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        fpcg = [s.learned_iterations for s in stats if s.learned_stop != 3]
+        out[name] = {"ms_per_step": round(1e3 * wall / steps, 3),
+                     "solve_ms_mean": round(float(np.mean([s.solve_ms for s in stats])), 3),
+                     "pcg_iterations_mean": round(float(np.mean([s.pcg_iterations for s in stats])), 2),
+                     "fpcg_iterations_mean": round(float(np.mean(fpcg)), 2) if fpcg else None,
+                     "mic0_solves": sum(1 for s in stats if s.learned_stop == 3 or s.fallback),
+                     "fallbacks": sum(s.fallback for s in stats),
+                     "max_residual_rel_l2": max(s.residual_rel_l2 for s in stats),
+                     "final_particles": stats[-1].num_particles}
+        print(f"# boxes+source {name}: {out[name]}", file=sys.stderr, flush=True)
+        sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -401,6 +441,8 @@ def main():
     ap.add_argument("--learned-train-size", type=int, default=64)
     ap.add_argument("--learned-scenes", type=int, default=24)
     ap.add_argument("--learned-epochs", type=int, default=60)
+    ap.add_argument("--boxes-source-steps", type=int, default=50,
+                    help="steps of the 256^3 boxes-and-source learned run (the full scene is 500; 0: skip)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -462,10 +504,12 @@ def main():
     print(f"# per-step kernel ms: {json.dumps(breakdown)}", file=sys.stderr, flush=True)
     sim.close()
 
-    solve_cmp = None
+    solve_cmp, boxes = None, None
     if not args.no_learned:
         sizes = [int(v) for v in args.learned_sizes.split(",") if v.strip()]
-        solve_cmp = learned_solve_compare(sizes, args.learned_train_size, args.learned_scenes, args.learned_epochs)
+        solve_cmp, net = learned_solve_compare(sizes, args.learned_train_size, args.learned_scenes, args.learned_epochs)
+        if args.boxes_source_steps > 0:
+            boxes = boxes_source_run(net, args.boxes_source_steps)
 
     cpu, accuracy = None, None
     if not args.no_cpu_baseline:
@@ -498,6 +542,7 @@ def main():
         "accuracy": accuracy,
         "error_trace": trace,
         "pressure_solve": solve_cmp,
+        "boxes_and_source": boxes,
         "kernel_ms_per_step": breakdown,
     }
     print(json.dumps(out), flush=True)

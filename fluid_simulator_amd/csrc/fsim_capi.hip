@@ -109,7 +109,7 @@ static void free_all(fsim_ctx* ctx) {
                     ctx->u2, ctx->v2, ctx->w2, ctx->label, ctx->valid, ctx->density, ctx->cell_count, ctx->cell_start,
                     ctx->scan_tmp, ctx->nbinfo, ctx->chunk_flags, ctx->b, ctx->r, ctx->z, ctx->s, ctx->q, ctx->y, ctx->P32, ctx->P64,
                     ctx->x, ctx->partials, ctx->scal, ctx->obs_dev, ctx->ybnd, ctx->zbnd, ctx->s2, ctx->red_count,
-                    ctx->wave_ticket, ctx->wave_err, ctx->sor64};
+                    ctx->wave_ticket, ctx->wave_err, ctx->sor64, ctx->colw};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
@@ -576,6 +576,10 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
         if (value <= 0) return set_err(ctx, FSIM_ERR_INVALID, "the MIC(0) iteration count must be positive");
         ctx->last_mic_iterations = value;
         ctx->learned_since_mic = 0;
+        return FSIM_OK;
+    case FSIM_OPT_RELIABLE_DROP:
+        if (value < 0 || value > 15) return set_err(ctx, FSIM_ERR_INVALID, "reliable-update drop is 0 .. 15");
+        ctx->reliable_drop = value;
         return FSIM_OK;
     case FSIM_OPT_MG_COARSE:
         if (value < FSIM_MG_COARSE_NETWORK || value > FSIM_MG_COARSE_ACCURATE)

@@ -151,6 +151,7 @@ struct WaveArgs {
     unsigned long long ticket_base;
     uint32_t epoch;
     int* err;
+    const uint8_t* colw;          // [y * nz + z]: 1 if the (y, z) column holds a WATER cell
 };
 
 struct GridDims {
@@ -227,6 +228,7 @@ struct fsim_ctx {
     int32_t* cell_start = nullptr;   // Nc + 2
     int32_t* scan_tmp = nullptr;
     uint16_t* nbinfo = nullptr;
+    uint8_t* colw = nullptr;          // WATER columns (y, z) of the current labels, for the wavefront kernels
     uint32_t* chunk_flags = nullptr;  // solver-vector chunks (64 or 256 cells) holding a WATER cell, per solve
 
     // pressure solve
@@ -247,6 +249,7 @@ struct fsim_ctx {
     fsim::LearnedNet* learned = nullptr;   // learned preconditioner (network parameters and buffers)
     fsim::MgState* mg = nullptr;          // multigrid hierarchy of the learned preconditioner
     int mg_coarse = 0;                    // coarsest-level solve: FSIM_MG_COARSE_*
+    int reliable_drop = 3;                // PCG: true residual after a 10^drop drop of the recursive one (0: at the tolerance)
     bool mg_harvesting = false;           // a solve run by fsim_learned_harvest
     int partials_cap = 0;
     fsim::SolveScalars* scal = nullptr;   // device

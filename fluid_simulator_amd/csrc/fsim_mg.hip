@@ -138,20 +138,40 @@ __global__ void __launch_bounds__(kBlock) k_mg_coarsen(MgGrid F, int cx, int cy,
 }
 
 // ---- smoothing: one red-black half-sweep --------------------------------------------
+// Launch shape of the per-cell kernels: a block is kMgBy rows (y) x kMgBz
+// columns (z) of one plane (level 0: rotated plane p; coarse levels: x), the
+// grid (z chunks, y chunks, planes), so no thread divides its index.  In the
+// rotated layout x + z is constant along each 64-column z-tile of a plane
+// row (x = p - ((z - 1) mod 64) up to the wrap by nx), so for even nx a row
+// has one colour and a half-sweep's other-colour rows leave at once.
+constexpr int kMgBz = 64, kMgBy = 4;
+struct MgLaunch {
+    dim3 grid, block;
+};
+static MgLaunch mg_launch(int nx, int ny, int nz) {
+    int bz = kMgBz;
+    while (bz > 16 && bz / 2 >= nz) bz /= 2;   // small coarse levels: narrower z, more rows per block
+    const int by = kMgBz * kMgBy / bz;
+    return MgLaunch{dim3((unsigned)((nz + bz - 1) / bz), (unsigned)((ny + by - 1) / by), (unsigned)nx),
+                    dim3((unsigned)bz, (unsigned)by)};
+}
+
 // FIRST: the first half-sweep from x = 0 (no neighbour terms; the other colour
 // is set to 0 on the level's unknowns).
 template <int PAR, bool FIRST>
-__global__ void __launch_bounds__(kBlock) k_mg_rbgs0(Mg0 A) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.nc) return;
+__global__ void __launch_bounds__(kMgBz * kMgBy) k_mg_rbgs0(Mg0 A) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int p = blockIdx.z;
+    if (z >= A.nz || y >= A.ny) return;
+    const int x = unrot_x(p, z, A.nx);
+    const bool mine = ((x + y + z) & 1) == PAR;
+    if (!FIRST && !mine) return;
+    const int64_t i = ((int64_t)p * A.ny + y) * A.nz + z;
     const uint16_t nb = A.nbinfo[i];
     if (!(nb & NB_FLUID)) return;
-    const int64_t plane = (int64_t)A.ny * A.nz;
-    const int p = (int)(i / plane);
-    const int y = (int)((i / A.nz) % A.ny), z = (int)(i % A.nz);
-    const int x = unrot_x(p, z, A.nx);
-    if (((x + y + z) & 1) != PAR) {
-        if (FIRST) A.x[i] = 0.f;
+    if (!mine) {
+        A.x[i] = 0.f;   // FIRST only
         return;
     }
     float s = A.b[i] * A.binv;
@@ -168,15 +188,18 @@ __global__ void __launch_bounds__(kBlock) k_mg_rbgs0(Mg0 A) {
 }
 
 template <int PAR, bool FIRST>
-__global__ void __launch_bounds__(kBlock) k_mg_rbgs(MgGrid G) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)G.nx * G.ny * G.nz) return;
+__global__ void __launch_bounds__(kMgBz * kMgBy) k_mg_rbgs(MgGrid G) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int x = blockIdx.z;
+    if (z >= G.nz || y >= G.ny) return;
+    const bool mine = ((x + y + z) & 1) == PAR;
+    if (!FIRST && !mine) return;
+    const int64_t i = lin(G.nx, G.ny, G.nz, x, y, z);
     const float d = G.diag[i];
     if (d == 0.f) return;
-    int x, y, z;
-    unlin(i, G.ny, G.nz, x, y, z);
-    if (((x + y + z) & 1) != PAR) {
-        if (FIRST) G.x[i] = 0.f;
+    if (!mine) {
+        G.x[i] = 0.f;   // FIRST only
         return;
     }
     float s = G.b[i];
@@ -257,23 +280,24 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(MgGrid F, MgGrid C) {
 }
 
 // ---- prolongation: x_i += 1.6 x_{l+1}[parent(i)] on the level's unknowns ----------------
-__global__ void __launch_bounds__(kBlock) k_mg_prolong0(Mg0 A, MgGrid C) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.nc) return;
+__global__ void __launch_bounds__(kMgBz * kMgBy) k_mg_prolong0(Mg0 A, MgGrid C) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int p = blockIdx.z;
+    if (z >= A.nz || y >= A.ny) return;
+    const int64_t i = ((int64_t)p * A.ny + y) * A.nz + z;
     if (!(A.nbinfo[i] & NB_FLUID)) return;
-    const int64_t plane = (int64_t)A.ny * A.nz;
-    const int p = (int)(i / plane);
-    const int y = (int)((i / A.nz) % A.ny), z = (int)(i % A.nz);
     const int x = unrot_x(p, z, A.nx);
     A.x[i] += kMgOverCorrect * C.x[lin(C.nx, C.ny, C.nz, x >> 1, y >> 1, z >> 1)];
 }
 
-__global__ void __launch_bounds__(kBlock) k_mg_prolong(MgGrid F, MgGrid C) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)F.nx * F.ny * F.nz) return;
+__global__ void __launch_bounds__(kMgBz * kMgBy) k_mg_prolong(MgGrid F, MgGrid C) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int x = blockIdx.z;
+    if (z >= F.nz || y >= F.ny) return;
+    const int64_t i = lin(F.nx, F.ny, F.nz, x, y, z);
     if (F.diag[i] == 0.f) return;
-    int x, y, z;
-    unlin(i, F.ny, F.nz, x, y, z);
     F.x[i] += kMgOverCorrect * C.x[lin(C.nx, C.ny, C.nz, x >> 1, y >> 1, z >> 1)];
 }
 
@@ -373,7 +397,8 @@ int mg_setup(fsim_ctx* ctx) {
 }
 
 static void smooth0(fsim_ctx* ctx, const Mg0& A, int sweeps, bool first) {
-    const dim3 g(nblocks(A.nc)), b(kBlock);
+    const MgLaunch L = mg_launch(A.nx, A.ny, A.nz);
+    const dim3 g = L.grid, b = L.block;
     for (int k = 0; k < sweeps; k++) {
         if (first && k == 0) hipLaunchKernelGGL((k_mg_rbgs0<1, true>), g, b, 0, ctx->stream, A);
         else hipLaunchKernelGGL((k_mg_rbgs0<1, false>), g, b, 0, ctx->stream, A);
@@ -382,7 +407,8 @@ static void smooth0(fsim_ctx* ctx, const Mg0& A, int sweeps, bool first) {
 }
 static void smooth(fsim_ctx* ctx, const MgLevelBuf& L, int sweeps, bool first) {
     const MgGrid G = grid_of(L);
-    const dim3 g(nblocks(L.nc)), b(kBlock);
+    const MgLaunch S = mg_launch(L.nx, L.ny, L.nz);
+    const dim3 g = S.grid, b = S.block;
     for (int k = 0; k < sweeps; k++) {
         if (first && k == 0) hipLaunchKernelGGL((k_mg_rbgs<1, true>), g, b, 0, ctx->stream, G);
         else hipLaunchKernelGGL((k_mg_rbgs<1, false>), g, b, 0, ctx->stream, G);
@@ -442,11 +468,12 @@ int mg_apply(fsim_ctx* ctx, const float* r, float* z, double scale) {
     FSIM_TRY(coarse_solve(ctx, A, c));
     // up
     for (int l = c - 1; l >= 1; l--) {
-        hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(M->lv[l].nc)), dim3(kBlock), 0, ctx->stream, grid_of(M->lv[l]),
-                           grid_of(M->lv[l + 1]));
+        const MgLaunch P = mg_launch(M->lv[l].nx, M->lv[l].ny, M->lv[l].nz);
+        hipLaunchKernelGGL(k_mg_prolong, P.grid, P.block, 0, ctx->stream, grid_of(M->lv[l]), grid_of(M->lv[l + 1]));
         smooth(ctx, M->lv[l], kMgPostSweeps, false);
     }
-    hipLaunchKernelGGL(k_mg_prolong0, dim3(nblocks(A.nc)), dim3(kBlock), 0, ctx->stream, A, grid_of(M->lv[1]));
+    const MgLaunch P0 = mg_launch(A.nx, A.ny, A.nz);
+    hipLaunchKernelGGL(k_mg_prolong0, P0.grid, P0.block, 0, ctx->stream, A, grid_of(M->lv[1]));
     smooth0(ctx, A, kMgPostSweeps, false);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;

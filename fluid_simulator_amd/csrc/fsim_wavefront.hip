@@ -166,6 +166,14 @@ __device__ inline int z_of(const WaveArgs& A, int K, int tz) {
     return FWD ? 1 + K * WF_TZ + tz : 1 + (A.TK - 1 - K) * WF_TZ + (WF_TZ - 1 - tz);
 }
 
+// 1 if column (ty, tz) of tile (J, K) is interior and holds a WATER cell at any x.
+template <bool FWD>
+__device__ inline int col_water(const WaveArgs& A, int J, int K, int ty, int tz) {
+    const int z = z_of<FWD>(A, K, tz);
+    if (J * WF_TY + ty >= A.ny - 2 || z > A.nz - 2) return 0;
+    return A.colw[y_of<FWD>(A, J, ty) * A.nz + z];
+}
+
 template <int KIND>
 struct WaveLds {
     static constexpr int NP = Kind<KIND>::NP, RINGP = Kind<KIND>::RINGP;
@@ -185,11 +193,11 @@ struct WaveLds {
 // once, so the wave forwards whole steps: step s once all its packets carry
 // this launch's epoch and the ring slot s mod RINGP is free.
 template <int KIND, bool YEDGE>
-__device__ void comm_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
+__device__ void comm_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, bool yup, bool zup) {
     using KD = Kind<KIND>;
     constexpr int NP = KD::NP, B = KD::BATCH, RINGP = KD::RINGP;
     constexpr int NL = YEDGE ? WF_TZ : WF_TY;    // values per step
-    if (YEDGE ? J == 0 : K == 0) return;
+    if (YEDGE ? !yup : !zup) return;   // no upstream tile, or one without WATER (its edge is all zeros)
     const int l = threadIdx.x % 64;
     const bool on = l < NL;
     const int li = on ? l : 0;
@@ -309,7 +317,7 @@ __device__ void send_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
 // In the sweeps a neighbour that is not WATER carries exactly +0, whose term
 // -scale * (+0) = -0 leaves any partial sum unchanged, so they sum all three.
 template <int KIND>
-__device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
+__device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, bool yup, bool zup) {
 #pragma clang fp contract(off)
     using KD = Kind<KIND>;
     using T = typename KD::T;
@@ -367,7 +375,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
         lo4[r] = lo8[r] >> 1;
         lo2[r] = lo8[r] >> 2;
         // lane 0 of the row reads -z packets when there is an upstream z tile
-        colz[r] = K > 0 && rowin;
+        colz[r] = zup && rowin;
     });
     // the rotated plane of the next step; readfirstlane keeps the offset in a
     // scalar register (a per-lane copy turns each buffer access into a loop)
@@ -415,7 +423,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) 
     const uint32_t zo_stride = zl ? NP * TY : 0;   // words per ring slot
     const uint64_t* yp_base = &S.yp[0][0][tz];
     const uint64_t* zp_base = &S.zp[0][0][w * R];
-    const bool cy = w == 0 && J > 0;
+    const bool cy = w == 0 && yup;
     T prev[R];
     unroll<R>([&](auto rc) { prev[decltype(rc)::value] = 0; });
     // Upstream values of one step: the previous wave's last row (ring slot),
@@ -643,9 +651,23 @@ __global__ void __launch_bounds__(WF_THREADS) k_wave(WaveArgs A) {
     const int tile = take_tile(A);
     if (tile < 0) return;
     const int J = tile / A.TK, K = tile % A.TK;
+    // A tile none of whose (y, z) columns holds WATER has nothing to compute
+    // (its cells' values are 0, and only WATER cells are stored): it leaves at
+    // once.  Its downstream tiles take its edge values as the zeros they are,
+    // exactly as on a first tile, instead of waiting for packets.
+    int mine = 0, upy = 0, upz = 0;
+    for (int c = threadIdx.x; c < WF_TY * WF_TZ; c += WF_THREADS) {
+        const int ty = c / WF_TZ, tz = c % WF_TZ;
+        mine |= col_water<KD::FWD>(A, J, K, ty, tz);
+        if (J > 0) upy |= col_water<KD::FWD>(A, J - 1, K, ty, tz);
+        if (K > 0) upz |= col_water<KD::FWD>(A, J, K - 1, ty, tz);
+    }
+    if (!__syncthreads_or(mine)) return;
+    const bool yup = __syncthreads_or(upy) != 0, zup = __syncthreads_or(upz) != 0;
     for (int i = threadIdx.x; i < kRingV * WF_NW * NP * WF_TZ; i += WF_THREADS) (&S.v[0][0][0][0])[i] = ~0ull;
-    // sweeps on a first y / z tile read their -y / -z values from a ring of zeros (never refilled)
-    const uint64_t yp_init = KIND != 0 && J == 0 ? 0ull : ~0ull, zp_init = KIND != 0 && K == 0 ? 0ull : ~0ull;
+    // sweeps on a first y / z tile, or below / beside a tile without WATER, read their -y / -z
+    // values from a ring of zeros (never refilled)
+    const uint64_t yp_init = KIND != 0 && !yup ? 0ull : ~0ull, zp_init = KIND != 0 && !zup ? 0ull : ~0ull;
     for (int i = threadIdx.x; i < RINGP * NP * WF_TZ; i += WF_THREADS) (&S.yp[0][0][0])[i] = yp_init;
     for (int i = threadIdx.x; i < RINGP * NP * WF_TY; i += WF_THREADS) (&S.zp[0][0][0])[i] = zp_init;
     for (int i = threadIdx.x; i < kRingO * NP * WF_TZ; i += WF_THREADS) (&S.yo[0][0][0])[i] = ~0ull;
@@ -658,9 +680,9 @@ __global__ void __launch_bounds__(WF_THREADS) k_wave(WaveArgs A) {
     }
     __syncthreads();
     const int wave = threadIdx.x / 64;
-    if (wave < WF_NW) compute_wave<KIND>(A, S, J, K);
-    else if (wave == WF_NW) comm_wave<KIND, true>(A, S, J, K);
-    else if (wave == WF_NW + 1) comm_wave<KIND, false>(A, S, J, K);
+    if (wave < WF_NW) compute_wave<KIND>(A, S, J, K, yup, zup);
+    else if (wave == WF_NW) comm_wave<KIND, true>(A, S, J, K, yup, zup);
+    else if (wave == WF_NW + 1) comm_wave<KIND, false>(A, S, J, K, yup, zup);
     else if (wave == WF_NW + 2) send_wave<KIND, true>(A, S, J, K);
     else send_wave<KIND, false>(A, S, J, K);
 }

@@ -304,7 +304,12 @@ typedef enum fsim_option {
     FSIM_OPT_LEARNED_MIC_COUNT = 6,
     /* learned preconditioner: how the coarsest multigrid level is solved,
      * fsim_mg_coarse (default FSIM_MG_COARSE_NETWORK) */
-    FSIM_OPT_MG_COARSE = 7
+    FSIM_OPT_MG_COARSE = 7,
+    /* PCG reliable updates: the FP64 true residual is recomputed (and the
+     * iteration restarted from it) whenever the FP32 recursive residual has
+     * dropped by 10^value since the last true residual, 1 .. 15 (default 3);
+     * 0: only when the recursive residual meets the tolerance */
+    FSIM_OPT_RELIABLE_DROP = 8
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
 

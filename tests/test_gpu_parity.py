@@ -192,8 +192,9 @@ def test_solve_parity(gpu, rule):
     for f in (capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2):
         assert field_rel_err(sim.get_grid(f), ref.get_grid(f)) < 1e-6, f
     st_gpu, st_ref = sim.stats(), ref.stats()
-    # FP32 inner iterations with FP64 restarts need at most a few more iterations
-    assert st_ref.pcg_iterations - 2 <= st_gpu.pcg_iterations <= 1.5 * st_ref.pcg_iterations + 5
+    # FP32 inner iterations with FP64 restarts: the counts agree within max(3, 5 %)
+    say(f"  PCG iterations: GPU {st_gpu.pcg_iterations}, oracle {st_ref.pcg_iterations}")
+    assert abs(st_gpu.pcg_iterations - st_ref.pcg_iterations) <= max(3, 0.05 * st_ref.pcg_iterations)
     if rule == capi.STOP_REL_L2:
         assert st_gpu.residual_rel_l2 <= 1e-10
     else:
@@ -320,17 +321,24 @@ def test_config_128_apic_sphere_whole_steps(gpu):
     sim.close()
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (21, 23, 70), (9, 37, 131)])
-def test_wavefront_equals_planes(gpu, shape):
+@pytest.mark.parametrize("shape,subbox", [((16, 16, 16), False), ((21, 23, 70), False), ((9, 37, 131), False),
+                                          ((40, 50, 200), True)])
+def test_wavefront_equals_planes(gpu, shape, subbox):
     """The persistent wavefront kernels (tiles, boundary packets, tickets)
     give bit-identical factor and sweeps to one launch per i+j+k plane,
-    including partial tiles along y and z."""
+    including partial tiles along y and z.  The WATER of the last shape fills
+    only a sub-box (x 10-29, y 20-29, z 70-139), so tiles without WATER lie
+    upstream of WATER tiles along y and z in both sweep directions: they are
+    skipped and their edge values taken as zeros."""
     nx, ny, nz = shape
-    say(f"case: wavefront vs planes {shape}")
+    say(f"case: wavefront vs planes {shape}, water {'in a sub-box' if subbox else 'in the lower two thirds'}")
     grid = capi.Grid(nx, ny, nz, 1.0 / 32, 0.25 / 32)
     rng = np.random.default_rng(sum(shape))
     lab = np.full(shape, capi.AIR, np.uint8)
-    lab[:, : ny * 2 // 3] = capi.WATER
+    if subbox:
+        lab[10:30, 20:30, 70:140] = capi.WATER
+    else:
+        lab[:, : ny * 2 // 3] = capi.WATER
     lab[rng.random(shape) < 0.1] = capi.AIR
     lab[rng.random(shape) < 0.05] = capi.SOLID
     lab[0], lab[-1], lab[:, 0], lab[:, :, 0], lab[:, :, -1] = (capi.SOLID,) * 5
@@ -344,7 +352,8 @@ def test_wavefront_equals_planes(gpu, shape):
         sim.close()
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
-    assert np.count_nonzero(out[0][1]) > 0.3 * grid.nc
+    water = np.count_nonzero(lab == capi.WATER)
+    assert np.count_nonzero(out[0][1]) > 0.8 * water
 
 
 def test_whole_step_wavefront_equals_planes(gpu):
@@ -411,20 +420,32 @@ def test_config_256_flip_double_dam_break_properties(gpu):
         del key, start
         lab = sim.get_grid(capi.FIELD_LABEL)
         assert np.array_equal(lab == capi.WATER, count > 0)
-        # the FP64 residual b - A p and the divergence of the projected faces agree up to the FP32
-        # rounding of those faces: half an ulp per face, two roundings in u2 + s (p_i - p_j), six
-        # faces per cell, summed in L2 over the WATER cells
+        # the FP64 residual b - A p and the divergence of the projected faces differ by the FP32
+        # rounding of those faces: two roundings in u2 + s (p_i - p_j), each within half an ulp.
+        # Worst case (all aligned): one ulp per face, six faces summed per cell, L2 over the WATER
+        # cells.  Bound used: 1.5 times the expected L2 error of independent roundings, each uniform
+        # within half an ulp (variance ulp^2 / 12 per rounding, ulp^2 / 6 per face); over the 7.9M
+        # WATER cells the squared L2 norm has a relative spread of about N^-1/2 ~ 4e-4, so the norm
+        # stays within 0.1 % of that expectation.  That bound is itself about 1e-6 of |b| here (0.8e-6
+        # at the first step, 1.6e-6 at the second): FP32 faces cannot show a divergence below it, while
+        # the FP64 residual b - A p, the divergence of the exactly projected faces, is <= 1e-6.
         water = (lab == capi.WATER).reshape(grid.shape)
         del lab, count
-        halfulp = np.zeros(grid.shape)
+        worst = np.zeros(grid.shape)
+        var = np.zeros(grid.shape)
         for ax, f in enumerate((capi.FIELD_U2, capi.FIELD_V2, capi.FIELD_W2)):
             h = np.spacing(np.abs(sim.get_grid(f))).reshape(grid.shape).astype(np.float64)
-            halfulp += h + np.roll(h, 1, axis=ax)   # 2 x half an ulp, +a and -a face
-            del h
-        bound = float(np.sqrt(np.sum(np.where(water, halfulp, 0.0) ** 2))) / grid.cell_d / st.b_norm_l2
-        del halfulp, water
+            hm = np.roll(h, 1, axis=ax)
+            worst += h + hm                  # +a and -a face, one ulp each
+            var += (h * h + hm * hm) / 6.0
+            del h, hm
+        scale = grid.cell_d * st.b_norm_l2
+        worst_bound = float(np.sqrt(np.sum(np.where(water, worst, 0.0) ** 2))) / scale
+        bound = 1.5 * float(np.sqrt(np.sum(np.where(water, var, 0.0)))) / scale
+        del worst, var, water
         div = bench.divergence_l2(sim, grid) / st.b_norm_l2
-        say(f"  divergence / |b| {div:.3e}, residual {st.residual_rel_l2:.3e}, rounding bound {bound:.2e}")
+        say(f"  divergence / |b| {div:.3e}, residual {st.residual_rel_l2:.3e}, rounding bound {bound:.2e} "
+            f"(worst case {worst_bound:.2e})")
         assert abs(div - st.residual_rel_l2) <= bound
         assert div <= 1e-6 + bound
     sim.close()
