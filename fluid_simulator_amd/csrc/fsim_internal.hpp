@@ -250,6 +250,8 @@ struct fsim_ctx {
     fsim::MgState* mg = nullptr;          // multigrid hierarchy of the learned preconditioner
     int mg_coarse = 0;                    // coarsest-level solve: FSIM_MG_COARSE_*
     int reliable_drop = 3;                // PCG: true residual after a 10^drop drop of the recursive one (0: at the tolerance)
+    int mg_graph = 1;                     // replay the multigrid V-cycle as a HIP graph
+    int learned_gen = 0;                  // bumped when the network's per-size buffers are reallocated
     bool mg_harvesting = false;           // a solve run by fsim_learned_harvest
     int partials_cap = 0;
     fsim::SolveScalars* scal = nullptr;   // device
@@ -377,6 +379,7 @@ void mg_free(fsim_ctx* ctx);
 int learned_apply_level(fsim_ctx* ctx, int nx, int ny, int nz, const float* b, const float* diag, const float* ox,
                         const float* oy, const float* oz, double norm, float* x);
 // the network on the solve's own grid (rotated layout): z = out * max|r| * binv
+int learned_prepare(fsim_ctx* ctx, int nx, int ny, int nz);
 int learned_apply_level0(fsim_ctx* ctx, const float* r, float* z, float binv);
 int64_t learned_num_params();
 int learned_init(fsim_ctx* ctx, uint64_t seed);

@@ -92,6 +92,8 @@ struct ConvArgs {
     const __bf16* xc;     // coarse parent [V/8][cu] when cu > 0
     int cu;
     int lds;              // 1: the LDS-staged kernel where the layer and the level fit it
+    float* part;          // split-K partial sums [slice][V][cout] (null: no split)
+    int64_t part_cap;     // floats in part
 };
 
 template <int CIN, int CU, int NT, int MT, bool OUT_F32>
@@ -126,8 +128,11 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int j = 0; j < NT; j++) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // split K: block z sums the K chunks of its slice (one slice: all of them)
+    const int cps = (NCH + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int c_lo = (int)blockIdx.z * cps, c_hi = min(NCH, c_lo + cps);
 #pragma unroll 2
-    for (int c = 0; c < NCH; c++) {
+    for (int c = c_lo; c < c_hi; c++) {
         const int k0 = 32 * c + 8 * kq;
         const bool kin = (KTOT % 32 == 0) || k0 < KTOT;
         const int t = k0 / CIN, ci = k0 - t * CIN;
@@ -152,6 +157,19 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
 #pragma unroll
             for (int j = 0; j < NT; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[j], acc[m][j], 0, 0, 0);
     }
+    if (gridDim.z > 1) {   // raw partial sums; k_conv3_reduce adds the slices, bias and ReLU
+        float* P = a.part + (int64_t)blockIdx.z * V * a.cout;
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int64_t v = m0 + 16 * m + 4 * kq + i;
+                    if (v < V) P[v * a.cout + co0 + 16 * j + (lane & 15)] = acc[m][j][i];
+                }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < NT; j++) {
         const int co = co0 + 16 * j + (lane & 15);
@@ -170,13 +188,49 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     }
 }
 
+// Sum of the split-K slices in slice order, then bias and ReLU (deterministic).
+template <bool OUT_F32>
+__global__ void __launch_bounds__(256) k_conv3_reduce(ConvArgs a, int slices) {
+    const int64_t n = (int64_t)a.D * a.H * a.W * a.cout;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < slices; k++) s += a.part[k * n + i];
+    float o = s + (a.bias ? a.bias[i % a.cout] : 0.f);
+    if (a.relu) o = fmaxf(o, 0.f);
+    if (OUT_F32) static_cast<float*>(a.y)[i] = o;
+    else static_cast<__bf16*>(a.y)[i] = (__bf16)o;
+}
+
+// Small levels (the coarsest multigrid level's network runs at 16^3 and below)
+// give a layer a handful of waves, each walking up to 162 K chunks with its
+// loads in series: such layers split K over blockIdx.z, about kSplitWaves
+// waves in all with at least 3 chunks per slice, as far as the partial-sum
+// buffer holds them.  Layers with fewer than kSplitMinChunks chunks do not
+// split: the reduction's launch costs more than their short K loop (measured
+// at 16^3: 8 -> 16, 7 chunks, 10.0 us unsplit, 7.6 + 7.3 us split).
+constexpr int64_t kSplitWaves = 1024;
+constexpr int kSplitMinChunks = 24;
+constexpr int64_t kConvPartFloats = int64_t(4) << 20;   // 16 MB of split-K partial sums per context
 template <int CIN, int CU, int COUT, int NT, int MT, bool OUT_F32>
 static void conv3_launch(hipStream_t st, const ConvArgs& a) {
     static_assert(COUT % (16 * NT) == 0 && CIN % 8 == 0 && CU % 8 == 0, "conv3 tile shape");
+    constexpr int NCH = (27 * CIN + 31) / 32;
     const int64_t V = (int64_t)a.D * a.H * a.W;
     const int64_t waves = (V + 16 * MT - 1) / (16 * MT);
-    hipLaunchKernelGGL((k_conv3<CIN, CU, NT, MT, OUT_F32>), dim3((unsigned)((waves + 3) / 4), COUT / (16 * NT)),
-                       dim3(256), 0, st, a);
+    const int64_t all = waves * (COUT / (16 * NT));
+    int64_t slices = 1;
+    if (a.part && all < kSplitWaves && NCH >= kSplitMinChunks) {
+        slices = std::min<int64_t>({(kSplitWaves + all - 1) / all, NCH / 3, a.part_cap / (V * COUT)});
+        slices = std::max<int64_t>(slices, 1);
+        const int64_t cps = (NCH + slices - 1) / slices;   // every slice gets at least one chunk
+        slices = (NCH + cps - 1) / cps;
+    }
+    hipLaunchKernelGGL((k_conv3<CIN, CU, NT, MT, OUT_F32>),
+                       dim3((unsigned)((waves + 3) / 4), COUT / (16 * NT), (unsigned)slices), dim3(256), 0, st, a);
+    if (slices > 1)
+        hipLaunchKernelGGL((k_conv3_reduce<OUT_F32>), dim3((unsigned)((V * COUT + 255) / 256)), dim3(256), 0, st, a,
+                           (int)slices);
 }
 
 // LDS-staged variant for the large levels (rows a multiple of 32 voxels).  A block owns TY rows x 32
@@ -764,6 +818,8 @@ struct LearnedNet {
     __bf16 *x0 = nullptr, *e[4] = {}, *pl[3] = {}, *cat[3] = {}, *dd[3] = {};   // cat/dd index = level
     float *out = nullptr, *scale = nullptr;
     unsigned* scale_bits = nullptr;
+    float* part = nullptr;            // split-K partial sums of the small levels' convolutions
+    int64_t part_cap = 0;
     // training (per size)
     bool train_ready = false;
     float *in_b = nullptr, *in_op = nullptr, *target = nullptr, *dout = nullptr, *dd0 = nullptr;
@@ -809,7 +865,9 @@ static int learned_alloc_params(fsim_ctx* ctx) {
     ctx->learned = N;
     param_offsets(N->woff, N->boff, &N->out_w, &N->out_b, &N->nparams);
     int rc;
-    if ((rc = nalloc(ctx, N->pallocs, &N->w32, N->nparams)) || (rc = nalloc(ctx, N->pallocs, &N->w16, N->nparams)))
+    N->part_cap = kConvPartFloats;
+    if ((rc = nalloc(ctx, N->pallocs, &N->w32, N->nparams)) || (rc = nalloc(ctx, N->pallocs, &N->w16, N->nparams)) ||
+        (rc = nalloc(ctx, N->pallocs, &N->part, N->part_cap)))
         return rc;
     return FSIM_OK;
 }
@@ -821,6 +879,7 @@ static int learned_alloc(fsim_ctx* ctx, int nx, int ny, int nz) {
     if (N->gx == nx && N->gy == ny && N->gz == nz) return FSIM_OK;
     FSIM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_sized(N);
+    ctx->learned_gen++;   // a captured V-cycle that used the old buffers is stale
     auto r8 = [](int n) { return (n + 7) / 8 * 8; };
     N->lv[0] = Dims{r8(nx), r8(ny), r8(nz)};
     for (int l = 1; l < 4; l++) N->lv[l] = Dims{N->lv[l - 1].D / 2, N->lv[l - 1].H / 2, N->lv[l - 1].W / 2};
@@ -894,7 +953,7 @@ static ConvArgs conv_args(const LearnedNet* N, int l, const __bf16* x, void* y, 
     const Dims d = N->lv[kLayers[l].level];
     const int cu = xc ? kLayers[l].cin - kLayers[l].cin / 3 : 0;
     return ConvArgs{x, N->w16 + N->woff[l], N->w32 + N->boff[l], y, d.D, d.H, d.W, kLayers[l].cin, kLayers[l].cout, 1,
-                    xc, cu, lds};
+                    xc, cu, lds, N->part, N->part_cap};
 }
 
 // forward from x0 (already packed) to out; for training the concatenated
@@ -990,7 +1049,7 @@ int learned_apply_level(fsim_ctx* ctx, int nx, int ny, int nz, const float* b, c
     LearnedNet* N = ctx->learned;
     hipStream_t st = ctx->stream;
     const int64_t nc = (int64_t)nx * ny * nz;
-    FSIM_HIP(ctx, hipMemsetAsync(N->scale_bits, 0, sizeof(unsigned), st));
+    FSIM_TRY(fill_i32(ctx, reinterpret_cast<int32_t*>(N->scale_bits), 0, 1));   // a kernel: graph-capturable
     hipLaunchKernelGGL(k_absmax_op, dim3(std::min<unsigned>(gb(nc), 2048u)), dim3(256), 0, st, b, diag, nc, N->scale_bits);
     const float* sc = reinterpret_cast<const float*>(N->scale_bits);   // the max as a float
     hipLaunchKernelGGL(k_net_input_op, dim3(gb(N->lv[0].V())), dim3(256), 0, st, b, diag, ox, oy, oz, (float)norm, sc,
@@ -1002,6 +1061,10 @@ int learned_apply_level(fsim_ctx* ctx, int nx, int ny, int nz, const float* b, c
     return FSIM_OK;
 }
 
+// The network's buffers for a level of nx x ny x nz cells, made before a
+// multigrid V-cycle is captured into a graph (nothing may allocate then).
+int learned_prepare(fsim_ctx* ctx, int nx, int ny, int nz) { return learned_alloc(ctx, nx, ny, nz); }
+
 // The network on the solve's own grid (rotated r and z): z = WATER ? out * max|r| * binv : 0
 int learned_apply_level0(fsim_ctx* ctx, const float* r, float* z, float binv) {
     FSIM_PROF(ctx, KID_LEARNED);
@@ -1009,7 +1072,7 @@ int learned_apply_level0(fsim_ctx* ctx, const float* r, float* z, float binv) {
     LearnedNet* N = ctx->learned;
     hipStream_t st = ctx->stream;
     const int64_t nc = ctx->g.nc;
-    FSIM_HIP(ctx, hipMemsetAsync(N->scale_bits, 0, sizeof(unsigned), st));
+    FSIM_TRY(fill_i32(ctx, reinterpret_cast<int32_t*>(N->scale_bits), 0, 1));   // a kernel: graph-capturable
     hipLaunchKernelGGL(k_absmax, dim3(std::min<unsigned>(gb(nc), 2048u)), dim3(256), 0, st, r, ctx->nbinfo, nc,
                        N->scale_bits);
     static_assert(sizeof(unsigned) == sizeof(float), "scale bits");
@@ -1170,7 +1233,7 @@ static int dgrad(fsim_ctx* ctx, int l, const __bf16* g16, float* dx) {
     const int cin = kLayers[l].cin, cout = kLayers[l].cout;
     hipLaunchKernelGGL(k_flip_weights, dim3(gb((int64_t)cout * 27 * cin)), dim3(256), 0, ctx->stream, N->w16 + N->woff[l],
                        N->wflip, cin, cout);
-    ConvArgs a{g16, N->wflip, nullptr, dx, d.D, d.H, d.W, cout, cin, 0, nullptr, 0, 0};
+    ConvArgs a{g16, N->wflip, nullptr, dx, d.D, d.H, d.W, cout, cin, 0, nullptr, 0, 0, N->part, N->part_cap};
     if (!launch_conv3<true>(ctx->stream, a))
         return set_err(ctx, FSIM_ERR_INVALID, "learned network: convolution shape without a kernel");
     FSIM_HIP(ctx, hipGetLastError());

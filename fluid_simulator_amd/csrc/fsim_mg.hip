@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "fsim_internal.hpp"
@@ -157,15 +158,19 @@ static MgLaunch mg_launch(int nx, int ny, int nz) {
 }
 
 // FIRST: the first half-sweep from x = 0 (no neighbour terms; the other colour
-// is set to 0 on the level's unknowns).
-template <int PAR, bool FIRST>
+// is set to 0 on the level's unknowns).  ROWS (even nx, not FIRST): the grid
+// covers only the rows of colour PAR: x + z = p + 1 (mod 2) on every rotated
+// row, since (z - 1) mod 64 has the parity of z - 1 and the wrap adds nx.
+template <int PAR, bool FIRST, bool ROWS = false>
 __global__ void __launch_bounds__(kMgBz * kMgBy) k_mg_rbgs0(Mg0 A) {
+    static_assert(!(FIRST && ROWS), "the first half-sweep also clears the other colour");
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int p = blockIdx.z;
+    const int yr = blockIdx.y * blockDim.y + threadIdx.y;
+    const int y = ROWS ? 2 * yr + ((PAR + p + 1) & 1) : yr;
     if (z >= A.nz || y >= A.ny) return;
     const int x = unrot_x(p, z, A.nx);
-    const bool mine = ((x + y + z) & 1) == PAR;
+    const bool mine = ROWS || ((x + y + z) & 1) == PAR;
     if (!FIRST && !mine) return;
     const int64_t i = ((int64_t)p * A.ny + y) * A.nz + z;
     const uint16_t nb = A.nbinfo[i];
@@ -217,26 +222,31 @@ __global__ void __launch_bounds__(kMgBz * kMgBy) k_mg_rbgs(MgGrid G) {
 }
 
 // ---- residual + restriction -----------------------------------------------------------
-// b_{l+1}[I] = sum over the fluid children i of (b_i - (L x)_i)
-__global__ void __launch_bounds__(kBlock) k_mg_restrict0(Mg0 A, MgGrid C) {
-    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= (int64_t)C.nx * C.ny * C.nz) return;
-    if (C.diag[I] == 0.f) {
-        C.b[I] = 0.f;
-        return;
-    }
-    int X, Y, Z;
-    unlin(I, C.ny, C.nz, X, Y, Z);
-    const SolveArgs S = mg0_args(A);
-    float sum = 0.f;
-    for (int k = 0; k < 8; k++) {
-        const int x = 2 * X + (k >> 2), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k & 1);
-        if (x >= A.nx || y >= A.ny || z >= A.nz) continue;
-        const int64_t i = ridx(x, y, z, A.nx, A.ny, A.nz);
-        const uint16_t nb = A.nbinfo[i];
-        if (!(nb & NB_FLUID)) continue;
-        const int p = rot_plane(x, z, A.nx);
-        const RNeighbours n = rneighbours(S, i, p, y, z);
+// b_{l+1}[I] = sum over the fluid children i of (b_i - (L x)_i).
+// Level 0 takes two passes.  The children of a coarse cell lie in different
+// rotated planes (x + ((z - 1) mod 64) differs between them), so a thread per
+// coarse cell gathering them reads every word from another plane.  Instead,
+// k_mg_resid0 writes the level-0 residual in the rotated layout (0 off the
+// unknowns), a block-coalesced pass like the smoothing; then each block of
+// k_mg_restrict_rot reads the rotated rows that hold a 64 (x) x 2 (y) x 64 (z)
+// fine box, each row a contiguous run along z, into LDS, and sums every
+// coarse cell's 8 children there in a fixed order.
+// The residual follows the last pre-smoothing half-sweep, which solved the
+// rows of colour 0 exactly: their residual is 0 (up to rounding) and ROWS
+// (even nx, whose rotated rows each have one colour) computes only the rows
+// of colour 1; the buffer's colour-0 rows keep the zeros of its allocation.
+template <bool ROWS>
+__global__ void __launch_bounds__(kMgBz * kMgBy) k_mg_resid0(Mg0 A, float* __restrict__ res) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.z;
+    const int yr = blockIdx.y * blockDim.y + threadIdx.y;
+    const int y = ROWS ? 2 * yr + (p & 1) : yr;   // colour 1: p + 1 + y odd
+    if (z >= A.nz || y >= A.ny) return;
+    const int64_t i = ((int64_t)p * A.ny + y) * A.nz + z;
+    const uint16_t nb = A.nbinfo[i];
+    float r = 0.f;
+    if (nb & NB_FLUID) {
+        const RNeighbours n = rneighbours(mg0_args(A), i, p, y, z);
         float ax = (float)nb_nonsolid(nb) * A.x[i];
         if (nb & NB_MX) ax -= A.x[n.mx];
         if (nb & NB_PX) ax -= A.x[n.px];
@@ -244,9 +254,58 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict0(Mg0 A, MgGrid C) {
         if (nb & NB_PY) ax -= A.x[n.py];
         if (nb & NB_MZ) ax -= A.x[n.mz];
         if (nb & NB_PZ) ax -= A.x[n.pz];
-        sum += A.b[i] * A.binv - ax;
+        r = A.b[i] * A.binv - ax;
     }
-    C.b[I] = sum;
+    res[i] = r;
+}
+
+constexpr int kRtC = 32;   // coarse cells per block along x and along z (a 64 x 2 x 64 fine box)
+constexpr int kRtBatch = 8;   // rotated rows a wave loads before storing them
+__global__ void __launch_bounds__(256) k_mg_restrict_rot(Mg0 A, const float* __restrict__ res, MgGrid C) {
+    // fine box [yy][x - fx0][z - fz0]; a row of 64 words: the lanes of one rotated
+    // row store with a stride of -63 words, a different bank each
+    __shared__ float F[2][2 * kRtC][2 * kRtC];
+    const int X0 = blockIdx.x * kRtC, Y = blockIdx.y, Z0 = blockIdx.z * kRtC;
+    const int fx0 = 2 * X0, fz0 = 2 * Z0;
+    for (int k = threadIdx.x; k < 2 * 2 * kRtC * 2 * kRtC; k += blockDim.x) (&F[0][0][0])[k] = 0.f;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int z = fz0 + lane;
+    const int rz = rot_of(z);
+    // the cells x in [fx0, fx0 + 64) of column z are in the rotated rows fx0 + rz .. fx0 + rz + 63:
+    // rows fx0 .. fx0 + 126 cover the box; each wave loads kRtBatch rows, then stores them
+    constexpr int kRows = 2 * kRtC + 63;
+    for (int k0 = wave * kRtBatch; k0 < 2 * kRows; k0 += nwaves * kRtBatch) {
+        float v[kRtBatch];
+        int dst[kRtBatch];
+#pragma unroll
+        for (int j = 0; j < kRtBatch; j++) {
+            const int k = k0 + j;
+            const int yy = k / kRows, pr = fx0 + k % kRows;
+            const int y = 2 * Y + yy;
+            const int x = pr - rz;
+            const bool ok = k < 2 * kRows && y < A.ny && z < A.nz && x >= fx0 && x < fx0 + 2 * kRtC && x < A.nx;
+            const int prow = pr < A.nx ? pr : pr % A.nx;   // rot_plane(x, z)
+            v[j] = ok ? res[((int64_t)prow * A.ny + y) * A.nz + z] : 0.f;
+            dst[j] = ok ? (yy * 2 * kRtC + (x - fx0)) * 2 * kRtC + lane : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kRtBatch; j++)
+            if (dst[j] >= 0) (&F[0][0][0])[dst[j]] = v[j];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kRtC * kRtC; k += blockDim.x) {
+        const int xl = k / kRtC, zl = k % kRtC;
+        const int X = X0 + xl, Z = Z0 + zl;
+        if (X >= C.nx || Z >= C.nz) continue;
+        const int64_t I = lin(C.nx, C.ny, C.nz, X, Y, Z);
+        float sum = 0.f;
+        if (C.diag[I] != 0.f) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) sum += F[(c >> 1) & 1][2 * xl + (c >> 2)][2 * zl + (c & 1)];
+        }
+        C.b[I] = sum;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(MgGrid F, MgGrid C) {
@@ -310,17 +369,36 @@ struct MgLevelBuf {
 struct MgState {
     int nlev = 0;   // levels 0 .. nlev-1; nlev-1 is the coarse (network) level
     MgLevelBuf lv[kMgMaxLevels];
+    float* res0 = nullptr;   // level-0 residual, rotated layout
     std::vector<void*> allocs;
     // harvest: up to cap coarse (b, x) pairs of the running solve
     int harvest_cap = 0, harvest_n = 0;
     float *hb = nullptr, *hx = nullptr;
+    // the V-cycle captured as a HIP graph, with what its kernel arguments were made from
+    hipGraphExec_t exec = nullptr;
+    struct Key {
+        const float* r;
+        float* z;
+        double scale;
+        int mode, nu, conv, net_gen;
+        bool operator==(const Key& o) const {
+            return r == o.r && z == o.z && scale == o.scale && mode == o.mode && nu == o.nu && conv == o.conv &&
+                   net_gen == o.net_gen;
+        }
+    } key{};
 };
 
 static MgGrid grid_of(const MgLevelBuf& L) { return MgGrid{L.nx, L.ny, L.nz, L.diag, L.ox, L.oy, L.oz, L.b, L.x}; }
 
+static void drop_graph(MgState* M) {
+    if (M->exec) (void)hipGraphExecDestroy(M->exec);
+    M->exec = nullptr;
+}
+
 void mg_free(fsim_ctx* ctx) {
     MgState* M = ctx->mg;
     if (!M) return;
+    drop_graph(M);
     for (void* p : M->allocs) (void)hipFree(p);
     delete M;
     ctx->mg = nullptr;
@@ -346,6 +424,12 @@ static int mg_alloc(fsim_ctx* ctx) {
     M->lv[0].ny = ctx->g.ny;
     M->lv[0].nz = ctx->g.nz;
     M->lv[0].nc = ctx->g.nc;
+    if (M->nlev > 1) {
+        if (hipMalloc(reinterpret_cast<void**>(&M->res0), sizeof(float) * (size_t)ctx->g.nc) != hipSuccess)
+            return set_err(ctx, FSIM_ERR_OOM, "multigrid residual allocation failed");
+        M->allocs.push_back(M->res0);
+        FSIM_TRY(fill_f32(ctx, M->res0, 0.0f, ctx->g.nc));   // the colour-0 rows stay 0 (k_mg_resid0)
+    }
     for (int l = 1; l < M->nlev; l++) {
         MgLevelBuf& L = M->lv[l];
         L.nx = (M->lv[l - 1].nx + 1) / 2;
@@ -381,6 +465,11 @@ static Mg0 mg0(fsim_ctx* ctx, const float* r, float* z, double scale) {
 int mg_setup(fsim_ctx* ctx) {
     FSIM_TRY(mg_alloc(ctx));
     MgState* M = ctx->mg;
+    if (ctx->mg_coarse == FSIM_MG_COARSE_NETWORK) {   // the network's buffers exist before any capture
+        const MgLevelBuf& L = M->lv[M->nlev - 1];
+        if (M->nlev > 1) FSIM_TRY(learned_prepare(ctx, L.nx, L.ny, L.nz));
+        else FSIM_TRY(learned_prepare(ctx, ctx->g.nx, ctx->g.ny, ctx->g.nz));
+    }
     FSIM_PROF(ctx, KID_MG);
     if (M->nlev > 1) {
         MgLevelBuf& C = M->lv[1];
@@ -399,10 +488,15 @@ int mg_setup(fsim_ctx* ctx) {
 static void smooth0(fsim_ctx* ctx, const Mg0& A, int sweeps, bool first) {
     const MgLaunch L = mg_launch(A.nx, A.ny, A.nz);
     const dim3 g = L.grid, b = L.block;
+    // even nx: the non-first half-sweeps launch the rows of their colour only
+    const MgLaunch H = mg_launch(A.nx, (A.ny + 1) / 2, A.nz);
+    const bool rows = (A.nx & 1) == 0;
     for (int k = 0; k < sweeps; k++) {
         if (first && k == 0) hipLaunchKernelGGL((k_mg_rbgs0<1, true>), g, b, 0, ctx->stream, A);
+        else if (rows) hipLaunchKernelGGL((k_mg_rbgs0<1, false, true>), H.grid, H.block, 0, ctx->stream, A);
         else hipLaunchKernelGGL((k_mg_rbgs0<1, false>), g, b, 0, ctx->stream, A);
-        hipLaunchKernelGGL((k_mg_rbgs0<0, false>), g, b, 0, ctx->stream, A);
+        if (rows) hipLaunchKernelGGL((k_mg_rbgs0<0, false, true>), H.grid, H.block, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_mg_rbgs0<0, false>), g, b, 0, ctx->stream, A);
     }
 }
 static void smooth(fsim_ctx* ctx, const MgLevelBuf& L, int sweeps, bool first) {
@@ -447,9 +541,8 @@ static int coarse_solve(fsim_ctx* ctx, const Mg0& A, int c) {
 }
 
 // z = V-cycle(r) / s: level-0 vectors in the solver's rotated layout.
-int mg_apply(fsim_ctx* ctx, const float* r, float* z, double scale) {
+static int vcycle(fsim_ctx* ctx, const float* r, float* z, double scale) {
     MgState* M = ctx->mg;
-    FSIM_PROF(ctx, KID_MG);
     const Mg0 A = mg0(ctx, r, z, scale);
     const int c = M->nlev - 1;
     if (c == 0) {   // the grid itself is the coarse level
@@ -459,7 +552,18 @@ int mg_apply(fsim_ctx* ctx, const float* r, float* z, double scale) {
     }
     // down: level 0, then 1 .. c-1
     smooth0(ctx, A, kMgPreSweeps, true);
-    hipLaunchKernelGGL(k_mg_restrict0, dim3(nblocks(M->lv[1].nc)), dim3(kBlock), 0, ctx->stream, A, grid_of(M->lv[1]));
+    {
+        if ((A.nx & 1) == 0) {
+            const MgLaunch R = mg_launch(A.nx, (A.ny + 1) / 2, A.nz);
+            hipLaunchKernelGGL(k_mg_resid0<true>, R.grid, R.block, 0, ctx->stream, A, M->res0);
+        } else {
+            const MgLaunch R = mg_launch(A.nx, A.ny, A.nz);
+            hipLaunchKernelGGL(k_mg_resid0<false>, R.grid, R.block, 0, ctx->stream, A, M->res0);
+        }
+        const MgLevelBuf& L1 = M->lv[1];
+        const dim3 g((unsigned)((L1.nx + kRtC - 1) / kRtC), (unsigned)L1.ny, (unsigned)((L1.nz + kRtC - 1) / kRtC));
+        hipLaunchKernelGGL(k_mg_restrict_rot, g, dim3(256), 0, ctx->stream, A, M->res0, grid_of(L1));
+    }
     for (int l = 1; l < c; l++) {
         smooth(ctx, M->lv[l], kMgPreSweeps, true);
         hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(M->lv[l + 1].nc)), dim3(kBlock), 0, ctx->stream,
@@ -476,6 +580,41 @@ int mg_apply(fsim_ctx* ctx, const float* r, float* z, double scale) {
     hipLaunchKernelGGL(k_mg_prolong0, P0.grid, P0.block, 0, ctx->stream, A, grid_of(M->lv[1]));
     smooth0(ctx, A, kMgPostSweeps, false);
     FSIM_HIP(ctx, hipGetLastError());
+    return FSIM_OK;
+}
+
+// The V-cycle is some 50 launches, most of them small: enqueued one by one they
+// keep the GPU waiting on the host.  With FSIM_OPT_MG_GRAPH (default) it is
+// captured once into a HIP graph and replayed; the kernels' arguments depend
+// only on Key (the buffers are the context's, allocated before the capture),
+// so a new capture is made only when Key changes.  The V-cycle records no
+// memset node (the network clears its max|b| with a fill kernel).  Profiling
+// (per-kernel events) and harvesting (host-side sample counts) run it
+// directly.
+int mg_apply(fsim_ctx* ctx, const float* r, float* z, double scale) {
+    MgState* M = ctx->mg;
+    FSIM_PROF(ctx, KID_MG);
+    if (!ctx->mg_graph || ctx->prof_on || ctx->mg_harvesting) return vcycle(ctx, r, z, scale);
+    const MgState::Key key{r, z, scale, ctx->mg_coarse, ctx->learned_smoothing, ctx->learned_conv_lds, ctx->learned_gen};
+    if (!M->exec || !(key == M->key)) {
+        drop_graph(M);
+        FSIM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = vcycle(ctx, r, z, scale);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != FSIM_OK || ec != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != FSIM_OK ? rc : set_err(ctx, FSIM_ERR_HIP, std::string("V-cycle capture: ") + hipGetErrorString(ec));
+        }
+        const hipError_t ei = hipGraphInstantiate(&M->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            M->exec = nullptr;
+            return set_err(ctx, FSIM_ERR_HIP, std::string("V-cycle graph: ") + hipGetErrorString(ei));
+        }
+        M->key = key;
+    }
+    FSIM_HIP(ctx, hipGraphLaunch(M->exec, ctx->stream));
     return FSIM_OK;
 }
 

@@ -156,13 +156,49 @@ def forward_timing(n: int, reps: int = 10, seed: int = 1, conv: int = 1) -> dict
             "gflop_per_forward": flops_per_apply(grid.shape) / 1e9, "checksum": float(np.abs(out).sum())}
 
 
+def solve_timing(n: int, coarse: int, reps: int = 3, seed: int = 1) -> dict:
+    """Pressure solves of one state of the n^3 FLIP double dam break (3 MIC(0)-PCG steps in), repeated
+    `reps` times with the learned multigrid FPCG and the given coarsest-level mode (capi.MG_COARSE_*;
+    NETWORK uses initial parameters from `seed`, so its iteration count is not a trained one): for
+    profiling the V-cycle's kernels.  Returns the solves' ms and iteration counts."""
+    import time
+    sc = scenes.double_dam_break(n, capi.FLIP, seed=1)
+    sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
+    sim.set_particles(sc.particles())
+    for _ in range(3):
+        st = sim.step(DT)
+    state = sim.get_particles()
+    k_mic = st.pcg_iterations
+    sim.set_params(scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED))
+    sim.set_option(capi.OPT_MG_COARSE, coarse)
+    sim.set_option(capi.OPT_LEARNED_REFRESH, 0)
+    sim.learned_init(seed)
+    out = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.set_particles(*state)
+        sim.set_option(capi.OPT_LEARNED_MIC_COUNT, 10 * k_mic)
+        st = sim.step(DT)
+        out.append({"solve_ms": round(st.solve_ms, 3), "iterations": st.learned_iterations, "stop": st.learned_stop})
+    wall = time.perf_counter() - t0
+    sim.close()
+    return {"grid": [n] * 3, "coarse": coarse, "mic0_iterations": k_mic, "solves": out, "wall_s": round(wall, 2)}
+
+
 if __name__ == "__main__":
     import argparse
     import json
 
-    ap = argparse.ArgumentParser(description="Time the learned network's forward pass (fsim_learned_forward)")
+    ap = argparse.ArgumentParser(description="Time the learned network's forward pass (fsim_learned_forward), "
+                                             "or (--solve) learned multigrid pressure solves")
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--conv", type=int, default=1, choices=(0, 1))
+    ap.add_argument("--solve", action="store_true", help="time solves of a double dam break state instead")
+    ap.add_argument("--coarse", type=int, default=capi.MG_COARSE_SMOOTH,
+                    help="with --solve: coarsest-level mode (0 network, 1 smooth, 2 accurate)")
     a = ap.parse_args()
-    print(json.dumps(forward_timing(a.size, a.reps, conv=a.conv)), flush=True)
+    if a.solve:
+        print(json.dumps(solve_timing(a.size, a.coarse, a.reps)), flush=True)
+    else:
+        print(json.dumps(forward_timing(a.size, a.reps, conv=a.conv)), flush=True)

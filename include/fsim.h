@@ -309,7 +309,11 @@ typedef enum fsim_option {
      * iteration restarted from it) whenever the FP32 recursive residual has
      * dropped by 10^value since the last true residual, 1 .. 15 (default 3);
      * 0: only when the recursive residual meets the tolerance */
-    FSIM_OPT_RELIABLE_DROP = 8
+    FSIM_OPT_RELIABLE_DROP = 8,
+    /* learned preconditioner: 1 (default) replays the multigrid V-cycle as a
+     * HIP graph captured on first use; 0 launches its kernels one by one.
+     * Both give the same bits. */
+    FSIM_OPT_MG_GRAPH = 9
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
 

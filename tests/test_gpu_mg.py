@@ -55,8 +55,11 @@ def gpu_vcycle(lab, vec, coarse_mode, params=None, smoothing=None):
 
 
 @pytest.mark.parametrize("n,mode", [(16, capi.MG_COARSE_SMOOTH), (32, capi.MG_COARSE_SMOOTH),
-                                    (64, capi.MG_COARSE_SMOOTH), (32, capi.MG_COARSE_ACCURATE)])
+                                    (64, capi.MG_COARSE_SMOOTH), (32, capi.MG_COARSE_ACCURATE),
+                                    (33, capi.MG_COARSE_SMOOTH), (131, capi.MG_COARSE_SMOOTH)])
 def test_vcycle_matches_oracle(gpu, n, mode):
+    """One V-cycle against the oracle's.  Odd sides (33, 131) take the per-cell colour test of the
+    level-0 smoothing (even nx launches the rows of one colour only) and partial restriction tiles."""
     say(f"case: one V-cycle {n}^3, coarse mode {mode}")
     lab = dam_labels(n, seed=n)
     ops = mg.hierarchy(lab)
@@ -165,3 +168,37 @@ def test_fpcg_with_multigrid_converges_and_agrees_with_mic0(gpu):
     k_s, k_a = res["smooth"][0].learned_iterations, res["accurate"][0].learned_iterations
     assert k_a * 4 < st_m.pcg_iterations
     assert k_a < k_s < st_m.pcg_iterations
+
+
+@pytest.mark.parametrize("coarse", [capi.MG_COARSE_SMOOTH, capi.MG_COARSE_NETWORK])
+def test_graph_replays_equal_direct_launches(gpu, coarse):
+    """The V-cycle replayed as a HIP graph (FSIM_OPT_MG_GRAPH = 1) against its kernels launched one by one:
+    three whole steps of a 64^3 dam break with the learned FPCG, every V-cycle of every solve a replay
+    with the step's own kernels and host synchronisations between them.  Pressures bit-identical and the
+    same iteration counts at every step."""
+    say(f"case: V-cycle graph replays vs direct launches, 64^3 dam break, 3 steps, coarse mode {coarse}")
+    sc = scenes.dam_break(64, capi.FLIP)
+    pos = sc.particles()
+    grid = sc.grid
+    init = capi.Simulation(grid, scenes.params())
+    init.learned_init(5)
+    params = init.learned_get_params()
+    init.close()
+    out = {}
+    for graph in (0, 1):
+        sim = capi.Simulation(grid, scenes.params(precond=capi.PRECOND_LEARNED))
+        sim.set_option(capi.OPT_MG_COARSE, coarse)
+        sim.set_option(capi.OPT_MG_GRAPH, graph)
+        sim.set_option(capi.OPT_LEARNED_MIC_COUNT, 200)
+        sim.learned_set_params(params)
+        sim.set_particles(pos)
+        res = []
+        for _ in range(3):
+            st = sim.step(DT)
+            res.append((st.learned_iterations, st.learned_stop, sim.get_grid(capi.FIELD_PRESSURE)))
+        out[graph] = res
+        sim.close()
+    for k, (a, b) in enumerate(zip(out[0], out[1])):
+        say(f"  step {k}: FPCG {a[0]} / {b[0]} it (direct / graph)")
+        assert a[0] == b[0] and a[1] == b[1], k
+        assert np.array_equal(a[2], b[2]), k
