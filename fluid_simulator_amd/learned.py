@@ -156,11 +156,11 @@ def forward_timing(n: int, reps: int = 10, seed: int = 1, conv: int = 1) -> dict
             "gflop_per_forward": flops_per_apply(grid.shape) / 1e9, "checksum": float(np.abs(out).sum())}
 
 
-def solve_timing(n: int, coarse: int, reps: int = 3, seed: int = 1) -> dict:
+def solve_timing(n: int, coarse: int, reps: int = 3, seed: int = 1, graph: int = 1) -> dict:
     """Pressure solves of one state of the n^3 FLIP double dam break (3 MIC(0)-PCG steps in), repeated
     `reps` times with the learned multigrid FPCG and the given coarsest-level mode (capi.MG_COARSE_*;
     NETWORK uses initial parameters from `seed`, so its iteration count is not a trained one): for
-    profiling the V-cycle's kernels.  Returns the solves' ms and iteration counts."""
+    profiling the V-cycle's kernels; graph: FSIM_OPT_MG_GRAPH.  Returns the solves' ms and iteration counts."""
     import time
     sc = scenes.double_dam_break(n, capi.FLIP, seed=1)
     sim = capi.Simulation(sc.grid, scenes.params(transfer=sc.transfer))
@@ -172,6 +172,7 @@ def solve_timing(n: int, coarse: int, reps: int = 3, seed: int = 1) -> dict:
     sim.set_params(scenes.params(transfer=sc.transfer, precond=capi.PRECOND_LEARNED))
     sim.set_option(capi.OPT_MG_COARSE, coarse)
     sim.set_option(capi.OPT_LEARNED_REFRESH, 0)
+    sim.set_option(capi.OPT_MG_GRAPH, graph)
     sim.learned_init(seed)
     out = []
     t0 = time.perf_counter()
@@ -182,7 +183,8 @@ def solve_timing(n: int, coarse: int, reps: int = 3, seed: int = 1) -> dict:
         out.append({"solve_ms": round(st.solve_ms, 3), "iterations": st.learned_iterations, "stop": st.learned_stop})
     wall = time.perf_counter() - t0
     sim.close()
-    return {"grid": [n] * 3, "coarse": coarse, "mic0_iterations": k_mic, "solves": out, "wall_s": round(wall, 2)}
+    return {"grid": [n] * 3, "coarse": coarse, "graph": graph, "mic0_iterations": k_mic, "solves": out,
+            "wall_s": round(wall, 2)}
 
 
 if __name__ == "__main__":
@@ -197,8 +199,9 @@ if __name__ == "__main__":
     ap.add_argument("--solve", action="store_true", help="time solves of a double dam break state instead")
     ap.add_argument("--coarse", type=int, default=capi.MG_COARSE_SMOOTH,
                     help="with --solve: coarsest-level mode (0 network, 1 smooth, 2 accurate)")
+    ap.add_argument("--no-graph", action="store_true", help="with --solve: launch the V-cycle's kernels one by one")
     a = ap.parse_args()
     if a.solve:
-        print(json.dumps(solve_timing(a.size, a.coarse, a.reps)), flush=True)
+        print(json.dumps(solve_timing(a.size, a.coarse, a.reps, graph=0 if a.no_graph else 1)), flush=True)
     else:
         print(json.dumps(forward_timing(a.size, a.reps, conv=a.conv)), flush=True)
