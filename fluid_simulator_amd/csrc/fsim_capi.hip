@@ -577,6 +577,10 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
         ctx->last_mic_iterations = value;
         ctx->learned_since_mic = 0;
         return FSIM_OK;
+    case FSIM_OPT_P2G_KERNEL:
+        if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "P2G kernel option is 0 or 1");
+        ctx->p2g_rows = value;
+        return FSIM_OK;
     case FSIM_OPT_MG_GRAPH:
         if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "multigrid graph option is 0 or 1");
         ctx->mg_graph = value;

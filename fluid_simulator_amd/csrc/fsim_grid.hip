@@ -238,6 +238,144 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
     A.density[i] = dens;
 }
 
+// Tiled gather: a block owns TX x TY cell rows (x, y) of 64 cells along z,
+// one wave per cell row and one thread per cell, and stages each of the
+// (TX + 2) x (TY + 2) particle rows around them once, 66 cells of each
+// (k_p2g_gather stages 9 rows for every cell row: each particle crossed HBM
+// about 9 times).  A wave works on a staged row when its cell row's 3 x 3
+// (x, y) neighbourhood holds it.  Rows are staged in (x, y) order, so every
+// cell adds the rows of its neighbourhood in k_p2g_gather's (dx, dy) order,
+// and each row's particles in sorted order, with the same arithmetic: the
+// sums are bit-identical.
+#ifndef FSIM_P2G_TX
+#define FSIM_P2G_TX 2
+#endif
+#ifndef FSIM_P2G_TY
+#define FSIM_P2G_TY 4
+#endif
+constexpr int P2G_TX = FSIM_P2G_TX, P2G_TY = FSIM_P2G_TY, P2G_TZ = 64;
+constexpr int P2G_TILE_CAP = 1024;   // particles per staged chunk (a row segment is 66 cells)
+template <int APIC, int TX, int TY>
+__global__ void __launch_bounds__(P2G_TZ * TX * TY) k_p2g_tile(P2GArgs A) {
+    constexpr int CAP = P2G_TILE_CAP;
+    constexpr int NC = APIC ? 9 : 1;
+    constexpr int NT = P2G_TZ * TX * TY;
+    __shared__ float s_pos[3][CAP], s_vel[3][CAP];
+    __shared__ int s_t[3][CAP];
+    __shared__ uint8_t s_h[CAP];
+    __shared__ float s_c[NC][APIC ? CAP : 1];
+    const GridArgs& G = A.G;
+    const int tid = threadIdx.x + P2G_TZ * threadIdx.y;
+    const int zc0 = blockIdx.x * P2G_TZ;
+    const int ri = threadIdx.y / TY, rj = threadIdx.y % TY;   // this wave's cell row in the tile
+    const int x0 = blockIdx.z * TX, y0 = blockIdx.y * TY;
+    const int x = x0 + ri, y = y0 + rj, z = zc0 + threadIdx.x;
+    const bool valid = x < G.nx && y < G.ny && z < G.nz;
+    const int xc = min(x, G.nx - 1), yc = min(y, G.ny - 1), zc = min(z, G.nz - 1);
+    float fal[3], fce[3];
+    {
+        float f[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) face_pos(G, a, xc, yc, zc, f[a]);
+        fal[0] = f[0][0], fal[1] = f[1][1], fal[2] = f[2][2];
+        fce[0] = f[1][0], fce[1] = f[0][1], fce[2] = f[0][2];
+    }
+    const int cc[3] = {xc, yc, zc};
+    float swv[3] = {0.f, 0.f, 0.f}, sw[3] = {0.f, 0.f, 0.f};
+    float dens = 0.f;
+    for (int rx = x0 - 1; rx <= x0 + TX; rx++) {
+        if (rx < 0 || rx >= G.nx) continue;
+        for (int ry = y0 - 1; ry <= y0 + TY; ry++) {
+            if (ry < 0 || ry >= G.ny) continue;
+            const int64_t rb = ((int64_t)rx * G.ny + ry) * G.nz;
+            // the block's particle range: cells zc0 - 1 .. zc0 + 64 of the row (clamped)
+            const int zlo = max(zc0 - 1, 0), zhi = min(zc0 + P2G_TZ, G.nz - 1);
+            const int u0 = A.start[rb + zlo], u1 = A.start[rb + zhi + 1];
+            if (u0 >= u1) continue;   // block-uniform
+            // this wave works on the row if it is in its cell row's neighbourhood (wave-uniform)
+            const bool near = valid && rx >= x - 1 && rx <= x + 1 && ry >= y - 1 && ry <= y + 1;
+            int j0 = 0, j1 = 0;
+            if (near) {
+                j0 = A.start[rb + (z > 0 ? z - 1 : z)];
+                j1 = A.start[rb + (z < G.nz - 1 ? z + 2 : z + 1)];
+            }
+            for (int base = u0; base < u1; base += CAP) {
+                const int cnt = min(CAP, u1 - base);
+                __syncthreads();   // the previous chunk is consumed
+                for (int k = tid; k < cnt; k += NT) {
+                    const int j = base + k;
+                    uint8_t h = 0;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const float pv = A.pos[d][j];
+                        const double v = (double)pv * G.cdinv;
+                        const int t = (int)v;
+                        s_pos[d][k] = pv;
+                        s_vel[d][k] = A.vel[d][j];
+                        s_t[d][k] = t;
+                        if ((int)(v - 0.5) == t) h |= (uint8_t)(1u << d);
+                    }
+                    s_h[k] = h;
+                    if (APIC) {
+#pragma unroll
+                        for (int m = 0; m < NC; m++) s_c[m][k] = A.c[m][j];
+                    }
+                }
+                __syncthreads();
+                const int e = min(j1, base + cnt);
+#pragma unroll 4
+                for (int j = max(j0, base); j < e; j++) {
+                    const int k = j - base;
+                    const float p[3] = {s_pos[0][k], s_pos[1][k], s_pos[2][k]};
+                    const uint32_t h = s_h[k];
+                    int ua[3], uc[3];
+                    float tal[3], tce[3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        ua[d] = cc[d] - s_t[d][k] + 1;
+                        uc[d] = ua[d] - (int)((h >> d) & 1u);
+                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * G.invf);
+                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * G.invf);
+                    }
+                    if ((unsigned)uc[0] <= 1u && (unsigned)uc[1] <= 1u && (unsigned)uc[2] <= 1u)
+                        dens += tce[0] * tce[1] * tce[2];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        bool in = true;
+                        float tw[3];
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            const int u = d == a ? ua[d] : uc[d];
+                            in = in && (unsigned)u <= 1u;
+                            tw[d] = d == a ? tal[d] : tce[d];
+                        }
+                        if (!in) continue;
+                        const float w = tw[0] * tw[1] * tw[2];
+                        float va = s_vel[a][k];
+                        if (APIC) {
+                            const float f0 = a == 0 ? fal[0] : fce[0], f1 = a == 1 ? fal[1] : fce[1],
+                                        f2 = a == 2 ? fal[2] : fce[2];
+                            const float d0 = f0 - p[0], d1 = f1 - p[1], d2 = f2 - p[2];
+                            va = va + (s_c[3 * a][k] * d0 + s_c[3 * a + 1][k] * d1 + s_c[3 * a + 2][k] * d2);
+                        }
+                        swv[a] += va * w;
+                        sw[a] += w;
+                    }
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const int64_t i = cidx(G, x, y, z);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        A.uvw[a][i] = sw[a] > 1e-6f ? swv[a] / sw[a] : 0.0f;
+        A.uvw2[a][i] = 0.0f;
+    }
+    A.label[i] = (A.fuse_mark && A.count[i] > 0) ? (uint8_t)FSIM_WATER : (uint8_t)FSIM_AIR;
+    A.density[i] = dens;
+}
+
 static int p2g(fsim_ctx* ctx, int fuse_mark) {
     if (ctx->prm.transfer == FSIM_APIC && !ctx->c_valid) {
         // APIC after PIC/FLIP steps: the matrices were not carried, start from 0
@@ -260,8 +398,16 @@ static int p2g(fsim_ctx* ctx, int fuse_mark) {
     A.density = ctx->density;
     A.apic = ctx->prm.transfer == FSIM_APIC;
     A.fuse_mark = fuse_mark;
-    if (A.apic) hipLaunchKernelGGL(k_p2g_gather<1>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
-    else hipLaunchKernelGGL(k_p2g_gather<0>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
+    if (ctx->p2g_rows) {   // FSIM_OPT_P2G_KERNEL = 1: one cell row per block
+        if (A.apic) hipLaunchKernelGGL(k_p2g_gather<1>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
+        else hipLaunchKernelGGL(k_p2g_gather<0>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
+    } else {
+        const dim3 g((unsigned)((ctx->g.nz + P2G_TZ - 1) / P2G_TZ), (unsigned)((ctx->g.ny + P2G_TY - 1) / P2G_TY),
+                     (unsigned)((ctx->g.nx + P2G_TX - 1) / P2G_TX));
+        const dim3 b(P2G_TZ, P2G_TX * P2G_TY);
+        if (A.apic) hipLaunchKernelGGL((k_p2g_tile<1, P2G_TX, P2G_TY>), g, b, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_p2g_tile<0, P2G_TX, P2G_TY>), g, b, 0, ctx->stream, A);
+    }
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }

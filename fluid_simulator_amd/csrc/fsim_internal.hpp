@@ -251,6 +251,7 @@ struct fsim_ctx {
     int mg_coarse = 0;                    // coarsest-level solve: FSIM_MG_COARSE_*
     int reliable_drop = 3;                // PCG: true residual after a 10^drop drop of the recursive one (0: at the tolerance)
     int mg_graph = 1;                     // replay the multigrid V-cycle as a HIP graph
+    int p2g_rows = 0;                     // 1: the one-row-per-block P2G gather instead of the tiled one
     int learned_gen = 0;                  // bumped when the network's per-size buffers are reallocated
     bool mg_harvesting = false;           // a solve run by fsim_learned_harvest
     int partials_cap = 0;

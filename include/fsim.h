@@ -313,7 +313,11 @@ typedef enum fsim_option {
     /* learned preconditioner: 1 (default) replays the multigrid V-cycle as a
      * HIP graph captured on first use; 0 launches its kernels one by one.
      * Both give the same bits. */
-    FSIM_OPT_MG_GRAPH = 9
+    FSIM_OPT_MG_GRAPH = 9,
+    /* P2G gather: 0 (default) blocks of 2 x 4 cell rows staging each particle
+     * row once; 1 one cell row per block (9 stagings per particle).  Both
+     * give the same bits. */
+    FSIM_OPT_P2G_KERNEL = 10
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
 

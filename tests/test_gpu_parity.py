@@ -132,6 +132,31 @@ def test_p2g_mark_parity(gpu, transfer, pressure_enabled):
     assert np.array_equal(sim.get_grid(capi.FIELD_LABEL), ref.get_grid(capi.FIELD_LABEL))
 
 
+@pytest.mark.parametrize("transfer,shape", [(capi.FLIP, (32, 32, 32)), (capi.APIC, (32, 32, 32)),
+                                            (capi.FLIP, (23, 19, 300)), (capi.APIC, (23, 19, 300))])
+def test_p2g_tile_equals_row_gather(gpu, transfer, shape):
+    """The tiled P2G gather (2 x 4 cell rows per block, each particle row staged once) against the
+    one-row-per-block gather (FSIM_OPT_P2G_KERNEL = 1): the same sums in the same order, so faces,
+    density and labels bit-identical.  The second shape has partial tiles along x and y and two
+    256-cell chunks along z."""
+    nx, ny, nz = shape
+    grid = capi.Grid(nx, ny, nz, 1.0 / 32, 0.25 / 32)
+    say(f"case: tiled vs row P2G {shape} transfer={transfer}, seed 7")
+    pos, vel, c = random_particles(grid, 40 * nx * ny, seed=7, apic=transfer == capi.APIC)
+    out = {}
+    for rows in (1, 0):
+        sim = capi.Simulation(grid, scenes.params(transfer=transfer))
+        sim.set_option(capi.OPT_P2G_KERNEL, rows)
+        sim.set_particles(pos, vel, c)
+        sim.run_stage(capi.STAGE_P2G, DT)
+        out[rows] = [sim.get_grid(f) for f in (capi.FIELD_U, capi.FIELD_V, capi.FIELD_W, capi.FIELD_DENSITY,
+                                               capi.FIELD_LABEL)]
+        sim.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    assert np.count_nonzero(out[0][3]) > 0
+
+
 @pytest.mark.parametrize("obs_name", list(OBSTACLE_SETS))
 @pytest.mark.parametrize("top", [0, 1])
 def test_prep_parity(gpu, obs_name, top):
