@@ -34,7 +34,8 @@ def test_pmc_traffic_lookup():
     fwd, bwd = bench.pmc_traffic("sweep_fwd", 256), bench.pmc_traffic("sweep_bwd", 256)
     assert fwd is not None and bwd is not None and fwd > 1e8 and bwd > 1e8
     assert bench.pmc_traffic("sweep_fwd", 128) is None     # another workload
-    assert bench.pmc_traffic("p2g", 256) is None           # not in the table
+    assert bench.pmc_traffic("extrapolate", 256) is None   # not in the table
+    assert bench.pmc_traffic("p2g", 256) > 1e9
 
 
 @pytest.mark.parametrize("tol", [1e-10, 1e-4])
