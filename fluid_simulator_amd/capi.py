@@ -44,7 +44,7 @@ OP_APPLY_A, OP_APPLY_MIC0, OP_FACTOR_MIC0, OP_APPLY_MG = 0, 1, 2, 3
 
 class GridDesc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("cell_d", C.c_double),
-                ("particle_r", C.c_double), ("two_d", C.c_int)]
+                ("particle_r", C.c_double), ("two_d", C.c_int), ("cell_d_z", C.c_double)]
 
 
 class Params(C.Structure):
@@ -191,14 +191,26 @@ def make_obstacle(kind: int, pos, speed=(0.0, 0.0, 0.0), size=(0.0, 0.0, 0.0), r
 
 @dataclass
 class Grid:
+    """MacGrid's size and cell size; two_d: the reference's 2D mode (nz = 3, cell_d_z = the depth / 3)."""
     nx: int
     ny: int
     nz: int
     cell_d: float
     particle_r: float
+    two_d: int = 0
+    cell_d_z: float = 0.0
 
     def desc(self) -> GridDesc:
-        return GridDesc(self.nx, self.ny, self.nz, self.cell_d, self.particle_r, 0)
+        return GridDesc(self.nx, self.ny, self.nz, self.cell_d, self.particle_r, self.two_d, self.cell_d_z)
+
+    @property
+    def cell_dz(self) -> float:
+        """cellD.z (cell_d outside the 2D mode)."""
+        return self.cell_d_z if self.two_d else self.cell_d
+
+    @property
+    def dims(self):
+        return (self.nx * self.cell_d, self.ny * self.cell_d, self.nz * self.cell_dz)
 
     @property
     def nc(self) -> int:

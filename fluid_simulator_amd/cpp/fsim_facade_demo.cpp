@@ -8,6 +8,10 @@
 //   builds HashedParticles(num, ...) on a 32^3 unit grid, grows it to
 //   <grown> and shrinks it to <shrunk> with setParticleNum (host only, no
 //   device needed) and prints the ranges it saw
+//        fsim_facade_demo 2d <n> <steps> <transfer> <num> <init.bin> <out.bin>
+//   the 2D mode: MacGrid(1 x 1 x 0.1, n, twoD), HashedParticles(num, ...,
+//   zConst at the depth / 2) with push-apart on; writes the initial and the
+//   final positions (float32, 3 per particle)
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
@@ -48,7 +52,57 @@ static int particles_demo(int num, int grown, int shrunk) {
     return 0;
 }
 
+static void write_positions(particles::HashedParticles& hp, const char* path) {
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) throw std::runtime_error("cannot open output file");
+    hp.forEachConst([&](const particles::Particle& p) {
+        const float v[3] = {(float)p.pos.x, (float)p.pos.y, (float)p.pos.z};
+        std::fwrite(v, sizeof(float), 3, f);
+    });
+    std::fclose(f);
+}
+
+// The reference's SimulationManager in 2D (simulationManager.cpp:10-27):
+// the grid gets twoD, the particles zConst with z = the target depth / 2.
+static int demo_2d(int n, int steps, int transfer, int num, const char* init, const char* out) {
+    try {
+        const dvec3 target(1, 1, 0.1);
+        auto grid = std::make_shared<macgrid::MacGrid>(target, (double)n, true);
+        grid->pressureEnabled = false;
+        grid->incompressibilityMaxIterationCount = 10000;
+        const double r = grid->cellD.x / 4;
+        auto parts = std::make_shared<particles::HashedParticles>(num, r, grid->dimensions, grid->cellD, true,
+                                                                  target.z / 2, 11);
+        write_positions(*parts, init);
+        simulator::Simulator::SimulatorConfig cfg;
+        cfg.transferType = (simulator::Simulator::P2G2PType)transfer;
+        cfg.flipRatio = 0.95f;
+        cfg.gravity = -9.81f;
+        cfg.pushApartEnabled = true;
+        simulator::Simulator sim(cfg, parts, grid);
+        sim.stopRule = FSIM_STOP_REL_L2;
+        for (int s = 0; s < steps; s++) sim.simulate(1.0 / 240.0);
+        write_positions(*parts, out);
+        int water = 0;
+        for (int x = 0; x < grid->gridSize.x; x++)
+            for (int y = 0; y < grid->gridSize.y; y++)
+                for (int z = 0; z < grid->gridSize.z; z++) water += grid->cellType(x, y, z) == macgrid::CellType::WATER;
+        std::printf("{\"particles\": %zu, \"grid\": [%d, %d, %d], \"cell_d\": [%.17g, %.17g, %.17g], "
+                    "\"dims\": [%.17g, %.17g, %.17g], \"water_cells\": %d, \"pcg_iterations\": %d}\n",
+                    parts->getParticleNum(), grid->gridSize.x, grid->gridSize.y, grid->gridSize.z, grid->cellD.x,
+                    grid->cellD.y, grid->cellD.z, grid->dimensions.x, grid->dimensions.y, grid->dimensions.z, water,
+                    sim.lastStats().pcg_iterations);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 3;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 7 && std::string(argv[1]) == "2d")
+        return demo_2d(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), argv[6],
+                       argv[7]);
     if (argc > 4 && std::string(argv[1]) == "particles")
         return particles_demo(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]));
     const int n = argc > 1 ? std::atoi(argv[1]) : 32;
@@ -90,15 +144,7 @@ int main(int argc, char** argv) {
         for (int x = 0; x < n; x++)
             for (int y = 0; y < n; y++)
                 for (int z = 0; z < n; z++) water += grid->cellType(x, y, z) == macgrid::CellType::WATER;
-        if (out) {
-            std::FILE* f = std::fopen(out, "wb");
-            if (!f) throw std::runtime_error("cannot open output file");
-            parts->forEachConst([&](const particles::Particle& p) {
-                const float v[3] = {(float)p.pos.x, (float)p.pos.y, (float)p.pos.z};
-                std::fwrite(v, sizeof(float), 3, f);
-            });
-            std::fclose(f);
-        }
+        if (out) write_positions(*parts, out);
         const auto d = sim.getStepDuration();
         std::printf("{\"particles\": %zu, \"water_cells\": %d, \"pcg_iterations\": %d, \"ymin\": %.6f, \"ymax\": %.6f, "
                     "\"Incompressibility_us\": %lld, \"P2GTransfer_us\": %lld, \"it_count\": %lld, "

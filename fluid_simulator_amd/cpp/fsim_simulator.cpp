@@ -155,12 +155,12 @@ void HashedParticles::addParticles(const std::vector<Particle>& ps) {
 
 namespace macgrid {
 
-MacGrid::MacGrid(const dvec3& targetDimensions, int resolution, bool twoD_)
-    : cellD(targetDimensions.x / resolution, targetDimensions.x / resolution, targetDimensions.x / resolution),
-      cellDInv(resolution / targetDimensions.x, resolution / targetDimensions.x, resolution / targetDimensions.x),
-      gridSize{resolution, (int)std::lround(targetDimensions.y / (targetDimensions.x / resolution)),
-               (int)std::lround(targetDimensions.z / (targetDimensions.x / resolution))},
-      dimensions(targetDimensions),
+MacGrid::MacGrid(const dvec3& targetDimensions, double resolution, bool twoD_)
+    : cellD(1 / resolution, 1 / resolution, twoD_ ? targetDimensions.z / 3 : 1 / resolution),
+      cellDInv(1.0 / cellD.x, 1.0 / cellD.y, 1.0 / cellD.z),
+      gridSize{(int)(targetDimensions.x / cellD.x), (int)(targetDimensions.y / cellD.y),
+               twoD_ ? 3 : (int)(targetDimensions.z / cellD.z)},
+      dimensions(gridSize.x * cellD.x, gridSize.y * cellD.y, twoD_ ? targetDimensions.z : gridSize.z * cellD.z),
       twoD(twoD_) {}
 
 CellType MacGrid::cellType(int x, int y, int z) {
@@ -225,6 +225,7 @@ void Simulator::createContext() {
     g.cell_d = macGrid->cellD.x;
     g.particle_r = hashedParticles->getParticleR();
     g.two_d = macGrid->twoD ? 1 : 0;
+    g.cell_d_z = macGrid->twoD ? macGrid->cellD.z : 0.0;
     const fsim_params p = makeParams();
     check(fsim_create(&g, &p, &ctx_), ctx_, "fsim_create");
     hashedParticles->device_stale_ = true;

@@ -153,8 +153,10 @@ enum class CellType { WATER = FSIM_WATER, AIR = FSIM_AIR, SOLID = FSIM_SOLID };
 
 class MacGrid {
 public:
-    // MacGrid(targetDimensions, resolution) with equal cubic cells (macGrid.cpp:9-13)
-    MacGrid(const dvec3& targetDimensions, int resolution, bool twoD = false);
+    // macGrid.cpp:9-13: cellD = 1 / resolution on every axis, gridSize =
+    // targetDimensions / cellD (truncated), dimensions = gridSize * cellD.
+    // 2D: z is 3 cells of the target depth / 3, and dimensions.z = the depth.
+    MacGrid(const dvec3& targetDimensions, double resolution, bool twoD = false);
     virtual ~MacGrid() = default;
 
     const dvec3 cellD;
@@ -191,7 +193,7 @@ private:
 class BridsonSolverGrid : public MacGrid {
 public:
     BridsonSolverGrid(const dvec3& targetDimensions, float resolution, bool twoD, double fluidDensity_ = 1.0)
-        : MacGrid(targetDimensions, (int)resolution, twoD) {
+        : MacGrid(targetDimensions, resolution, twoD) {
         fluidDensity = fluidDensity_;
     }
 };

@@ -78,19 +78,25 @@ using namespace fsim;
 
 static int check_grid(const fsim_grid_desc* g) {
     if (!g) return FSIM_ERR_INVALID;
-    if (g->two_d) return FSIM_ERR_UNSUPPORTED;
-    if (g->nx < 4 || g->ny < 4 || g->nz < 4 || !(g->cell_d > 0.0) || !(g->particle_r >= 0.0)) return FSIM_ERR_INVALID;
+    const bool two_d = g->two_d != 0;
+    if (g->nx < 4 || g->ny < 4 || (two_d ? g->nz != 3 : g->nz < 4) || !(g->cell_d > 0.0) || !(g->particle_r >= 0.0))
+        return FSIM_ERR_INVALID;
+    // 2D: the z cell size is the target depth / 3 (macGrid.cpp:10); 3D: cellD.z == cellD.x
+    if (two_d ? !(g->cell_d_z > 0.0) : (g->cell_d_z != 0.0 && g->cell_d_z != g->cell_d)) return FSIM_ERR_INVALID;
     if ((int64_t)g->nx * g->ny * g->nz >= (int64_t)INT32_MAX - 8) return FSIM_ERR_INVALID;
     return FSIM_OK;
 }
 
-static int check_params(const fsim_params* p) {
+// two_d: the learned multigrid preconditioner (no reference counterpart) is
+// built for 3D grids only; the 2D mode takes MIC(0)-PCG or red-black SOR.
+static int check_params(const fsim_params* p, bool two_d) {
     if (!p) return FSIM_ERR_INVALID;
     if (p->transfer < FSIM_PIC || p->transfer > FSIM_APIC) return FSIM_ERR_INVALID;
     if (p->max_iterations < 0 || !(p->fluid_density > 0.0)) return FSIM_ERR_INVALID;
     if (p->stop_rule != FSIM_STOP_ABS_INF && p->stop_rule != FSIM_STOP_REL_L2) return FSIM_ERR_INVALID;
     if (p->precond != FSIM_PRECOND_MIC0 && p->precond != FSIM_PRECOND_LEARNED && p->precond != FSIM_SOLVER_BASIC_SOR)
         return FSIM_ERR_INVALID;
+    if (two_d && p->precond == FSIM_PRECOND_LEARNED) return FSIM_ERR_INVALID;
     return FSIM_OK;
 }
 
@@ -147,7 +153,7 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
     if (!out) return FSIM_ERR_INVALID;
     *out = nullptr;
     FSIM_TRY(check_grid(grid));
-    FSIM_TRY(check_params(params));
+    FSIM_TRY(check_params(params, grid->two_d != 0));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FSIM_ERR_NO_DEVICE;
     fsim_ctx* ctx = new (std::nothrow) fsim_ctx();
@@ -167,9 +173,12 @@ int fsim_create(const fsim_grid_desc* grid, const fsim_params* params, fsim_ctx*
     g.nc = (int64_t)g.nx * g.ny * g.nz;
     g.cd = grid->cell_d;
     g.cdinv = 1.0 / grid->cell_d;
+    g.two_d = grid->two_d ? 1 : 0;
+    g.cdz = g.two_d ? grid->cell_d_z : grid->cell_d;
+    g.cdzinv = 1.0 / g.cdz;
     g.dims[0] = g.nx * g.cd;
     g.dims[1] = g.ny * g.cd;
-    g.dims[2] = g.nz * g.cd;
+    g.dims[2] = g.nz * g.cdz;
     g.pr = grid->particle_r;
     ctx->prm = *params;
     const int64_t nc = g.nc;
@@ -228,7 +237,7 @@ int fsim_destroy(fsim_ctx* ctx) {
 
 int fsim_set_params(fsim_ctx* ctx, const fsim_params* params) {
     if (!ctx) return FSIM_ERR_INVALID;
-    int rc = check_params(params);
+    int rc = check_params(params, ctx->g.two_d != 0);
     if (rc) return set_err(ctx, rc, "invalid parameters");
     ctx->prm = *params;
     return FSIM_OK;

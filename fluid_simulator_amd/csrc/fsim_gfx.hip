@@ -57,7 +57,7 @@ struct GfxArgs {
     fsim_gfx_particle* out;
     int64_t n;
     int nx, ny, nz;
-    double cd, cdinv;
+    double cd[3], cdinv[3];   // per axis (z differs in 2D)
 };
 
 // getCellsAround's order of the 8 cells (macGrid.cpp:192-201)
@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(kBlock) k_gfx_snapshot(GfxArgs A) {
     const double v[3] = {(double)A.vel[0][i], (double)A.vel[1][i], (double)A.vel[2][i]};
     int base[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) base[d] = (int)(p[d] * A.cdinv - 0.5);
+    for (int d = 0; d < 3; d++) base[d] = (int)(p[d] * A.cdinv[d] - 0.5);
     float dens = 0.f;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(kBlock) k_gfx_snapshot(GfxArgs A) {
             continue;
         double w = 1.0;
 #pragma unroll
-        for (int d = 0; d < 3; d++) w *= 1.0 - fabs(((c[d] + 0.5) * A.cd - p[d]) * A.cdinv);
+        for (int d = 0; d < 3; d++) w *= 1.0 - fabs(((c[d] + 0.5) * A.cd[d] - p[d]) * A.cdinv[d]);
         const float rho = A.density[((int64_t)c[0] * A.ny + c[1]) * A.nz + c[2]];
         dens = (float)((double)dens + w * (double)rho);
     }
@@ -108,8 +108,10 @@ static GfxArgs gfx_args(fsim_ctx* ctx, fsim_gfx_particle* dev_out) {
     A.nx = ctx->g.nx;
     A.ny = ctx->g.ny;
     A.nz = ctx->g.nz;
-    A.cd = ctx->g.cd;
-    A.cdinv = ctx->g.cdinv;
+    for (int d = 0; d < 3; d++) {
+        A.cd[d] = d == 2 ? ctx->g.cdz : ctx->g.cd;
+        A.cdinv[d] = d == 2 ? ctx->g.cdzinv : ctx->g.cdinv;
+    }
     return A;
 }
 

@@ -37,9 +37,15 @@ int fill_i32(fsim_ctx* ctx, int32_t* p, int32_t v, int64_t n) { return fill(ctx,
 struct GridArgs {
     int nx, ny, nz;
     int64_t nc;
-    double cd, cdinv;
-    float cdf, invf;
+    double cd, cdinv;     // cellD.x == cellD.y
+    double cdz, cdzinv;   // cellD.z (cd except in 2D)
+    float cdf, invf, invfz;
+    int two_d;
 };
+// cellD, cellDInv of axis d (macGrid.cpp:10)
+__host__ __device__ inline double cd_of(const GridArgs& G, int d) { return d == 2 ? G.cdz : G.cd; }
+__host__ __device__ inline double cdinv_of(const GridArgs& G, int d) { return d == 2 ? G.cdzinv : G.cdinv; }
+__host__ __device__ inline float invf_of(const GridArgs& G, int d) { return d == 2 ? G.invfz : G.invf; }
 static GridArgs grid_args(const fsim_ctx* ctx) {
     GridArgs G;
     G.nx = ctx->g.nx;
@@ -48,8 +54,12 @@ static GridArgs grid_args(const fsim_ctx* ctx) {
     G.nc = ctx->g.nc;
     G.cd = ctx->g.cd;
     G.cdinv = ctx->g.cdinv;
+    G.cdz = ctx->g.cdz;
+    G.cdzinv = ctx->g.cdzinv;
     G.cdf = (float)ctx->g.cd;
     G.invf = (float)ctx->g.cdinv;
+    G.invfz = (float)ctx->g.cdzinv;
+    G.two_d = ctx->g.two_d;
     return G;
 }
 
@@ -59,7 +69,7 @@ __device__ inline int64_t cidx(const GridArgs& G, int x, int y, int z) { return 
 // (macGrid.cpp:142-148): ivec3(pos*cellDInv - offset_a), then coord[a] -= 1.
 // Evaluated in FP64 so stencil membership is decided exactly as in FP64.
 __device__ inline void face_base(const GridArgs& G, float px, float py, float pz, int a, int b[3]) {
-    const double p[3] = {(double)px * G.cdinv, (double)py * G.cdinv, (double)pz * G.cdinv};
+    const double p[3] = {(double)px * G.cdinv, (double)py * G.cdinv, (double)pz * G.cdzinv};
 #pragma unroll
     for (int d = 0; d < 3; d++) b[d] = (int)(p[d] - (d == a ? 0.0 : 0.5));
     b[a] -= 1;
@@ -69,7 +79,7 @@ __device__ inline void face_base(const GridArgs& G, float px, float py, float pz
 __device__ inline void face_pos(const GridArgs& G, int a, int x, int y, int z, float f[3]) {
     f[0] = (float)((x + (a == 0 ? 1.0 : 0.5)) * G.cd);
     f[1] = (float)((y + (a == 1 ? 1.0 : 0.5)) * G.cd);
-    f[2] = (float)((z + (a == 2 ? 1.0 : 0.5)) * G.cd);
+    f[2] = (float)((z + (a == 2 ? 1.0 : 0.5)) * G.cdz);
 }
 
 // trilinearInterpoll, interpolation.h:7-20
@@ -167,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         const float pv = A.pos[d][j];
-                        const double v = (double)pv * G.cdinv;
+                        const double v = (double)pv * cdinv_of(G, d);
                         const int t = (int)v;
                         s_pos[d][k] = pv;
                         s_vel[d][k] = A.vel[d][j];
@@ -195,8 +205,8 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
                     for (int d = 0; d < 3; d++) {
                         ua[d] = cc[d] - s_t[d][k] + 1;
                         uc[d] = ua[d] - (int)((h >> d) & 1u);
-                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * G.invf);
-                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * G.invf);
+                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * invf_of(G, d));
+                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * invf_of(G, d));
                     }
                     // getCellsAround (macGrid.cpp:188-202): base (int)(p/dx - 0.5) on every axis
                     if ((unsigned)uc[0] <= 1u && (unsigned)uc[1] <= 1u && (unsigned)uc[2] <= 1u)
@@ -308,7 +318,7 @@ __global__ void __launch_bounds__(P2G_TZ * TX * TY) k_p2g_tile(P2GArgs A) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         const float pv = A.pos[d][j];
-                        const double v = (double)pv * G.cdinv;
+                        const double v = (double)pv * cdinv_of(G, d);
                         const int t = (int)v;
                         s_pos[d][k] = pv;
                         s_vel[d][k] = A.vel[d][j];
@@ -334,8 +344,8 @@ __global__ void __launch_bounds__(P2G_TZ * TX * TY) k_p2g_tile(P2GArgs A) {
                     for (int d = 0; d < 3; d++) {
                         ua[d] = cc[d] - s_t[d][k] + 1;
                         uc[d] = ua[d] - (int)((h >> d) & 1u);
-                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * G.invf);
-                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * G.invf);
+                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * invf_of(G, d));
+                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * invf_of(G, d));
                     }
                     if ((unsigned)uc[0] <= 1u && (unsigned)uc[1] <= 1u && (unsigned)uc[2] <= 1u)
                         dens += tce[0] * tce[1] * tce[2];
@@ -502,7 +512,7 @@ int stage_obstacles(fsim_ctx* ctx) {
         A.label = ctx->label;
         A.uvw[0] = ctx->u; A.uvw[1] = ctx->v; A.uvw[2] = ctx->w;
         for (int a = 0; a < 3; a++) {
-            A.ctr[a] = o.pos[a] * ctx->g.cdinv;
+            A.ctr[a] = o.pos[a] * (a == 2 ? ctx->g.cdzinv : ctx->g.cdinv);
             A.speed[a] = (float)o.speed[a];
         }
         int mx[3];
@@ -517,7 +527,7 @@ int stage_obstacles(fsim_ctx* ctx) {
         } else if (o.kind == FSIM_OBS_BOX) {
             A.sphere = 0;
             for (int a = 0; a < 3; a++) {
-                double d = o.size[a] * ctx->g.cdinv * 0.5;
+                double d = o.size[a] * (a == 2 ? ctx->g.cdzinv : ctx->g.cdinv) * 0.5;
                 A.mn[a] = (int)std::fmax(std::round(A.ctr[a] - d), 1.0);
                 mx[a] = (int)std::fmin(std::round(A.ctr[a] + d), n[a] - 1.0) - 1;   // exclusive -> inclusive
             }
@@ -722,9 +732,13 @@ __global__ void __launch_bounds__(kBlock) k_g2p(G2PArgs A) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
     const float px = A.pos[0][i], py = A.pos[1][i], pz = A.pos[2][i];
-    const float inv = G.invf;
+    const float inv = G.invf, invz = G.invfz;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
+        if (a == 2 && G.two_d) {   // 2D: v.z = 0 and no z row (simulator.cpp:382-385)
+            A.vel[2][i] = 0.0f;
+            break;
+        }
         int b[3];
         face_base(G, px, py, pz, a, b);
         // stay inside the grid for positions outside the clamp box (the step
@@ -738,10 +752,10 @@ __global__ void __launch_bounds__(kBlock) k_g2p(G2PArgs A) {
         for (int o = 0; o < 2; o++) {
             float fx = (float)((b[0] + o + (a == 0 ? 1.0 : 0.5)) * G.cd);
             float fy = (float)((b[1] + o + (a == 1 ? 1.0 : 0.5)) * G.cd);
-            float fz = (float)((b[2] + o + (a == 2 ? 1.0 : 0.5)) * G.cd);
+            float fz = (float)((b[2] + o + (a == 2 ? 1.0 : 0.5)) * G.cdz);
             dxs[o] = (px - fx) * inv;
             dys[o] = (py - fy) * inv;
-            dzs[o] = (pz - fz) * inv;
+            dzs[o] = (pz - fz) * invz;
             tx[o] = 1.0f - fabsf(dxs[o]);
             ty[o] = 1.0f - fabsf(dys[o]);
             tz[o] = 1.0f - fabsf(dzs[o]);
@@ -760,7 +774,7 @@ __global__ void __launch_bounds__(kBlock) k_g2p(G2PArgs A) {
                 float sx = dxs[ox] > 0.f ? -1.f : 1.f, sy = dys[oy] > 0.f ? -1.f : 1.f, sz = dzs[oz] > 0.f ? -1.f : 1.f;
                 g0 += sx * ty[oy] * tz[oz] * inv * v2;
                 g1 += sy * tx[ox] * tz[oz] * inv * v2;
-                g2 += sz * tx[ox] * ty[oy] * inv * v2;
+                g2 += sz * tx[ox] * ty[oy] * invz * v2;
             }
         }
         if (A.transfer == FSIM_PIC) {

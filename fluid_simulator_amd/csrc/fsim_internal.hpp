@@ -157,9 +157,11 @@ struct WaveArgs {
 struct GridDims {
     int nx, ny, nz;
     int64_t nc;
-    double cd, cdinv;
+    double cd, cdinv;     // cellD.x == cellD.y
+    double cdz, cdzinv;   // cellD.z (cd except in 2D)
     double dims[3];
     double pr;
+    int two_d;
 };
 
 // Device-resident scalars of the pressure solve (one cache line each group).

@@ -417,6 +417,7 @@ int mg_levels_for(int nx, int ny, int nz) {
 
 static int mg_alloc(fsim_ctx* ctx) {
     if (ctx->mg) return FSIM_OK;
+    if (ctx->g.two_d) return set_err(ctx, FSIM_ERR_INVALID, "the multigrid preconditioner is built for 3D grids only");
     MgState* M = new MgState();
     ctx->mg = M;
     M->nlev = mg_levels_for(ctx->g.nx, ctx->g.ny, ctx->g.nz);

@@ -45,11 +45,13 @@ struct MoveArgs {
     const fsim_obstacle* obs;
     int nobs;
     int64_t n;
-    double dt, pr, cd, cdinv;
+    double dt, pr;
+    double cd[3], cdinv[3];   // cellD, cellDInv per axis (z differs in 2D)
     double dims[3];
     int ncell[3];
     int flags;
     int despawn;
+    int two_d;
 };
 
 __host__ __device__ inline bool is_sphere(int k) { return k == FSIM_OBS_SPHERE || k == FSIM_OBS_SOURCE || k == FSIM_OBS_SINK; }
@@ -177,12 +179,12 @@ __host__ __device__ bool advect_one(const MoveArgs& A, double pos[3], double v[3
 __host__ __device__ inline void rect_bounds(const MoveArgs& A, const fsim_obstacle& o, double lo[3], double hi[3]) {
 #pragma clang fp contract(off)
     for (int a = 0; a < 3; a++) {
-        double c = o.pos[a] * A.cdinv;
-        double d = o.size[a] * A.cdinv * 0.5;
+        double c = o.pos[a] * A.cdinv[a];
+        double d = o.size[a] * A.cdinv[a] * 0.5;
         int mn = (int)fmax(round(c - d), 1.0);
         int mx = (int)fmin(round(c + d), A.ncell[a] - 1.0);
-        lo[a] = (double)mn * A.cd - A.pr;
-        hi[a] = (double)mx * A.cd + A.pr;
+        lo[a] = (double)mn * A.cd[a] - A.pr;
+        hi[a] = (double)mx * A.cd[a] + A.pr;
     }
 }
 
@@ -240,8 +242,9 @@ __global__ void __launch_bounds__(kBlock) k_move(MoveArgs A) {
     bool removed = false;
     if (A.flags & 1) {
         double lo[3], hi[3];
+        // gridLow = cellD + (r, r, 2D ? 0 : r) * 1.01 (simulator.cpp:140)
         for (int a = 0; a < 3; a++) {
-            lo[a] = A.cd + A.pr * 1.01;
+            lo[a] = A.cd[a] + (a == 2 && A.two_d ? 0.0 : A.pr) * 1.01;
             hi[a] = A.dims[a] - lo[a];
         }
         removed = advect_one(A, pos, v, lo, hi);
@@ -249,8 +252,8 @@ __global__ void __launch_bounds__(kBlock) k_move(MoveArgs A) {
     }
     if ((A.flags & 2) && !removed) {
         double plo[3], phi[3];
-        for (int a = 0; a < 3; a++) {
-            plo[a] = A.cd + A.pr * 1.01;
+        for (int a = 0; a < 3; a++) {   // particleLow, simulator.cpp:257
+            plo[a] = A.cd[a] + A.pr * 1.01;
             phi[a] = A.dims[a] - plo[a];
         }
         push_out_one(A, pos, plo, phi);
@@ -285,8 +288,11 @@ int stage_move(fsim_ctx* ctx, double dt, int flags) {
     A.n = ctx->P.n;
     A.dt = dt;
     A.pr = ctx->g.pr;
-    A.cd = ctx->g.cd;
-    A.cdinv = ctx->g.cdinv;
+    for (int a = 0; a < 3; a++) {
+        A.cd[a] = a == 2 ? ctx->g.cdz : ctx->g.cd;
+        A.cdinv[a] = a == 2 ? ctx->g.cdzinv : ctx->g.cdinv;
+    }
+    A.two_d = ctx->g.two_d;
     A.ncell[0] = ctx->g.nx;
     A.ncell[1] = ctx->g.ny;
     A.ncell[2] = ctx->g.nz;
@@ -311,9 +317,12 @@ struct ApartArgs {
     float* out[3];
     const int32_t* start;
     int64_t n;
-    int nx, ny, nz, R;
-    double cdinv, D, D2;
+    int nx, ny, nz;
+    int R[3];            // search reach in cells per axis
+    double cdinv[3], D, D2;
     double lo[3], hi[3];
+    int two_d;           // 2D: z is set to the depth / 2 (hashedParticles.cpp:96-101)
+    double zc;
 };
 
 __global__ void __launch_bounds__(kBlock) k_push_apart(ApartArgs A) {
@@ -324,12 +333,12 @@ __global__ void __launch_bounds__(kBlock) k_push_apart(ApartArgs A) {
     const int n3[3] = {A.nx, A.ny, A.nz};
     int c[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) c[a] = min(max((int)(pi[a] * A.cdinv), 0), n3[a] - 1);   // the bin key's cell
+    for (int a = 0; a < 3; a++) c[a] = min(max((int)(pi[a] * A.cdinv[a]), 0), n3[a] - 1);   // the bin key's cell
     double q[3] = {pi[0], pi[1], pi[2]};
-    for (int x = max(c[0] - A.R, 0); x <= min(c[0] + A.R, A.nx - 1); x++)
-        for (int y = max(c[1] - A.R, 0); y <= min(c[1] + A.R, A.ny - 1); y++) {
+    for (int x = max(c[0] - A.R[0], 0); x <= min(c[0] + A.R[0], A.nx - 1); x++)
+        for (int y = max(c[1] - A.R[1], 0); y <= min(c[1] + A.R[1], A.ny - 1); y++) {
             const int64_t row = ((int64_t)x * A.ny + y) * A.nz;
-            const int z0 = max(c[2] - A.R, 0), z1 = min(c[2] + A.R, A.nz - 1);
+            const int z0 = max(c[2] - A.R[2], 0), z1 = min(c[2] + A.R[2], A.nz - 1);
             // the cells z0..z1 of one row are one contiguous slot range
             const int b = A.start[row + z0], e = A.start[row + z1 + 1];
             for (int j = b; j < e; j++) {
@@ -346,6 +355,7 @@ __global__ void __launch_bounds__(kBlock) k_push_apart(ApartArgs A) {
         }
 #pragma unroll
     for (int a = 0; a < 3; a++) A.out[a][i] = (float)(q[a] < A.lo[a] ? A.lo[a] : (q[a] > A.hi[a] ? A.hi[a] : q[a]));
+    if (A.two_d) A.out[2][i] = (float)A.zc;
 }
 
 int stage_push_apart(fsim_ctx* ctx) {
@@ -357,18 +367,22 @@ int stage_push_apart(fsim_ctx* ctx) {
     for (int a = 0; a < 3; a++) {
         A.pos[a] = ctx->P.pos[a];
         A.out[a] = ctx->Palt.pos[a];
-        A.lo[a] = ctx->g.cd + ctx->g.pr * 1.01;
+        A.lo[a] = (a == 2 ? ctx->g.cdz : ctx->g.cd) + ctx->g.pr * 1.01;
         A.hi[a] = ctx->g.dims[a] - A.lo[a];
     }
+    A.two_d = ctx->g.two_d;
+    A.zc = ctx->g.dims[2] / 2;
     A.start = ctx->cell_start;
     A.n = n;
     A.nx = ctx->g.nx;
     A.ny = ctx->g.ny;
     A.nz = ctx->g.nz;
-    A.cdinv = ctx->g.cdinv;
     A.D = ctx->g.pr * 2.0;
     A.D2 = A.D * A.D;
-    A.R = (int)std::ceil(A.D * ctx->g.cdinv);
+    for (int a = 0; a < 3; a++) {
+        A.cdinv[a] = a == 2 ? ctx->g.cdzinv : ctx->g.cdinv;
+        A.R[a] = (int)std::ceil(A.D * A.cdinv[a]);
+    }
     hipLaunchKernelGGL(k_push_apart, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, A);
     FSIM_HIP(ctx, hipGetLastError());
     for (int a = 0; a < 3; a++)
@@ -538,8 +552,8 @@ int particles_reserve(fsim_ctx* ctx, int64_t n) {
 __global__ void __launch_bounds__(kBlock) k_bin_keys(const float* __restrict__ px, const float* __restrict__ py,
                                                      const float* __restrict__ pz, const uint8_t* __restrict__ removed,
                                                      uint32_t* __restrict__ key, int32_t* __restrict__ rank,
-                                                     int32_t* __restrict__ count, int64_t n, double cdinv, int nx,
-                                                     int ny, int nz, uint32_t nc) {
+                                                     int32_t* __restrict__ count, int64_t n, double cdinv,
+                                                     double cdzinv, int nx, int ny, int nz, uint32_t nc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < n;
     uint32_t k = 0xffffffffu;   // lanes past n: a key of their own, no count
@@ -552,7 +566,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_keys(const float* __restrict__ p
             // advection clamps them) are clamped into it so no key leaves [0, Nc)
             int cx = min(max((int)((double)px[i] * cdinv), 0), nx - 1);
             int cy = min(max((int)((double)py[i] * cdinv), 0), ny - 1);
-            int cz = min(max((int)((double)pz[i] * cdinv), 0), nz - 1);
+            int cz = min(max((int)((double)pz[i] * cdzinv), 0), nz - 1);
             k = (uint32_t)(((int64_t)cx * ny + cy) * nz + cz);
         }
         key[i] = k;
@@ -765,7 +779,7 @@ int stage_bin(fsim_ctx* ctx) {
         FSIM_PROF(ctx, KID_BIN_KEYS);
         hipLaunchKernelGGL(k_bin_keys, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->P.pos[0], ctx->P.pos[1],
                            ctx->P.pos[2], ctx->removed, ctx->key, ctx->rank, ctx->cell_count, n, ctx->g.cdinv,
-                           ctx->g.nx, ctx->g.ny, ctx->g.nz, (uint32_t)nc);
+                           ctx->g.cdzinv, ctx->g.nx, ctx->g.ny, ctx->g.nz, (uint32_t)nc);
         FSIM_HIP(ctx, hipGetLastError());
     }
     {
@@ -839,10 +853,11 @@ using namespace fsim;
 static bool scene_cell_solid(const fsim_grid_desc* g, int x, int y, int z, int nobs, const fsim_obstacle* obs) {
     const int n[3] = {g->nx, g->ny, g->nz};
     const double inv = 1.0 / g->cell_d;
+    const double invd[3] = {inv, inv, 1.0 / (g->two_d ? g->cell_d_z : g->cell_d)};
     const int p[3] = {x, y, z};
     for (int i = 0; i < nobs; i++) {
         const fsim_obstacle& o = obs[i];
-        double ctr[3] = {o.pos[0] * inv, o.pos[1] * inv, o.pos[2] * inv};
+        double ctr[3] = {o.pos[0] * invd[0], o.pos[1] * invd[1], o.pos[2] * invd[2]};
         if (o.kind == FSIM_OBS_SPHERE || o.kind == FSIM_OBS_SOURCE) {
             double r = inv * o.r, r2 = r * r;
             bool in = true;
@@ -855,7 +870,7 @@ static bool scene_cell_solid(const fsim_grid_desc* g, int x, int y, int z, int n
         } else if (o.kind == FSIM_OBS_BOX) {
             bool in = true;
             for (int a = 0; a < 3; a++) {
-                double d = o.size[a] * inv * 0.5;
+                double d = o.size[a] * invd[a] * 0.5;
                 int mn = (int)std::fmax(std::round(ctr[a] - d), 1.0), mx = (int)std::fmin(std::round(ctr[a] + d), n[a] - 1.0);
                 if (p[a] < mn || p[a] >= mx) in = false;
             }
@@ -868,14 +883,18 @@ static bool scene_cell_solid(const fsim_grid_desc* g, int x, int y, int z, int n
 extern "C" int fsim_scene_dam_break(const fsim_grid_desc* g, int n_blocks, const double* blk, int n_obstacles,
                                     const fsim_obstacle* obs, uint64_t seed, int64_t capacity, float* pos,
                                     int64_t* n_out) {
-    if (!g || !n_out || n_blocks < 0 || (n_blocks > 0 && !blk) || g->nx < 4 || g->ny < 4 || g->nz < 4)
+    const bool two_d = g && g->two_d;
+    if (!g || !n_out || n_blocks < 0 || (n_blocks > 0 && !blk) || g->nx < 4 || g->ny < 4 ||
+        (two_d ? g->nz != 3 || !(g->cell_d_z > 0.0) : g->nz < 4))
         return FSIM_ERR_INVALID;
     const double cd = g->cell_d;
+    const double cdd[3] = {cd, cd, two_d ? g->cell_d_z : cd};
+    const int per = two_d ? 4 : 8;   // 2D: 2 x 2 sub-cells in (x, y), all at the mid-depth z
     int64_t np = 0;
     for (int x = 1; x < g->nx - 1; x++)
         for (int y = 1; y < g->ny - 1; y++)
             for (int z = 1; z < g->nz - 1; z++) {
-                const double c[3] = {(x + 0.5) * cd, (y + 0.5) * cd, (z + 0.5) * cd};
+                const double c[3] = {(x + 0.5) * cdd[0], (y + 0.5) * cdd[1], (z + 0.5) * cdd[2]};
                 bool in = false;
                 for (int b = 0; b < n_blocks && !in; b++) {
                     const double* lo = blk + 6 * b;
@@ -884,18 +903,19 @@ extern "C" int fsim_scene_dam_break(const fsim_grid_desc* g, int n_blocks, const
                 }
                 if (!in || scene_cell_solid(g, x, y, z, n_obstacles, obs)) continue;
                 if (pos) {
-                    if (np + 8 > capacity) return FSIM_ERR_INVALID;
+                    if (np + per > capacity) return FSIM_ERR_INVALID;
                     const int ci[3] = {x, y, z};
-                    for (int s = 0; s < 8; s++) {
-                        const int sub[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1};
+                    for (int s = 0; s < per; s++) {
+                        const int sub[3] = {two_d ? (s >> 1) & 1 : (s >> 2) & 1, two_d ? s & 1 : (s >> 1) & 1, s & 1};
                         int64_t id = np + s;
                         for (int a = 0; a < 3; a++) {
                             float u = u01f(splitmix64(seed, 3ull * (uint64_t)id + (uint64_t)a));
-                            pos[3 * id + a] = (float)((ci[a] + ((double)sub[a] + (double)u) * 0.5) * cd);
+                            pos[3 * id + a] = two_d && a == 2 ? (float)(1.5 * cdd[2])
+                                                              : (float)((ci[a] + ((double)sub[a] + (double)u) * 0.5) * cdd[a]);
                         }
                     }
                 }
-                np += 8;
+                np += per;
             }
     *n_out = np;
     return FSIM_OK;

@@ -19,6 +19,16 @@ def grid(n: int) -> capi.Grid:
     return capi.Grid(n, n, n, 1.0 / n, 0.25 / n)
 
 
+DEPTH_2D = 0.1
+
+
+def grid_2d(n: int, depth: float = DEPTH_2D) -> capi.Grid:
+    """The 2D mode: MacGrid((1, 1, depth), resolution n, twoD) is n x n x 3
+    cells with cellD.z = depth / 3 (macGrid.cpp:9-13); particles stay at
+    z = depth / 2."""
+    return capi.Grid(n, n, 3, 1.0 / n, 0.25 / n, two_d=1, cell_d_z=depth / 3)
+
+
 def params(transfer=capi.FLIP, stop_rule=capi.STOP_REL_L2, tol=1e-6, max_iterations=10000, **kw) -> capi.Params:
     base = dict(transfer=transfer, flip_ratio=0.95, gravity=-9.81, gravity_enabled=1, push_apart_enabled=0,
                 pressure_enabled=0, top_solid=0, fluid_density=1.0, residual_tolerance=tol,
@@ -36,10 +46,11 @@ class Scene:
     obstacles: list = field(default_factory=list)
     seed: int = 0
     steps: int = 100
+    two_d: bool = False
 
     @property
     def grid(self) -> capi.Grid:
-        return grid(self.n)
+        return grid_2d(self.n) if self.two_d else grid(self.n)
 
     def particles(self):
         return capi.scene_dam_break(self.grid, self.blocks, self.obstacles, self.seed)
@@ -49,6 +60,14 @@ def dam_break(n: int, transfer=capi.FLIP, seed=0, sphere=False, steps=100) -> Sc
     """Block x < 0.4, y < 0.6 (full z), 8 particles per fluid cell."""
     obs = [capi.make_obstacle(capi.OBS_SPHERE, (0.5, 0.5, 0.5), r=0.1)] if sphere else []
     return Scene(f"dam_break_{n}", n, transfer, [[0.0, 0.0, 0.0, 0.4, 0.6, 1.0]], obs, seed, steps)
+
+
+def dam_break_2d(n: int, transfer=capi.FLIP, seed=0, sphere=False, steps=100) -> Scene:
+    """The 2D mode's dam break: block x < 0.4, y < 0.6 of the n x n x 3 grid,
+    4 particles per fluid cell at z = depth / 2; the 2D preset turns push-apart
+    on (simulationGui.cpp:40)."""
+    obs = [capi.make_obstacle(capi.OBS_SPHERE, (0.5, 0.5, DEPTH_2D / 2), r=0.1)] if sphere else []
+    return Scene(f"dam_break_2d_{n}", n, transfer, [[0.0, 0.0, 0.0, 0.4, 0.6, DEPTH_2D]], obs, seed, steps, True)
 
 
 def double_dam_break(n: int, transfer=capi.FLIP, seed=1, steps=100) -> Scene:

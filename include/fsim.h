@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FSIM_ABI_VERSION 2
+#define FSIM_ABI_VERSION 3
 
 typedef struct fsim_ctx fsim_ctx;
 
@@ -68,7 +68,8 @@ typedef enum fsim_stop_rule { FSIM_STOP_ABS_INF = 0, FSIM_STOP_REL_L2 = 1 } fsim
  * (macGrid.h:165) chosen by SimulationConfig::GridSolverType
  * (simulationManager.h:35-38).  MIC0 = BridsonSolverGrid, MIC(0)-PCG
  * (bridsonSolverGrid.cpp:91-293).  LEARNED = flexible PCG with the learned
- * preconditioner (no reference counterpart), MIC(0)-PCG as fallback.
+ * preconditioner (no reference counterpart), MIC(0)-PCG as fallback; 3D
+ * grids only (FSIM_ERR_INVALID with two_d).
  * BASIC_SOR = BasicMacGrid (GridSolverType::BASIC): max_iterations red-black
  * SOR sweeps on the faces (basicMacGrid.cpp:5-102), no pressure field. */
 typedef enum fsim_precond {
@@ -78,10 +79,14 @@ typedef enum fsim_precond {
 } fsim_precond;
 
 typedef struct fsim_grid_desc {
-    int nx, ny, nz;      /* MacGrid::gridSize (macGrid.cpp:11), every axis >= 4 */
-    double cell_d;       /* MacGrid::cellD.x == .y == .z (macGrid.cpp:10) */
+    int nx, ny, nz;      /* MacGrid::gridSize (macGrid.cpp:11): nx, ny >= 4; nz >= 4, or 3 in 2D */
+    double cell_d;       /* MacGrid::cellD.x == .y (macGrid.cpp:10) */
     double particle_r;   /* HashedParticles::getParticleR (hashedParticles.h:84) */
-    int two_d;           /* must be 0: the 2D mode is not implemented */
+    int two_d;           /* MacGrid::twoD (macGrid.h:182): 1 = the 2D mode, nz = 3 (macGrid.cpp:10-12):
+                            particles keep their z, the clamp box has no r margin along z
+                            (simulator.cpp:140) and G2P sets v.z = 0 (simulator.cpp:382-385) */
+    double cell_d_z;     /* MacGrid::cellD.z (macGrid.cpp:10): in 2D the target depth / 3; in 3D 0 (= cell_d)
+                            or cell_d */
 } fsim_grid_desc;
 
 typedef struct fsim_params {
@@ -409,7 +414,9 @@ int fsim_learned_harvest(fsim_ctx* ctx, double dt, int capacity, float* b, float
  * particles, one per 2x2x2 sub-cell at (i + (s + u)/2) * cellD with
  * u ~ U[0,1)^3 from the SplitMix64 counter RNG keyed by (seed, particle id).
  * Particles are numbered by cell in flat order, then sub-cell (z fastest).
- * Call with pos == NULL to get the count only. */
+ * 2D grids (two_d): 4 particles per cell, one per 2x2 sub-cell in (x, y), at
+ * z = 1.5 cellD.z (the depth / 2, where HashedParticles keeps 2D particles,
+ * hashedParticles.cpp:23-35).  Call with pos == NULL to get the count only. */
 int fsim_scene_dam_break(const fsim_grid_desc* grid, int n_blocks, const double* blocks_lo_hi,
                          int n_obstacles, const fsim_obstacle* obstacles, uint64_t seed,
                          int64_t capacity, float* pos, int64_t* n_out);

@@ -47,7 +47,10 @@ typedef struct { double pos[3], v[3], c[3][3]; } fo_particle;
 struct fo_sim {
     int n[3];
     int64_t nc;
-    double cd, cdinv;
+    double cd, cdinv;          /* cellD.x (== .y): the solver's scales (bridsonSolverGrid.cpp:41,81,296) */
+    double cdv[3], cdinvv[3];  /* cellD, cellDInv per axis: z is the depth / 3 in 2D (macGrid.cpp:10) */
+    int two_d;                 /* MacGrid::twoD */
+    double zc;                 /* 2D: the z particles keep (HashedParticles z = depth / 2) */
     double dims[3];
     double pr;
     fsim_params prm;
@@ -86,19 +89,19 @@ static inline double u01d(uint64_t x) { return (double)(x >> 11) * (1.0 / 900719
 #define FO_SPAWN_SALT 0x5350415741ULL
 
 /* ---- util/interpolation.h:7-35 ------------------------------------------ */
-static inline double tri_w(const double c[3], const double p[3], double inv) {
-    double x = 1.0 - fabs((p[0] - c[0]) * inv);
-    double y = 1.0 - fabs((p[1] - c[1]) * inv);
-    double z = 1.0 - fabs((p[2] - c[2]) * inv);
+static inline double tri_w(const double c[3], const double p[3], const double inv[3]) {
+    double x = 1.0 - fabs((p[0] - c[0]) * inv[0]);
+    double y = 1.0 - fabs((p[1] - c[1]) * inv[1]);
+    double z = 1.0 - fabs((p[2] - c[2]) * inv[2]);
     return x * y * z;
 }
-static inline void tri_grad(const double c[3], const double p[3], double inv, double g[3]) {
-    double vx = (p[0] - c[0]) * inv, vy = (p[1] - c[1]) * inv, vz = (p[2] - c[2]) * inv;
+static inline void tri_grad(const double c[3], const double p[3], const double inv[3], double g[3]) {
+    double vx = (p[0] - c[0]) * inv[0], vy = (p[1] - c[1]) * inv[1], vz = (p[2] - c[2]) * inv[2];
     double xa = 1.0 - fabs(vx), ya = 1.0 - fabs(vy), za = 1.0 - fabs(vz);
     double xs = vx > 0.0 ? -1.0 : 1.0, ys = vy > 0.0 ? -1.0 : 1.0, zs = vz > 0.0 ? -1.0 : 1.0;
-    g[0] = xs * ya * za * inv;
-    g[1] = ys * xa * za * inv;
-    g[2] = zs * xa * ya * inv;
+    g[0] = xs * ya * za * inv[0];
+    g[1] = ys * xa * za * inv[1];
+    g[2] = zs * xa * ya * inv[2];
 }
 
 static inline double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -112,7 +115,7 @@ static const int CORNER[8][3] = {{1, 1, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0},
 static inline void faces_base(const fo_sim* s, const double pos[3], int base[3][3]) {
     for (int a = 0; a < 3; a++) {
         for (int d = 0; d < 3; d++) {
-            double g = pos[d] * s->cdinv - (d == a ? 0.0 : 0.5);
+            double g = pos[d] * s->cdinvv[d] - (d == a ? 0.0 : 0.5);
             base[a][d] = (int)g;
         }
         base[a][a] -= 1;
@@ -126,23 +129,31 @@ static void init_cells(fo_sim* s) {
             for (int z = 0; z < s->n[2]; z++) {
                 fo_cell* c = &s->cells[cidx(s, x, y, z)];
                 memset(c, 0, sizeof(*c));
-                double cd = s->cd;
-                c->f[0].pos[0] = (x + 1) * cd; c->f[0].pos[1] = (y + 0.5) * cd; c->f[0].pos[2] = (z + 0.5) * cd;
-                c->f[1].pos[0] = (x + 0.5) * cd; c->f[1].pos[1] = (y + 1) * cd; c->f[1].pos[2] = (z + 0.5) * cd;
-                c->f[2].pos[0] = (x + 0.5) * cd; c->f[2].pos[1] = (y + 0.5) * cd; c->f[2].pos[2] = (z + 1) * cd;
-                c->pos[0] = (x + 0.5) * cd; c->pos[1] = (y + 0.5) * cd; c->pos[2] = (z + 0.5) * cd;
+                const double cx = s->cdv[0], cy = s->cdv[1], cz = s->cdv[2];
+                c->f[0].pos[0] = (x + 1) * cx; c->f[0].pos[1] = (y + 0.5) * cy; c->f[0].pos[2] = (z + 0.5) * cz;
+                c->f[1].pos[0] = (x + 0.5) * cx; c->f[1].pos[1] = (y + 1) * cy; c->f[1].pos[2] = (z + 0.5) * cz;
+                c->f[2].pos[0] = (x + 0.5) * cx; c->f[2].pos[1] = (y + 0.5) * cy; c->f[2].pos[2] = (z + 1) * cz;
+                c->pos[0] = (x + 0.5) * cx; c->pos[1] = (y + 0.5) * cy; c->pos[2] = (z + 0.5) * cz;
                 c->type = FSIM_AIR;
             }
 }
 
 fo_sim* fo_create(const fsim_grid_desc* g, const fsim_params* prm) {
-    if (!g || !prm || g->nx < 4 || g->ny < 4 || g->nz < 4 || g->two_d) return NULL;
+    if (!g || !prm || g->nx < 4 || g->ny < 4) return NULL;
+    if (g->two_d ? (g->nz != 3 || !(g->cell_d_z > 0.0)) : (g->nz < 4 || (g->cell_d_z != 0.0 && g->cell_d_z != g->cell_d)))
+        return NULL;
     fo_sim* s = (fo_sim*)calloc(1, sizeof(fo_sim));
     s->n[0] = g->nx; s->n[1] = g->ny; s->n[2] = g->nz;
     s->nc = (int64_t)g->nx * g->ny * g->nz;
     s->cd = g->cell_d;
     s->cdinv = 1.0 / g->cell_d;
-    for (int a = 0; a < 3; a++) s->dims[a] = s->n[a] * s->cd;
+    s->two_d = g->two_d ? 1 : 0;
+    for (int a = 0; a < 3; a++) {
+        s->cdv[a] = a == 2 && s->two_d ? g->cell_d_z : g->cell_d;
+        s->cdinvv[a] = 1.0 / s->cdv[a];
+        s->dims[a] = s->n[a] * s->cdv[a];
+    }
+    s->zc = s->dims[2] / 2;
     s->pr = g->particle_r;
     s->prm = *prm;
     s->nthreads = 1;
@@ -383,8 +394,8 @@ static void advect_one(const fo_sim* s, fo_particle* P, double dt, const double 
 
 static void fo_advect(fo_sim* s, double dt) {
     double lo[3], hi[3];
-    for (int a = 0; a < 3; a++) {
-        lo[a] = s->cd + s->pr * 1.01;
+    for (int a = 0; a < 3; a++) {   /* gridLow = cellD + (r, r, twoD ? 0 : r) * 1.01, simulator.cpp:140 */
+        lo[a] = s->cdv[a] + (a == 2 && s->two_d ? 0.0 : s->pr) * 1.01;
         hi[a] = s->dims[a] - lo[a];
     }
     unsigned char* rm = (unsigned char*)calloc(s->np ? s->np : 1, 1);
@@ -398,12 +409,12 @@ static void fo_advect(fo_sim* s, double dt) {
 /* MacGrid::getMinMaxRect, macGrid.h:139-145 */
 static void min_max_rect(const fo_sim* s, const double pos[3], const double size[3], double lo[3], double hi[3]) {
     for (int a = 0; a < 3; a++) {
-        double c = pos[a] * s->cdinv;
-        double d = size[a] * s->cdinv * 0.5;
+        double c = pos[a] * s->cdinvv[a];
+        double d = size[a] * s->cdinvv[a] * 0.5;
         int mn = (int)fmax(round(c - d), 1.0);
         int mx = (int)fmin(round(c + d), s->n[a] - 1.0);
-        lo[a] = (double)mn * s->cd;
-        hi[a] = (double)mx * s->cd;
+        lo[a] = (double)mn * s->cdv[a];
+        hi[a] = (double)mx * s->cdv[a];
     }
 }
 
@@ -412,7 +423,7 @@ static void fo_push_out(fo_sim* s) {
     const double pr = s->pr;
     double plo[3], phi[3];
     for (int a = 0; a < 3; a++) {
-        plo[a] = s->cd + pr * 1.01;
+        plo[a] = s->cdv[a] + pr * 1.01;
         phi[a] = s->dims[a] - plo[a];
     }
     int64_t np = s->np;
@@ -479,27 +490,28 @@ static void fo_push_apart(fo_sim* s) {
     if (np == 0) return;
     const double D = s->pr * 2.0, D2 = D * D;
     double lo[3], hi[3];
-    for (int a = 0; a < 3; a++) {
-        lo[a] = s->cd + s->pr * 1.01;
+    for (int a = 0; a < 3; a++) {   /* particleLow, hashedParticles.cpp:67-68 */
+        lo[a] = s->cdv[a] + s->pr * 1.01;
         hi[a] = s->dims[a] - lo[a];
     }
     int32_t* count = (int32_t*)malloc(sizeof(int32_t) * (size_t)s->nc);
     int32_t* start = (int32_t*)malloc(sizeof(int32_t) * (size_t)(s->nc + 1));
     int64_t* perm = (int64_t*)malloc(sizeof(int64_t) * (size_t)np);
     fo_bin(s, count, start, perm);
-    const int R = (int)ceil(D * s->cdinv);
+    int R[3];
+    for (int a = 0; a < 3; a++) R[a] = (int)ceil(D * s->cdinvv[a]);
     double* out = (double*)malloc(sizeof(double) * 3 * (size_t)np);
 #pragma omp parallel for num_threads(fo_threads(s)) schedule(static)
     for (int64_t i = 0; i < np; i++) {
         const double* pi = s->p[i].pos;
         int c[3];
-        for (int a = 0; a < 3; a++) c[a] = (int)(pi[a] * s->cdinv);
+        for (int a = 0; a < 3; a++) c[a] = (int)(pi[a] * s->cdinvv[a]);
         double q[3] = {pi[0], pi[1], pi[2]};
-        for (int x = c[0] - R; x <= c[0] + R; x++) {
+        for (int x = c[0] - R[0]; x <= c[0] + R[0]; x++) {
             if (x < 0 || x >= s->n[0]) continue;
-            for (int y = c[1] - R; y <= c[1] + R; y++) {
+            for (int y = c[1] - R[1]; y <= c[1] + R[1]; y++) {
                 if (y < 0 || y >= s->n[1]) continue;
-                for (int z = c[2] - R; z <= c[2] + R; z++) {
+                for (int z = c[2] - R[2]; z <= c[2] + R[2]; z++) {
                     if (z < 0 || z >= s->n[2]) continue;
                     const int64_t cell = cidx(s, x, y, z);
                     for (int32_t k = start[cell]; k < start[cell + 1]; k++) {
@@ -517,6 +529,7 @@ static void fo_push_apart(fo_sim* s) {
             }
         }
         for (int a = 0; a < 3; a++) out[3 * i + a] = q[a] < lo[a] ? lo[a] : (q[a] > hi[a] ? hi[a] : q[a]);
+        if (s->two_d) out[3 * i + 2] = s->zc;   /* 2D: z = depth / 2 (hashedParticles.cpp:96-101) */
     }
     for (int64_t i = 0; i < np; i++)
         for (int a = 0; a < 3; a++) s->p[i].pos[a] = out[3 * i + a];
@@ -558,7 +571,7 @@ static void fo_p2g(fo_sim* s) {
             for (int k = 0; k < 8; k++) {
                 fo_face* f = &s->cells[cidx(s, base[a][0] + CORNER[k][0], base[a][1] + CORNER[k][1],
                                             base[a][2] + CORNER[k][2])].f[a];
-                double w = tri_w(f->pos, P->pos, s->cdinv);
+                double w = tri_w(f->pos, P->pos, s->cdinvv);
                 double add;
                 if (!apic) {
                     add = P->v[a] * w;
@@ -590,13 +603,13 @@ static void fo_mark(fo_sim* s) {
 #pragma omp parallel for num_threads(fo_threads(s)) schedule(static)
     for (int64_t i = 0; i < np; i++) {
         const fo_particle* P = &s->p[i];
-        int cx = (int)(P->pos[0] * s->cdinv), cy = (int)(P->pos[1] * s->cdinv), cz = (int)(P->pos[2] * s->cdinv);
+        int cx = (int)(P->pos[0] * s->cdinvv[0]), cy = (int)(P->pos[1] * s->cdinvv[1]), cz = (int)(P->pos[2] * s->cdinvv[2]);
         s->cells[cidx(s, cx, cy, cz)].type = FSIM_WATER;
         int b[3];
-        for (int d = 0; d < 3; d++) b[d] = (int)(P->pos[d] * s->cdinv - 0.5);
+        for (int d = 0; d < 3; d++) b[d] = (int)(P->pos[d] * s->cdinvv[d] - 0.5);
         for (int k = 0; k < 8; k++) {
             fo_cell* c = &s->cells[cidx(s, b[0] + CORNER[k][0], b[1] + CORNER[k][1], b[2] + CORNER[k][2])];
-            double w = tri_w(c->pos, P->pos, s->cdinv);
+            double w = tri_w(c->pos, P->pos, s->cdinvv);
 #pragma omp atomic
             c->avg += w;
         }
@@ -618,9 +631,9 @@ static void solidify(fo_sim* s, int x, int y, int z, const double speed[3]) {
 
 static void fo_add_obstacle(fo_sim* s, const fsim_obstacle* o) {
     double ctr[3];
-    for (int a = 0; a < 3; a++) ctr[a] = o->pos[a] * s->cdinv;
+    for (int a = 0; a < 3; a++) ctr[a] = o->pos[a] * s->cdinvv[a];
     if ((o->kind == FSIM_OBS_SPHERE || o->kind == FSIM_OBS_SOURCE)) {
-        double r = s->cdinv * o->r, r2 = r * r;
+        double r = s->cdinv * o->r, r2 = r * r;   /* cellDInv.x * r */
         int mn[3], mx[3];
         for (int a = 0; a < 3; a++) {
             mn[a] = (int)fmax(ctr[a] - r, 1.0);
@@ -636,7 +649,7 @@ static void fo_add_obstacle(fo_sim* s, const fsim_obstacle* o) {
     if (o->kind == FSIM_OBS_BOX) {
         int mn[3], mx[3];
         for (int a = 0; a < 3; a++) {
-            double d = o->size[a] * s->cdinv * 0.5;
+            double d = o->size[a] * s->cdinvv[a] * 0.5;
             mn[a] = (int)fmax(round(ctr[a] - d), 1.0);
             mx[a] = (int)fmin(round(ctr[a] + d), s->n[a] - 1.0);
         }
@@ -1067,16 +1080,20 @@ static void fo_g2p(fo_sim* s) {
         int base[3][3];
         faces_base(s, P->pos, base);
         for (int a = 0; a < 3; a++) {
+            if (a == 2 && s->two_d) {   /* 2D: v.z = 0, c[2] kept (simulator.cpp:382-385) */
+                P->v[2] = 0.0;
+                break;
+            }
             double pic = 0, flip = 0, cv[3] = {0, 0, 0};
             for (int k = 0; k < 8; k++) {
                 const fo_face* f = &s->cells[cidx(s, base[a][0] + CORNER[k][0], base[a][1] + CORNER[k][1],
                                                   base[a][2] + CORNER[k][2])].f[a];
-                double w = tri_w(f->pos, P->pos, s->cdinv);
+                double w = tri_w(f->pos, P->pos, s->cdinvv);
                 pic += f->v2 * w;
                 if (tt == FSIM_FLIP) flip += (f->v2 - f->v) * w;
                 if (tt == FSIM_APIC) {
                     double g[3];
-                    tri_grad(f->pos, P->pos, s->cdinv, g);
+                    tri_grad(f->pos, P->pos, s->cdinvv, g);
                     for (int d = 0; d < 3; d++) cv[d] += g[d] * f->v2;
                 }
             }
@@ -1097,7 +1114,7 @@ void fo_bin(const fo_sim* s, int32_t* count, int32_t* start, int64_t* perm) {
     int64_t* key = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->np ? s->np : 1));
     for (int64_t i = 0; i < s->np; i++) {
         const double* p = s->p[i].pos;
-        key[i] = cidx(s, (int)(p[0] * s->cdinv), (int)(p[1] * s->cdinv), (int)(p[2] * s->cdinv));
+        key[i] = cidx(s, (int)(p[0] * s->cdinvv[0]), (int)(p[1] * s->cdinvv[1]), (int)(p[2] * s->cdinvv[2]));
         count[key[i]]++;
     }
     int64_t acc = 0;
@@ -1115,12 +1132,12 @@ void fo_gfx_snapshot(const fo_sim* s, double* out) {
         const fo_particle* P = &s->p[i];
         /* getCellsAround, macGrid.cpp:188-202 */
         int base[3];
-        for (int d = 0; d < 3; d++) base[d] = (int)(P->pos[d] * s->cdinv - 0.5);
+        for (int d = 0; d < 3; d++) base[d] = (int)(P->pos[d] * s->cdinvv[d] - 0.5);
         float density = 0; /* the reference's float accumulator (:226) */
         for (int k = 0; k < 8; k++) {
             const int x = base[0] + CORNER[k][0], y = base[1] + CORNER[k][1], z = base[2] + CORNER[k][2];
             const fo_cell* c = &s->cells[cidx(s, x, y, z)];
-            density += tri_w(P->pos, c->pos, s->cdinv) * c->avg; /* trilinearInterpoll(p.pos, c.cell.pos) */
+            density += tri_w(P->pos, c->pos, s->cdinvv) * c->avg; /* trilinearInterpoll(p.pos, c.cell.pos) */
         }
         out[5 * i + 0] = (float)P->pos[0];
         out[5 * i + 1] = (float)P->pos[1];

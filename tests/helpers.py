@@ -89,7 +89,8 @@ def copy_grid_to_oracle(sim: capi.Simulation, ref, fields) -> None:
         ref.set_grid(f, sim.get_grid(f).astype(np.float64))
 
 
-def staged_step(sim: capi.Simulation, ref, dt: float, spawn: bool = False, stop: bool = False) -> dict:
+def staged_step(sim: capi.Simulation, ref, dt: float, spawn: bool = False, stop: bool = False,
+                push_apart: bool = False) -> dict:
     """One step of the reference's stage order (simulator.cpp:51-100) on both
     backends, stage by stage, each stage started from identical inputs: after
     every stage the oracle takes the GPU's FP32 result (particles or grid
@@ -108,7 +109,11 @@ def staged_step(sim: capi.Simulation, ref, dt: float, spawn: bool = False, stop:
         p_r, v_r, _ = ref.get_particles()
         e["spawn_pos_abs"] = float(np.abs(p_g - p_r).max()) if len(p_g) else 0.0
         sync_oracle_from_gpu(sim, ref)
-    for stage, name in ((capi.STAGE_ADVECT, "advect"), (capi.STAGE_PUSH_OUT, "push_out")):
+    stages = [(capi.STAGE_ADVECT, "advect")]
+    if push_apart:   # pushParticlesApart runs between advection and push-out (simulator.cpp:61-64)
+        stages.append((capi.STAGE_PUSH_APART, "push_apart"))
+    stages.append((capi.STAGE_PUSH_OUT, "push_out"))
+    for stage, name in stages:
         sim.run_stage(stage, dt)
         ref.run_stage(stage, dt)
         sim.run_stage(capi.STAGE_BIN, dt)   # drops the particles a sink removed, as the oracle's advection does
@@ -164,10 +169,18 @@ def staged_step(sim: capi.Simulation, ref, dt: float, spawn: bool = False, stop:
     p_g, v_g, c_g = sim.get_particles()
     p_r, v_r, c_r = ref.get_particles()
     e["g2p_vel"] = field_rel_err(v_g, v_r)
-    e["g2p_c"] = field_rel_err(c_g, c_r) if np.abs(c_r).max() > 0 else 0.0
+    # 2D: the particles sit on the cell-centre plane z = depth / 2, the peak of the z tent, where
+    # trilinearInterpollGradient's sign(p.z - face.z) (interpolation.h:23-35) decides the z component
+    # of the u and v gradients.  For the depth 0.1 of these scenes the reference (FP64 positions) and
+    # the GPU (FP32 positions and face positions) both see p.z - face.z = 0 there; the oracle, fed the
+    # FP32 z = 0.05f against its FP64 face z = 0.05, sees +7e-10 and takes the other side.  Those two components (c[0].z, c[1].z)
+    # have no effect on P2G (the faces of the particle's layer are at its z, the other layers weigh
+    # 0), so they are left out of the 2D comparison; c[2] (never written in 2D) is compared.
+    cols = [0, 1, 3, 4, 6, 7, 8] if sim.grid.two_d else list(range(9))
+    e["g2p_c"] = field_rel_err(c_g[:, cols], c_r[:, cols]) if np.abs(c_r[:, cols]).max() > 0 else 0.0
     return e
 
 
 STAGE_FIELD_KEYS = ("p2g", "density", "prep", "rhs", "pressure", "projected", "extrapolate", "g2p_vel")
 STAGE_EXACT_KEYS = ("bin_count_mismatch", "bin_start_mismatch", "bin_perm_mismatch", "mark_label_mismatch",
-                    "prep_label_mismatch", "advect_count_diff", "push_out_count_diff")
+                    "prep_label_mismatch", "advect_count_diff", "push_apart_count_diff", "push_out_count_diff")

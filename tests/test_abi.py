@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings_and_version():
     lib = capi.load()
-    assert lib.fsim_abi_version() == 2
+    assert lib.fsim_abi_version() == 3
     assert lib.fsim_status_string(0) == b"ok"
     assert lib.fsim_status_string(3) == b"unsupported feature"
 
@@ -51,7 +51,7 @@ int main(void) {
   printf("fsim_obstacle %zu\n", sizeof(fsim_obstacle));
   printf("fsim_step_stats %zu\n", sizeof(fsim_step_stats));
   printf("fsim_gfx_particle %zu\n", sizeof(fsim_gfx_particle));
-  P(fsim_grid_desc, cell_d) P(fsim_grid_desc, two_d)
+  P(fsim_grid_desc, cell_d) P(fsim_grid_desc, two_d) P(fsim_grid_desc, cell_d_z)
   P(fsim_params, pressure_k) P(fsim_params, max_iterations) P(fsim_params, fluid_density)
   P(fsim_params, stop_rule) P(fsim_params, seed)
   P(fsim_obstacle, pos) P(fsim_obstacle, r) P(fsim_obstacle, last_spawn_fraction)
@@ -106,8 +106,17 @@ def test_argument_validation_without_device():
     p = capi.make_params()
     bad = capi.GridDesc(2, 8, 8, 0.125, 0.03, 0)
     assert lib.fsim_create(C.byref(bad), C.byref(p), C.byref(ctx)) == capi.ERR_INVALID
-    two_d = capi.GridDesc(8, 8, 3, 0.125, 0.03, 1)
-    assert lib.fsim_create(C.byref(two_d), C.byref(p), C.byref(ctx)) == capi.ERR_UNSUPPORTED
+    # the 2D mode: nz = 3 and a z cell size (the depth / 3); the 3D mode: cellD.z == cellD.x
+    for nz, two_d, cdz, status in ((3, 1, 0.0, capi.ERR_INVALID), (4, 1, 0.3, capi.ERR_INVALID),
+                                   (8, 0, 0.2, capi.ERR_INVALID), (3, 1, 0.3, capi.ERR_NO_DEVICE),
+                                   (8, 0, 0.125, capi.ERR_NO_DEVICE)):
+        g = capi.GridDesc(8, 8, nz, 0.125, 0.03, two_d, cdz)
+        assert lib.fsim_create(C.byref(g), C.byref(p), C.byref(ctx)) == status, (nz, two_d, cdz)
+    # the learned multigrid preconditioner is 3D only
+    learned = capi.make_params(precond=capi.PRECOND_LEARNED)
+    for g, status in ((capi.GridDesc(8, 8, 3, 0.125, 0.03, 1, 0.3), capi.ERR_INVALID),
+                      (capi.GridDesc(8, 8, 8, 0.125, 0.03, 0, 0.0), capi.ERR_NO_DEVICE)):
+        assert lib.fsim_create(C.byref(g), C.byref(learned), C.byref(ctx)) == status
 
 
 def test_scene_generator_structure():

@@ -50,12 +50,13 @@ def check_stages(e: dict, step: int, bar: float = BAR, counts_only: bool = False
         return
     for k in STAGE_FIELD_KEYS:
         assert e[k] < bar, (step, k, e[k])
-    assert e.get("advect_pos_abs", 0.0) < bar and e.get("push_out_pos_abs", 0.0) < bar, (step, e)
+    for k in ("advect_pos_abs", "push_apart_pos_abs", "push_out_pos_abs", "g2p_c"):
+        assert e.get(k, 0.0) < bar, (step, k, e)
     assert o - 2 <= g <= 1.5 * o + 5, (step, g, o)   # a 1e-10 solve
 
 
 def worst(errs: list) -> str:
-    keys = STAGE_FIELD_KEYS + ("advect_pos_abs", "push_out_pos_abs", "g2p_c")
+    keys = STAGE_FIELD_KEYS + ("advect_pos_abs", "push_apart_pos_abs", "push_out_pos_abs", "g2p_c")
     return ", ".join(f"{k} {max(e.get(k, 0.0) for e in errs):.1e}" for k in keys)
 
 
@@ -72,7 +73,7 @@ def run_staged(sc, prm, steps: int, threads: int, start_steps: int = 0, spawn: b
     ref = oracle.Oracle(sc.grid, prm, threads=threads)
     errs = []
     for k in range(steps):
-        e = staged_step(sim, ref, DT, spawn=spawn)
+        e = staged_step(sim, ref, DT, spawn=spawn, push_apart=bool(prm.push_apart_enabled))
         errs.append(e)
         if k % 10 == 0 or k == steps - 1:
             say(f"  step {k}: pcg {e['pcg_gpu']} / {e['pcg_oracle']} (oracle), p {e['pressure']:.1e}, "
