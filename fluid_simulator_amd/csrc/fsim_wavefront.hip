@@ -346,6 +346,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
         rA = rsrc(A.P32, (int64_t)nc * 4 + 8);
         rB = rsrc(A.in, (int64_t)nc * 4);
         rC = rsrc(A.out, (int64_t)nc * 4 + 4);
+        // opaque to the compiler from here on, so that it keeps each descriptor whole in
+        // four SGPRs instead of sharing equal words and reassembling them at every access
+        asm volatile("" : "+s"(rA), "+s"(rB), "+s"(rC));
         rD = rC;
     }
     const float ms = A.ms;
@@ -424,6 +427,27 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
     const uint64_t* yp_base = &S.yp[0][0][tz];
     const uint64_t* zp_base = &S.zp[0][0][w * R];
     const bool cy = w == 0 && yup;
+    // Sweeps: the readiness test as 64-bit lane masks.  Each tag test is one
+    // compare whose ballot is ANDed with a mask of the lanes that need it:
+    // the uniform conditions select a whole mask in scalar code, the lane
+    // conditions (row in the grid, x in the grid) are ballots of single
+    // compares, so no boolean is turned into a lane value and back.
+    // Row 0's value is needed by every lane for w > 0 (at t = 0 the ring's
+    // initial tag -1 matches t - 1), and for wave 0 by the lanes of its first
+    // row that are in the grid, when there is an upstream y tile.
+    // Row 0's value is needed by every lane for w > 0 (at t = 0 the ring's
+    // initial tag -1 matches t - 1), and for wave 0 by the lanes of its first
+    // row that are in the grid, when there is an upstream y tile.  The -z
+    // packets are needed by the rows with an upstream z tile while the row's
+    // lane 0 is in the grid.
+    const bool cy_u = __builtin_amdgcn_readfirstlane(cy ? 1 : 0) != 0;
+    const uint64_t ma_all = w > 0 ? ~0ull : 0ull;
+    const uint64_t ma_row = cy_u ? __builtin_amdgcn_ballot_w64(colr[0]) : 0ull;
+    uint64_t colz_m[R];
+    unroll<R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        colz_m[r] = __builtin_amdgcn_ballot_w64(colz[r]);
+    });
     T prev[R];
     unroll<R>([&](auto rc) { prev[decltype(rc)::value] = 0; });
     // Upstream values of one step: the previous wave's last row (ring slot),
@@ -498,14 +522,16 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             // read by every lane from one address, so their test is uniform)
             auto ready = [&]() {
                 if constexpr (KIND != 0) {
-                    // ballots taken on the comparisons themselves (no boolean lane value to rebuild)
                     const int ta = w > 0 ? t - 1 : t;
-                    const bool need_a = w > 0 ? need_up : need_y;
-                    uint64_t bad = __builtin_amdgcn_ballot_w64(need_a && tag_of(a[0]) != ta) |
-                                   __builtin_amdgcn_ballot_w64(need_bp && tag_of(bp) < t_bp);
+                    const uint64_t m_a = ma_all | (ma_row & __builtin_amdgcn_ballot_w64(xin));
+                    const bool xin0_u = (unsigned)(t - w) < (unsigned)nxi;
+                    uint64_t bad = __builtin_amdgcn_ballot_w64(tag_of(a[0]) != ta) & m_a;
+                    if constexpr (bp_step) {
+                        if (w < NW - 1 && t >= kRingV - kBpEvery) bad |= __builtin_amdgcn_ballot_w64(tag_of(bp) < t_bp);
+                    }
                     unroll<R>([&](auto rc) {
                         constexpr int r = decltype(rc)::value;
-                        bad |= __builtin_amdgcn_ballot_w64(need_z[r] && tag_of(zq[r][0]) != t);
+                        bad |= __builtin_amdgcn_ballot_w64(tag_of(zq[r][0]) != t) & (xin0_u ? colz_m[r] : 0ull);
                     });
                     return bo_ok && bad == 0;
                 }
