@@ -461,10 +461,16 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
     uint64_t a[NP], c[NP], zq[R][NP], bp = 0;
     const uint64_t* a0_base = (KIND != 0 && w == 0) ? yp_base : up_base;
     int spy = 0, spz = 0;   // progress of the sending waves (read on reuse-check steps)
+    // ring slots of the current block of kPrefetch steps: step t0 + d uses slot pz0 + d of the
+    // packet rings and so0 + d of the out-rings (no wrap inside a block), so the slot offsets of
+    // the unrolled steps are constants
+    static_assert(RINGP % kPrefetch == 0 && kRingO % kPrefetch == 0, "blocks do not wrap the rings");
+    int pz0 = 0, so0 = 0;
     auto rd_at = [&](auto dc2, int tt) {
         constexpr int d2 = decltype(dc2)::value;   // tt mod kPrefetch (= tt mod kRingV)
         constexpr int s_up = (d2 + kRingV - 1) % kRingV, s_bp = (d2 + kBpEvery) % kRingV;
-        const int ps = tt % RINGP;
+        // tt = t0 + d2 inside the block (d2 > 0), or the first step of the next block
+        const int ps = d2 != 0 ? pz0 + d2 : tt % RINGP;
         if constexpr (KIND != 0) {
             const int aoff = w > 0 ? s_up * kSlotV : ps * NP * WF_TZ;
 #pragma unroll
@@ -490,6 +496,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
     static_assert(kRingV == kPrefetch, "ring slots are indexed by the unrolled step");
     rd_at(std::integral_constant<int, 0>{}, 0);
     for (int t0 = 0; t0 < nsteps; t0 += kPrefetch) {
+        pz0 = t0 % RINGP;
+        so0 = t0 & (kRingO - 1);
+        uint64_t* const zblk = zo_lane + (uint32_t)so0 * zo_stride;
         // compile-time step index, so each prefetch slot stays in one register
         // and the value-ring slots are constants
         unroll<kPrefetch>([&](auto dc) {
@@ -637,7 +646,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             // every row into the +z out-ring (both sent by the sending waves), then read ahead,
             // advance and prefetch step t + kPrefetch
             // (the last wave also keeps writing its ring slot: wave NW-2's reuse check reads its tags)
-            const int so = t & (kRingO - 1);
+            const int so = so0 + d;   // t & (kRingO - 1)
 #pragma unroll
             for (int k = 0; k < NP; k++) lput(my_base + d * kSlotV + k * WF_TZ, pack(word(vnew[R - 1], k), (uint32_t)t));
             if (w == NW - 1) {
@@ -647,7 +656,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             // (the factor's FP64 values would make the sink writes of all lanes cost more LDS
             // bandwidth than the lane mask costs instructions: it keeps the mask)
             if (KIND != 0 || zl) {
-                uint64_t* zdst = zo_lane + (uint32_t)so * zo_stride;
+                uint64_t* zdst = zblk + (uint32_t)d * zo_stride;
                 unroll<R>([&](auto rc) {
                     constexpr int r = decltype(rc)::value;
 #pragma unroll
