@@ -556,7 +556,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
                 }
                 return bo_ok && __builtin_amdgcn_ballot_w64(lane == 0u) == 0;
             };
-            if (!ready()) {
+            if (__builtin_expect(!ready(), 0)) {   // the spin path out of line: the ready step falls through
                 int spins = 0;
                 do {
                     if (++spins > kSpinLimit) {
