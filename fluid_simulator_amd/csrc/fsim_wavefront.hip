@@ -105,6 +105,11 @@ __device__ inline int lpeek32(const int* p) { return __hip_atomic_load(p, __ATOM
 __device__ inline bool gave_up(const int* err) { return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0; }
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+// 16-byte LDS accesses (ds_read_b128 / ds_write_b128) through an explicit LDS pointer
+typedef __attribute__((address_space(3))) volatile u64x2 lds_u64x2;
+__device__ inline u64x2 lpeek2(const uint64_t* p) { return *(lds_u64x2*)(p); }
+__device__ inline void lput2(uint64_t* p, u64x2 v) { *(lds_u64x2*)(p) = v; }
 __device__ inline __amdgpu_buffer_rsrc_t rsrc(const void* p, int64_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, kRawBufferFlags);
 }
@@ -179,11 +184,12 @@ struct WaveLds {
     static constexpr int NP = Kind<KIND>::NP, RINGP = Kind<KIND>::RINGP;
     uint64_t v[kRingV][WF_NW][NP][WF_TZ];   // tagged last-row values of each compute wave
     uint64_t yp[RINGP][NP][WF_TZ];          // tagged -y packets for row 0
-    uint64_t zp[RINGP][NP][WF_TY];          // tagged -z packets for lane 0 of each row
+    alignas(16) uint64_t zp[RINGP][NP][WF_TY];   // tagged -z packets for lane 0 of each row
     int prog[WF_NW];                        // a step whose packets the compute wave has consumed
     uint64_t yo[kRingO][NP][WF_TZ];         // tagged last row of the last compute wave: the +y edge
-    uint64_t zo[kRingO][NP][WF_TY];         // tagged lane-63 value of every row: the +z edge
-    uint64_t zsink[KIND == 0 ? 1 : WF_NW][WF_TZ - 1][NP * WF_TY];   // sweeps: +z writes of lanes 0..62, never read
+    alignas(16) uint64_t zo[kRingO][NP][WF_TY];   // tagged lane-63 value of every row: the +z edge
+    // sweeps: +z writes of lanes 0..62, never read
+    alignas(16) uint64_t zsink[KIND == 0 ? 1 : WF_NW][WF_TZ - 1][NP * WF_TY];
     int sprog[2];                           // compute step up to which the +y / +z edge has been sent
 };
 
@@ -482,11 +488,18 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
                 c[k] = lpeek(yp_base + ps * NP * WF_TZ + k * WF_TZ);
             }
         }
-        unroll<R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
+        if constexpr (KIND != 0 && R == 2) {
+            // both rows' words in one 16-byte LDS read (each word carries its own tag)
+            const u64x2 q = lpeek2(zp_base + ps * TY);
+            zq[0][0] = q.x;
+            zq[1][0] = q.y;
+        } else {
+            unroll<R>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
 #pragma unroll
-            for (int k = 0; k < NP; k++) zq[r][k] = lpeek(zp_base + ps * NP * TY + k * TY + r);
-        });
+                for (int k = 0; k < NP; k++) zq[r][k] = lpeek(zp_base + ps * NP * TY + k * TY + r);
+            });
+        }
         if constexpr (d2 % kBpEvery == 0) {
             bp = lpeek(dn_base + s_bp * kSlotV);
             spy = lpeek32(&S.sprog[0]);
@@ -657,11 +670,19 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             // bandwidth than the lane mask costs instructions: it keeps the mask)
             if (KIND != 0 || zl) {
                 uint64_t* zdst = zblk + (uint32_t)d * zo_stride;
-                unroll<R>([&](auto rc) {
-                    constexpr int r = decltype(rc)::value;
+                if constexpr (KIND != 0 && R == 2) {
+                    // both rows in one 16-byte LDS write (the sending wave reads them word by word)
+                    u64x2 q;
+                    q.x = pack(word(vnew[0], 0), (uint32_t)t);
+                    q.y = pack(word(vnew[1], 0), (uint32_t)t);
+                    lput2(zdst, q);
+                } else {
+                    unroll<R>([&](auto rc) {
+                        constexpr int r = decltype(rc)::value;
 #pragma unroll
-                    for (int k = 0; k < NP; k++) lput(zdst + k * TY + r, pack(word(vnew[r], k), (uint32_t)t));
-                });
+                        for (int k = 0; k < NP; k++) lput(zdst + k * TY + r, pack(word(vnew[r], k), (uint32_t)t));
+                    });
+                }
             }
             rd_at(std::integral_constant<int, (d + 1) % kPrefetch>{}, t + 1);
             // the packet reads of steps <= t have completed (their values were used above)
