@@ -182,7 +182,8 @@ __device__ inline int col_water(const WaveArgs& A, int J, int K, int ty, int tz)
 template <int KIND>
 struct WaveLds {
     static constexpr int NP = Kind<KIND>::NP, RINGP = Kind<KIND>::RINGP;
-    uint64_t v[kRingV][WF_NW][NP][WF_TZ];   // tagged last-row values of each compute wave
+    // tagged last-row values of compute waves 0 .. WF_NW-2 (the last wave's go to yo)
+    uint64_t v[WF_NW - 1][kRingV][NP][WF_TZ];
     uint64_t yp[RINGP][NP][WF_TZ];          // tagged -y packets for row 0
     alignas(16) uint64_t zp[RINGP][NP][WF_TY];   // tagged -z packets for lane 0 of each row
     int prog[WF_NW];                        // a step whose packets the compute wave has consumed
@@ -421,10 +422,14 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
         advance(op);
         __builtin_amdgcn_sched_barrier(0);
     });
-    constexpr int kSlotV = NW * NP * WF_TZ;   // words per value-ring slot
-    const uint64_t* up_base = &S.v[0][w > 0 ? w - 1 : 0][0][tz];
-    const uint64_t* dn_base = &S.v[0][w < NW - 1 ? w + 1 : w][0][tz];
-    uint64_t* my_base = &S.v[0][w][0][tz];
+    // Value rings: wave w < NW-1 writes its last row into v[w] (slot t mod kRingV), the last wave
+    // into the +y out-ring yo (slot t mod kRingO), which the sending wave forwards and wave NW-2's
+    // reuse check reads.  Both have kSlotV words per slot, so a step's slot is a constant offset
+    // from the block's base.
+    constexpr int kSlotV = NP * WF_TZ;   // words per ring slot
+    const uint64_t* up_base = &S.v[w > 0 ? w - 1 : 0][0][0][tz];
+    const uint64_t* dn_base = &S.v[w < NW - 2 ? w + 1 : 0][0][0][tz];
+    const bool dn_yo = w == NW - 2;   // the downstream wave's ring is yo
     // +z out-ring: lane 63 writes its rows into zo[slot][k][w R + r]; the other lanes write the same
     // pattern into a sink of their own, so the store needs no lane mask (slot stride 0 there)
     const bool zl = tz == WF_TZ - 1;
@@ -501,7 +506,9 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             });
         }
         if constexpr (d2 % kBpEvery == 0) {
-            bp = lpeek(dn_base + s_bp * kSlotV);
+            // the downstream wave's slot of step tt - (kRingV - kBpEvery)
+            const int so_bp = d2 != 0 ? so0 + d2 - (kRingV - kBpEvery) : (tt - (kRingV - kBpEvery)) & (kRingO - 1);
+            bp = lpeek(dn_yo ? &S.yo[so_bp][0][tz] : dn_base + s_bp * kSlotV);
             spy = lpeek32(&S.sprog[0]);
             spz = lpeek32(&S.sprog[1]);
         }
@@ -511,6 +518,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
     for (int t0 = 0; t0 < nsteps; t0 += kPrefetch) {
         pz0 = t0 % RINGP;
         so0 = t0 & (kRingO - 1);
+        uint64_t* const my_blk = w == NW - 1 ? &S.yo[so0][0][tz] : &S.v[w][0][0][tz];
         uint64_t* const zblk = zo_lane + (uint32_t)so0 * zo_stride;
         // compile-time step index, so each prefetch slot stays in one register
         // and the value-ring slots are constants
@@ -658,14 +666,8 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             // ---- publish: the last row for wave w+1 (the last wave's into the +y out-ring), lane 63 of
             // every row into the +z out-ring (both sent by the sending waves), then read ahead,
             // advance and prefetch step t + kPrefetch
-            // (the last wave also keeps writing its ring slot: wave NW-2's reuse check reads its tags)
-            const int so = so0 + d;   // t & (kRingO - 1)
 #pragma unroll
-            for (int k = 0; k < NP; k++) lput(my_base + d * kSlotV + k * WF_TZ, pack(word(vnew[R - 1], k), (uint32_t)t));
-            if (w == NW - 1) {
-#pragma unroll
-                for (int k = 0; k < NP; k++) lput(&S.yo[so][k][tz], pack(word(vnew[R - 1], k), (uint32_t)t));
-            }
+            for (int k = 0; k < NP; k++) lput(my_blk + d * kSlotV + k * WF_TZ, pack(word(vnew[R - 1], k), (uint32_t)t));
             // (the factor's FP64 values would make the sink writes of all lanes cost more LDS
             // bandwidth than the lane mask costs instructions: it keeps the mask)
             if (KIND != 0 || zl) {
@@ -720,7 +722,7 @@ __global__ void __launch_bounds__(WF_THREADS) k_wave(WaveArgs A) {
     }
     if (!__syncthreads_or(mine)) return;
     const bool yup = __syncthreads_or(upy) != 0, zup = __syncthreads_or(upz) != 0;
-    for (int i = threadIdx.x; i < kRingV * WF_NW * NP * WF_TZ; i += WF_THREADS) (&S.v[0][0][0][0])[i] = ~0ull;
+    for (int i = threadIdx.x; i < (WF_NW - 1) * kRingV * NP * WF_TZ; i += WF_THREADS) (&S.v[0][0][0][0])[i] = ~0ull;
     // sweeps on a first y / z tile, or below / beside a tile without WATER, read their -y / -z
     // values from a ring of zeros (never refilled)
     const uint64_t yp_init = KIND != 0 && !yup ? 0ull : ~0ull, zp_init = KIND != 0 && !zup ? 0ull : ~0ull;
