@@ -336,7 +336,11 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
     const int nsteps = wave_steps(nxi);
     const uint32_t plane = (uint32_t)(A.ny * A.nz);
     const uint32_t nc = (uint32_t)nx * plane;   // below 2^28 (kWaveMaxCells)
-    const uint32_t plane8 = plane * 8u, nc8 = nc * 8u;
+    // Plane offsets are byte offsets for the widest element a kernel addresses: 8-byte (the
+    // factor's P64) or 4-byte (the sweeps), so the sweeps use them without a shift.
+    constexpr uint32_t kUnit = KIND == 0 ? 8u : 4u;
+    const uint32_t plane8 = plane * kUnit, nc8 = nc * kUnit;
+    auto off4 = [](uint32_t o) { return KIND == 0 ? o >> 1 : o; };   // 4-byte elements
     // buffer resources.  factor: rA nbinfo, rC P64, rD P32;  sweeps: rA P32, rB in, rC out.
     // The sweeps read P32 alone: it is 0 for every non-WATER cell (cleared before each factor,
     // which writes WATER cells only), so P != 0 is the WATER test and the neighbour flags are
@@ -406,8 +410,8 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             wring[d][r] = bload16(rA, lo2[r], op >> 2);
             iring[d][r] = 0.f;
         } else {
-            wring[d][r] = __float_as_uint(bload32f(rA, lo4[r], op >> 1));
-            iring[d][r] = bload32f(rB, lo4[r], op >> 1);
+            wring[d][r] = __float_as_uint(bload32f(rA, lo4[r], off4(op)));
+            iring[d][r] = bload32f(rB, lo4[r], off4(op));
         }
     };
     // issue order pinned, and each prologue step issues as many memory
@@ -477,15 +481,23 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
     // the unrolled steps are constants
     static_assert(RINGP % kPrefetch == 0 && kRingO % kPrefetch == 0, "blocks do not wrap the rings");
     int pz0 = 0, so0 = 0;
+    // sweeps: the block's base for row 0's -y value, so that step t0 + d2 (d2 = 1 .. kPrefetch-1)
+    // reads at a constant offset (d2 - 1 slots): the upstream wave's ring (slot d2 - 1) or, for
+    // wave 0, the packet ring (slot pz0 + d2)
+    const uint64_t* a_blk = up_base;
     auto rd_at = [&](auto dc2, int tt) {
         constexpr int d2 = decltype(dc2)::value;   // tt mod kPrefetch (= tt mod kRingV)
         constexpr int s_up = (d2 + kRingV - 1) % kRingV, s_bp = (d2 + kBpEvery) % kRingV;
         // tt = t0 + d2 inside the block (d2 > 0), or the first step of the next block
         const int ps = d2 != 0 ? pz0 + d2 : tt % RINGP;
         if constexpr (KIND != 0) {
-            const int aoff = w > 0 ? s_up * kSlotV : ps * NP * WF_TZ;
-#pragma unroll
-            for (int k = 0; k < NP; k++) a[k] = lpeek(a0_base + aoff + k * WF_TZ);
+            static_assert(NP == 1, "one word per sweep value");
+            if constexpr (d2 != 0) {
+                a[0] = lpeek(a_blk + (d2 - 1) * kSlotV);
+            } else {   // tt = t0 + kPrefetch, or tt = t0 (re-read) / 0 (first read)
+                const int aoff = w > 0 ? s_up * kSlotV : ps * NP * WF_TZ;
+                a[0] = lpeek(a0_base + aoff);
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < NP; k++) {
@@ -519,6 +531,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
         pz0 = t0 % RINGP;
         so0 = t0 & (kRingO - 1);
         uint64_t* const my_blk = w == NW - 1 ? &S.yo[so0][0][tz] : &S.v[w][0][0][tz];
+        a_blk = w > 0 ? up_base : yp_base + (pz0 + 1) * NP * WF_TZ;
         uint64_t* const zblk = zo_lane + (uint32_t)so0 * zo_stride;
         // compile-time step index, so each prefetch slot stays in one register
         // and the value-ring slots are constants
@@ -641,7 +654,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
                     vnew[r] = v;
                     // P64/P32 for WATER cells (both were cleared before the factor)
                     bstore64(__builtin_bit_cast(uint64_t, v), rC, fl ? lo8[r] : kOob, oc);
-                    bstore32(__float_as_uint(p32), rD, fl ? lo4[r] : kOob, oc >> 1);
+                    bstore32(__float_as_uint(p32), rD, fl ? lo4[r] : kOob, off4(oc));
                 } else {
                     const float p = __uint_as_float(wring[d][r]);
                     // stores of WATER cells only: storing every interior cell doubles the stores, and
@@ -660,7 +673,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
                         v = outv;
                     }
                     vnew[r] = fl ? v : 0.f;
-                    bstore32(__float_as_uint(outv), rC, fl ? lo4[r] : kOob, oc >> 1);
+                    bstore32(__float_as_uint(outv), rC, fl ? lo4[r] : kOob, off4(oc));
                 }
             });
             // ---- publish: the last row for wave w+1 (the last wave's into the +y out-ring), lane 63 of
