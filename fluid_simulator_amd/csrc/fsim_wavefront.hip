@@ -519,7 +519,7 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
         }
         if constexpr (d2 % kBpEvery == 0) {
             // the downstream wave's slot of step tt - (kRingV - kBpEvery)
-            const int so_bp = d2 != 0 ? so0 + d2 - (kRingV - kBpEvery) : (tt - (kRingV - kBpEvery)) & (kRingO - 1);
+            const int so_bp = ((d2 != 0 ? so0 + d2 : tt) - (kRingV - kBpEvery)) & (kRingO - 1);
             bp = lpeek(dn_yo ? &S.yo[so_bp][0][tz] : dn_base + s_bp * kSlotV);
             spy = lpeek32(&S.sprog[0]);
             spz = lpeek32(&S.sprog[1]);
