@@ -398,7 +398,10 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             o += plane8;
             if (o >= nc8) o -= nc8;
         } else {
-            o = o >= plane8 ? o - plane8 : o + nc8 - plane8;
+            // o - plane8 wraps above every offset when o < plane8: the smaller of the two is the
+            // wrapped-around plane (scalar subtract, add and min, as in the forward direction)
+            const uint32_t below = o - plane8, wrapped = o + (nc8 - plane8);
+            o = below < wrapped ? below : wrapped;
         }
         o = __builtin_amdgcn_readfirstlane(o);
     };
