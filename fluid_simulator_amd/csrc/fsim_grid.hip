@@ -727,6 +727,9 @@ struct G2PArgs {
     float flip, one_minus_flip;
 };
 
+// TT: the transfer type, a template argument so that the 8-corner loops have
+// no branches and all their loads are issued before the first is used.
+template <int TT>
 __global__ void __launch_bounds__(kBlock) k_g2p(G2PArgs A) {
     const GridArgs& G = A.G;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -768,8 +771,8 @@ __global__ void __launch_bounds__(kBlock) k_g2p(G2PArgs A) {
             float w = tx[ox] * ty[oy] * tz[oz];
             float v2 = A.uvw2[a][f];
             pic += v2 * w;
-            if (A.transfer == FSIM_FLIP) flip += (v2 - A.uvw[a][f]) * w;
-            if (A.transfer == FSIM_APIC) {
+            if constexpr (TT == FSIM_FLIP) flip += (v2 - A.uvw[a][f]) * w;
+            if constexpr (TT == FSIM_APIC) {
                 // trilinearInterpollGradient, interpolation.h:23-35 (sign(0) = +1)
                 float sx = dxs[ox] > 0.f ? -1.f : 1.f, sy = dys[oy] > 0.f ? -1.f : 1.f, sz = dzs[oz] > 0.f ? -1.f : 1.f;
                 g0 += sx * ty[oy] * tz[oz] * inv * v2;
@@ -777,9 +780,9 @@ __global__ void __launch_bounds__(kBlock) k_g2p(G2PArgs A) {
                 g2 += sz * tx[ox] * ty[oy] * invz * v2;
             }
         }
-        if (A.transfer == FSIM_PIC) {
+        if constexpr (TT == FSIM_PIC) {
             A.vel[a][i] = pic;
-        } else if (A.transfer == FSIM_FLIP) {
+        } else if constexpr (TT == FSIM_FLIP) {
             A.vel[a][i] = pic * A.one_minus_flip + (flip + A.vel[a][i]) * A.flip;
         } else {
             A.vel[a][i] = pic;
@@ -807,7 +810,9 @@ int stage_g2p(fsim_ctx* ctx) {
     A.one_minus_flip = 1.0f - ctx->prm.flip_ratio;
     if (A.transfer == FSIM_APIC) ctx->c_valid = true;
     FSIM_PROF(ctx, KID_G2P);
-    hipLaunchKernelGGL(k_g2p, dim3(nblocks(A.n)), dim3(kBlock), 0, ctx->stream, A);
+    if (A.transfer == FSIM_PIC) hipLaunchKernelGGL(k_g2p<FSIM_PIC>, dim3(nblocks(A.n)), dim3(kBlock), 0, ctx->stream, A);
+    else if (A.transfer == FSIM_FLIP) hipLaunchKernelGGL(k_g2p<FSIM_FLIP>, dim3(nblocks(A.n)), dim3(kBlock), 0, ctx->stream, A);
+    else hipLaunchKernelGGL(k_g2p<FSIM_APIC>, dim3(nblocks(A.n)), dim3(kBlock), 0, ctx->stream, A);
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
 }
