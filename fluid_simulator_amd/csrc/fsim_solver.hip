@@ -58,6 +58,7 @@ struct Reduce {
     unsigned* count;    // zero between launches
     SolveScalars* sc;
     int op;             // FinOp
+    SolveScalars* host = nullptr;   // pinned host copy written by the finishing block, or null
 };
 
 __device__ inline void apply_fin(int op, double s, double m, SolveScalars* sc) {
@@ -156,6 +157,14 @@ __device__ void block_finish(double sum, double aux, const Reduce& R) {
     if (threadIdx.x == 0) {
         apply_fin(R.op, lds_s[0], lds_m[0], R.sc);
         __hip_atomic_store(R.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (R.host) {   // the scalars straight into pinned host memory (no copy after the kernel)
+            static_assert(sizeof(SolveScalars) % 8 == 0, "copied as 8-byte words");
+            const uint64_t* src = reinterpret_cast<const uint64_t*>(R.sc);
+            uint64_t* dst = reinterpret_cast<uint64_t*>(R.host);
+            for (int k = 0; k < (int)(sizeof(SolveScalars) / 8); k++)
+                __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        }
     }
 }
 
@@ -709,6 +718,10 @@ static_assert(FSIM_SPMV_BLOCKS % 8 == 0 && FSIM_SPMV_BLOCKS <= (int)kMaxParts, "
 static unsigned spmv_blocks(int64_t n) { return std::min<unsigned>(red_blocks(n), FSIM_SPMV_BLOCKS); }
 
 static Reduce reduce_to(fsim_ctx* ctx, int op) { return Reduce{ctx->partials, ctx->red_count, ctx->scal, op}; }
+// the same, with the finished scalars also written to the pinned host copy (read after an event)
+static Reduce reduce_to_host(fsim_ctx* ctx, int op) {
+    return Reduce{ctx->partials, ctx->red_count, ctx->scal, op, ctx->scal_host};
+}
 
 static int read_scalars(fsim_ctx* ctx) {
     FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
@@ -924,13 +937,13 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
             FSIM_PROF(ctx, KID_UPDATE);
             if (nc % 4 == 0)
                 hipLaunchKernelGGL(k_update_r<true>, dim3(stream_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
-                                   ctx->chunk_flags, ctx->q, ctx->r, reduce_to(ctx, FIN_RES));
+                                   ctx->chunk_flags, ctx->q, ctx->r, reduce_to_host(ctx, FIN_RES));
             else
                 hipLaunchKernelGGL(k_update_r<false>, dim3(stream_blocks(nc)), dim3(kBlock), 0, ctx->stream, nc,
-                                   ctx->chunk_flags, ctx->q, ctx->r, reduce_to(ctx, FIN_RES));
+                                   ctx->chunk_flags, ctx->q, ctx->r, reduce_to_host(ctx, FIN_RES));
         }
         FSIM_HIP(ctx, hipGetLastError());
-        FSIM_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
+        // the r update's finishing block wrote the scalars into scal_host; the event orders the read
         FSIM_HIP(ctx, hipEventRecord(ctx->res_ev, ctx->stream));
         FSIM_TRY(precondition(ctx, S, flexible));
         FSIM_TRY(next_beta(ctx, flexible));
