@@ -42,10 +42,15 @@ struct GridArgs {
     float cdf, invf, invfz;
     int two_d;
 };
-// cellD, cellDInv of axis d (macGrid.cpp:10)
+// cellD, cellDInv of axis d (macGrid.cpp:10).  The P2G kernels take the 2D
+// switch as a template argument (TWOD): their 3D instances use the x scale on
+// every axis and hold no registers for the z scale (fewer scalar registers,
+// one more wave per SIMD for the row gather).
+template <int TWOD>
+__device__ inline double cdinv_t(const GridArgs& G, int d) { return TWOD && d == 2 ? G.cdzinv : G.cdinv; }
+template <int TWOD>
+__device__ inline float invf_t(const GridArgs& G, int d) { return TWOD && d == 2 ? G.invfz : G.invf; }
 __host__ __device__ inline double cd_of(const GridArgs& G, int d) { return d == 2 ? G.cdz : G.cd; }
-__host__ __device__ inline double cdinv_of(const GridArgs& G, int d) { return d == 2 ? G.cdzinv : G.cdinv; }
-__host__ __device__ inline float invf_of(const GridArgs& G, int d) { return d == 2 ? G.invfz : G.invf; }
 static GridArgs grid_args(const fsim_ctx* ctx) {
     GridArgs G;
     G.nx = ctx->g.nx;
@@ -76,10 +81,11 @@ __device__ inline void face_base(const GridArgs& G, float px, float py, float pz
 }
 
 // Face centre of axis a of cell (x,y,z) (macGrid.cpp:25-27), rounded to FP32.
+template <int TWOD = 1>
 __device__ inline void face_pos(const GridArgs& G, int a, int x, int y, int z, float f[3]) {
     f[0] = (float)((x + (a == 0 ? 1.0 : 0.5)) * G.cd);
     f[1] = (float)((y + (a == 1 ? 1.0 : 0.5)) * G.cd);
-    f[2] = (float)((z + (a == 2 ? 1.0 : 0.5)) * G.cdz);
+    f[2] = (float)((z + (a == 2 ? 1.0 : 0.5)) * (TWOD ? G.cdz : G.cd));
 }
 
 // trilinearInterpoll, interpolation.h:7-20
@@ -123,7 +129,7 @@ struct P2GArgs {
 // chunk in order.
 constexpr int P2G_CAP_FLIP = 512, P2G_CAP_APIC = 512;   // particles per staged chunk
 
-template <int APIC>
+template <int APIC, int TWOD>
 __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
     constexpr int CAP = APIC ? P2G_CAP_APIC : P2G_CAP_FLIP;
     constexpr int NC = APIC ? 9 : 1;
@@ -144,7 +150,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
     {
         float f[3][3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) face_pos(G, a, x, y, z, f[a]);
+        for (int a = 0; a < 3; a++) face_pos<TWOD>(G, a, x, y, z, f[a]);
         fal[0] = f[0][0], fal[1] = f[1][1], fal[2] = f[2][2];
         fce[0] = f[1][0], fce[1] = f[0][1], fce[2] = f[0][2];
     }
@@ -177,7 +183,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         const float pv = A.pos[d][j];
-                        const double v = (double)pv * cdinv_of(G, d);
+                        const double v = (double)pv * cdinv_t<TWOD>(G, d);
                         const int t = (int)v;
                         s_pos[d][k] = pv;
                         s_vel[d][k] = A.vel[d][j];
@@ -205,8 +211,8 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
                     for (int d = 0; d < 3; d++) {
                         ua[d] = cc[d] - s_t[d][k] + 1;
                         uc[d] = ua[d] - (int)((h >> d) & 1u);
-                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * invf_of(G, d));
-                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * invf_of(G, d));
+                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * invf_t<TWOD>(G, d));
+                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * invf_t<TWOD>(G, d));
                     }
                     // getCellsAround (macGrid.cpp:188-202): base (int)(p/dx - 0.5) on every axis
                     if ((unsigned)uc[0] <= 1u && (unsigned)uc[1] <= 1u && (unsigned)uc[2] <= 1u)
@@ -265,7 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_gather(P2GArgs A) {
 #endif
 constexpr int P2G_TX = FSIM_P2G_TX, P2G_TY = FSIM_P2G_TY, P2G_TZ = 64;
 constexpr int P2G_TILE_CAP = 1024;   // particles per staged chunk (a row segment is 66 cells)
-template <int APIC, int TX, int TY>
+template <int APIC, int TX, int TY, int TWOD>
 __global__ void __launch_bounds__(P2G_TZ * TX * TY) k_p2g_tile(P2GArgs A) {
     constexpr int CAP = P2G_TILE_CAP;
     constexpr int NC = APIC ? 9 : 1;
@@ -286,7 +292,7 @@ __global__ void __launch_bounds__(P2G_TZ * TX * TY) k_p2g_tile(P2GArgs A) {
     {
         float f[3][3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) face_pos(G, a, xc, yc, zc, f[a]);
+        for (int a = 0; a < 3; a++) face_pos<TWOD>(G, a, xc, yc, zc, f[a]);
         fal[0] = f[0][0], fal[1] = f[1][1], fal[2] = f[2][2];
         fce[0] = f[1][0], fce[1] = f[0][1], fce[2] = f[0][2];
     }
@@ -318,7 +324,7 @@ __global__ void __launch_bounds__(P2G_TZ * TX * TY) k_p2g_tile(P2GArgs A) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         const float pv = A.pos[d][j];
-                        const double v = (double)pv * cdinv_of(G, d);
+                        const double v = (double)pv * cdinv_t<TWOD>(G, d);
                         const int t = (int)v;
                         s_pos[d][k] = pv;
                         s_vel[d][k] = A.vel[d][j];
@@ -344,8 +350,8 @@ __global__ void __launch_bounds__(P2G_TZ * TX * TY) k_p2g_tile(P2GArgs A) {
                     for (int d = 0; d < 3; d++) {
                         ua[d] = cc[d] - s_t[d][k] + 1;
                         uc[d] = ua[d] - (int)((h >> d) & 1u);
-                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * invf_of(G, d));
-                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * invf_of(G, d));
+                        tal[d] = 1.0f - fabsf((p[d] - fal[d]) * invf_t<TWOD>(G, d));
+                        tce[d] = 1.0f - fabsf((p[d] - fce[d]) * invf_t<TWOD>(G, d));
                     }
                     if ((unsigned)uc[0] <= 1u && (unsigned)uc[1] <= 1u && (unsigned)uc[2] <= 1u)
                         dens += tce[0] * tce[1] * tce[2];
@@ -409,14 +415,25 @@ static int p2g(fsim_ctx* ctx, int fuse_mark) {
     A.apic = ctx->prm.transfer == FSIM_APIC;
     A.fuse_mark = fuse_mark;
     if (ctx->p2g_rows) {   // FSIM_OPT_P2G_KERNEL = 1: one cell row per block
-        if (A.apic) hipLaunchKernelGGL(k_p2g_gather<1>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
-        else hipLaunchKernelGGL(k_p2g_gather<0>, dim3(nblocks(ctx->g.nc)), dim3(kBlock), 0, ctx->stream, A);
+        const dim3 g(nblocks(ctx->g.nc)), b(kBlock);
+        if (ctx->g.two_d) {
+            if (A.apic) hipLaunchKernelGGL((k_p2g_gather<1, 1>), g, b, 0, ctx->stream, A);
+            else hipLaunchKernelGGL((k_p2g_gather<0, 1>), g, b, 0, ctx->stream, A);
+        } else {
+            if (A.apic) hipLaunchKernelGGL((k_p2g_gather<1, 0>), g, b, 0, ctx->stream, A);
+            else hipLaunchKernelGGL((k_p2g_gather<0, 0>), g, b, 0, ctx->stream, A);
+        }
     } else {
         const dim3 g((unsigned)((ctx->g.nz + P2G_TZ - 1) / P2G_TZ), (unsigned)((ctx->g.ny + P2G_TY - 1) / P2G_TY),
                      (unsigned)((ctx->g.nx + P2G_TX - 1) / P2G_TX));
         const dim3 b(P2G_TZ, P2G_TX * P2G_TY);
-        if (A.apic) hipLaunchKernelGGL((k_p2g_tile<1, P2G_TX, P2G_TY>), g, b, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((k_p2g_tile<0, P2G_TX, P2G_TY>), g, b, 0, ctx->stream, A);
+        if (ctx->g.two_d) {
+            if (A.apic) hipLaunchKernelGGL((k_p2g_tile<1, P2G_TX, P2G_TY, 1>), g, b, 0, ctx->stream, A);
+            else hipLaunchKernelGGL((k_p2g_tile<0, P2G_TX, P2G_TY, 1>), g, b, 0, ctx->stream, A);
+        } else {
+            if (A.apic) hipLaunchKernelGGL((k_p2g_tile<1, P2G_TX, P2G_TY, 0>), g, b, 0, ctx->stream, A);
+            else hipLaunchKernelGGL((k_p2g_tile<0, P2G_TX, P2G_TY, 0>), g, b, 0, ctx->stream, A);
+        }
     }
     FSIM_HIP(ctx, hipGetLastError());
     return FSIM_OK;
