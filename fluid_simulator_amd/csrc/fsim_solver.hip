@@ -157,13 +157,18 @@ __device__ void block_finish(double sum, double aux, const Reduce& R) {
     if (threadIdx.x == 0) {
         apply_fin(R.op, lds_s[0], lds_m[0], R.sc);
         __hip_atomic_store(R.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (R.host) {   // the scalars straight into pinned host memory (no copy after the kernel)
-            static_assert(sizeof(SolveScalars) % 8 == 0, "copied as 8-byte words");
-            const uint64_t* src = reinterpret_cast<const uint64_t*>(R.sc);
-            uint64_t* dst = reinterpret_cast<uint64_t*>(R.host);
-            for (int k = 0; k < (int)(sizeof(SolveScalars) / 8); k++)
-                __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        if (R.host) {
+            // the words the host tests after the r update (rr, rmax, nan_flag) straight into pinned
+            // host memory with system-scope stores; the kernel's end and the event after it order
+            // them before the host's read
+            auto put_host = [](void* dst, const void* src) {
+                __hip_atomic_store(reinterpret_cast<uint64_t*>(dst), *reinterpret_cast<const uint64_t*>(src),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            };
+            put_host(&R.host->rr, &R.sc->rr);
+            put_host(&R.host->rmax, &R.sc->rmax);
+            static_assert(offsetof(SolveScalars, pad) == offsetof(SolveScalars, nan_flag) + 4, "one 8-byte word");
+            put_host(&R.host->nan_flag, &R.sc->nan_flag);
         }
     }
 }
@@ -943,7 +948,7 @@ static int pcg_run(fsim_ctx* ctx, const SolveArgs& S, double bnorm, bool flexibl
                                    ctx->chunk_flags, ctx->q, ctx->r, reduce_to_host(ctx, FIN_RES));
         }
         FSIM_HIP(ctx, hipGetLastError());
-        // the r update's finishing block wrote the scalars into scal_host; the event orders the read
+        // the r update's finishing block wrote rr, rmax and nan_flag into scal_host; the event orders the read
         FSIM_HIP(ctx, hipEventRecord(ctx->res_ev, ctx->stream));
         FSIM_TRY(precondition(ctx, S, flexible));
         FSIM_TRY(next_beta(ctx, flexible));
