@@ -130,7 +130,8 @@ PROTOTYPES = {
     "fsim_reset_kernel_times": (C.c_int, [_P]),
 }
 (OPT_SWEEP_MODE, OPT_PROFILE, OPT_LEARNED_SMOOTHING, OPT_LEARNED_LOSS, OPT_LEARNED_CONV, OPT_LEARNED_REFRESH,
- OPT_LEARNED_MIC_COUNT, OPT_MG_COARSE, OPT_RELIABLE_DROP, OPT_MG_GRAPH, OPT_P2G_KERNEL) = range(11)
+ OPT_LEARNED_MIC_COUNT, OPT_MG_COARSE, OPT_RELIABLE_DROP, OPT_MG_GRAPH, OPT_P2G_KERNEL,
+ OPT_WAVE_SEND_DELAY) = range(12)
 MG_COARSE_NETWORK, MG_COARSE_SMOOTH, MG_COARSE_ACCURATE = 0, 1, 2
 
 _lib = None

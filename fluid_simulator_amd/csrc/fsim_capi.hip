@@ -590,6 +590,10 @@ int fsim_set_option(fsim_ctx* ctx, int option, int value) {
         if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "P2G kernel option is 0 or 1");
         ctx->p2g_rows = value;
         return FSIM_OK;
+    case FSIM_OPT_WAVE_SEND_DELAY:
+        if (value < 0 || value > 64) return set_err(ctx, FSIM_ERR_INVALID, "sending-wave delay is 0 .. 64");
+        ctx->wave_send_delay = value;
+        return FSIM_OK;
     case FSIM_OPT_MG_GRAPH:
         if (value != 0 && value != 1) return set_err(ctx, FSIM_ERR_INVALID, "multigrid graph option is 0 or 1");
         ctx->mg_graph = value;

@@ -152,6 +152,7 @@ struct WaveArgs {
     uint32_t epoch;
     int* err;
     const uint8_t* colw;          // [y * nz + z]: 1 if the (y, z) column holds a WATER cell
+    int send_delay;               // test option: s_sleep(127) periods of the sending waves per step
 };
 
 struct GridDims {
@@ -254,6 +255,7 @@ struct fsim_ctx {
     int reliable_drop = 3;                // PCG: true residual after a 10^drop drop of the recursive one (0: at the tolerance)
     int mg_graph = 1;                     // replay the multigrid V-cycle as a HIP graph
     int p2g_rows = 0;                     // 1: the one-row-per-block P2G gather instead of the tiled one
+    int wave_send_delay = 0;              // test option: sending-wave pause per step (s_sleep 127 units)
     int learned_gen = 0;                  // bumped when the network's per-size buffers are reallocated
     bool mg_harvesting = false;           // a solve run by fsim_learned_harvest
     int partials_cap = 0;

@@ -311,6 +311,7 @@ __device__ void send_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K) {
                                    __HIP_MEMORY_SCOPE_AGENT);
         }
         if ((s & 3) == 3 && l == 0) lput32(prog, t);
+        for (int i = 0; i < A.send_delay; i++) __builtin_amdgcn_s_sleep(127);
     }
     if (l == 0) lput32(prog, INT_MAX);
 }
@@ -552,7 +553,6 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             // out-ring slots of steps t .. t + kBpEvery - 1 are free once the sending waves passed
             // step t + kBpEvery - 1 - kRingO (the last wave feeds the +y ring, every wave the +z ring)
             const int t_bo = t + kBpEvery - 1 - kRingO;
-            const bool bo_ok = !bp_step || (spz >= t_bo && (w != NW - 1 || spy >= t_bo));
             // the lane's x-index is in the grid (same for all rows of the wave),
             // and the same test for lane 0 (the -z packet consumer)
             const bool xin = (unsigned)(t - base) < (unsigned)nxi;
@@ -565,8 +565,11 @@ __device__ void compute_wave(const WaveArgs& A, WaveLds<KIND>& S, int J, int K, 
             });
             const bool need_y = cy && act[0];
             // branch-free: per-lane masks, one ballot (the -z packet slots are
-            // read by every lane from one address, so their test is uniform)
+            // read by every lane from one address, so their test is uniform).
+            // The out-ring test reads the sending waves' progress as last read,
+            // so each re-read of the spin loop below can pass it.
             auto ready = [&]() {
+                const bool bo_ok = !bp_step || (spz >= t_bo && (w != NW - 1 || spy >= t_bo));
                 if constexpr (KIND != 0) {
                     const int ta = w > 0 ? t - 1 : t;
                     const uint64_t m_a = ma_all | (ma_row & __builtin_amdgcn_ballot_w64(xin));
@@ -772,6 +775,7 @@ int wave_launch(fsim_ctx* ctx, int kind, const float* in, float* out, double sca
     A.ticket_base = ctx->wave_ticket_base[kind];
     ctx->wave_ticket_base[kind] += ntiles;
     A.epoch = ++ctx->wave_epoch;
+    A.send_delay = ctx->wave_send_delay;
     if (kind == 0) hipLaunchKernelGGL(k_wave<0>, dim3(ntiles), dim3(WF_THREADS), 0, ctx->stream, A);
     else if (kind == 1) hipLaunchKernelGGL(k_wave<1>, dim3(ntiles), dim3(WF_THREADS), 0, ctx->stream, A);
     else hipLaunchKernelGGL(k_wave<2>, dim3(ntiles), dim3(WF_THREADS), 0, ctx->stream, A);

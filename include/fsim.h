@@ -322,7 +322,12 @@ typedef enum fsim_option {
     /* P2G gather: 0 (default) blocks of 2 x 4 cell rows staging each particle
      * row once; 1 one cell row per block (9 stagings per particle).  Both
      * give the same bits. */
-    FSIM_OPT_P2G_KERNEL = 10
+    FSIM_OPT_P2G_KERNEL = 10,
+    /* test option: the sending waves of the wavefront kernels pause about
+     * value x 3.4 us after each step they send (0 .. 64, default 0), so the
+     * compute waves run into the out-ring limit and wait for them.  The
+     * results are the same bits; only the time changes. */
+    FSIM_OPT_WAVE_SEND_DELAY = 11
 } fsim_option;
 int fsim_set_option(fsim_ctx* ctx, int option, int value);
 

@@ -381,6 +381,36 @@ def test_wavefront_equals_planes(gpu, shape, subbox):
     assert np.count_nonzero(out[0][1]) > 0.8 * water
 
 
+@pytest.mark.parametrize("delay", [1, 8])
+def test_wavefront_out_ring_back_pressure(gpu, delay):
+    """The sending waves slowed down (FSIM_OPT_WAVE_SEND_DELAY: a pause of
+    about delay x 3.4 us per step), so the compute waves fill the 32-step
+    out-rings and wait for the sending waves' progress before reusing a slot:
+    the factor and both sweeps still finish, with no boundary-wait timeout,
+    and give the same bits as the undelayed kernels.  The shape has tiles
+    along y and z, so both edges are sent."""
+    shape = (40, 50, 200)
+    nx, ny, nz = shape
+    say(f"case: wavefront out-ring back-pressure {shape}, sending-wave delay {delay}")
+    grid = capi.Grid(nx, ny, nz, 1.0 / 32, 0.25 / 32)
+    rng = np.random.default_rng(7)
+    lab = np.full(shape, capi.AIR, np.uint8)
+    lab[:, : ny * 2 // 3] = capi.WATER
+    lab[rng.random(shape) < 0.05] = capi.SOLID
+    lab[0], lab[-1], lab[:, 0], lab[:, :, 0], lab[:, :, -1] = (capi.SOLID,) * 5
+    vec = rng.standard_normal(grid.nc).astype(np.float32).astype(np.float64)
+    out = {}
+    for d in (0, delay):
+        sim = capi.Simulation(grid, scenes.params())
+        sim.set_option(capi.OPT_WAVE_SEND_DELAY, d)
+        sim.set_grid(capi.FIELD_LABEL, lab.reshape(-1))
+        out[d] = (sim.solver_apply(capi.OP_FACTOR_MIC0, DT), sim.solver_apply(capi.OP_APPLY_MIC0, DT, vec))
+        sim.close()
+    assert np.array_equal(out[0][0], out[delay][0])
+    assert np.array_equal(out[0][1], out[delay][1])
+    assert np.count_nonzero(out[0][1]) > 0.8 * np.count_nonzero(lab == capi.WATER)
+
+
 def test_whole_step_wavefront_equals_planes(gpu):
     say("case: 3 steps 24^3 dam break, wavefront vs planes")
     sc = scenes.dam_break(24, capi.FLIP, sphere=True)
