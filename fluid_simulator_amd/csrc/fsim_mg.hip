@@ -1,5 +1,6 @@
 // fsim_mg.hip — the learned multigrid preconditioner of flexible PCG (no
-// counterpart in the reference; MIC(0)-PCG stays the fallback and the judge).
+// counterpart in the reference; MIC(0)-PCG stays the fallback and the
+// solve its results are measured against).
 //
 // z = M(r) is one V-cycle on a hierarchy of aggregation levels whose
 // coarsest level is solved by the learned network:
