@@ -142,3 +142,19 @@ def test_scene_generator_structure():
     # one particle per sub-cell
     subkey = key * 8 + sub[:, 0] * 4 + sub[:, 1] * 2 + sub[:, 2]
     assert len(np.unique(subkey)) == len(pos)
+
+
+def test_python_constants_match_the_header():
+    """Every enum constant of include/fsim.h that the ctypes binding names has the header's value, and
+    every option, stage, field, operator and status code is bound (an enum value added on one side
+    only would pass the wrong integer through the ABI)."""
+    src = open(HEADER).read()
+    consts = {k: int(v) for k, v in re.findall(r"^\s*FSIM_([A-Z0-9_]+)\s*=\s*(-?\d+)", src, re.M)}
+    assert len(consts) > 60
+    for name, value in consts.items():
+        if hasattr(capi, name):
+            assert getattr(capi, name) == value, name
+    must = [k for k in consts if k.startswith(("OPT_", "STAGE_", "FIELD_", "OP_", "ERR_", "OBS_", "MG_COARSE_"))
+            and not k.startswith(("NUM_",))]
+    assert "OPT_WAVE_SEND_DELAY" in must
+    assert [k for k in must if not hasattr(capi, k)] == []
