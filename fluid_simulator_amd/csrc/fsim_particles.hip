@@ -549,52 +549,105 @@ int particles_reserve(fsim_ctx* ctx, int64_t n) {
 }
 
 // ---- binning: counting sort by grid cell --------------------------------------
+// Each wave takes kKeyChunks consecutive 64-particle chunks: the loads of all
+// of them are issued first and their run counts go out as atomics together,
+// so a wave has several memory round trips in flight instead of one.
+constexpr int kKeyChunks = 4;
+static unsigned key_blocks(int64_t n) {
+    return (unsigned)((n + (int64_t)kBlock * kKeyChunks - 1) / ((int64_t)kBlock * kKeyChunks));
+}
 __global__ void __launch_bounds__(kBlock) k_bin_keys(const float* __restrict__ px, const float* __restrict__ py,
                                                      const float* __restrict__ pz, const uint8_t* __restrict__ removed,
                                                      uint32_t* __restrict__ key, int32_t* __restrict__ rank,
                                                      int32_t* __restrict__ count, int64_t n, double cdinv,
                                                      double cdzinv, int nx, int ny, int nz, uint32_t nc) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = i < n;
-    uint32_t k = 0xffffffffu;   // lanes past n: a key of their own, no count
-    if (in) {
-        if (removed[i]) {
-            k = nc;
-        } else {
-            // ivec3(pos * cellDInv), simulator.cpp:358-359, evaluated in FP64
-            // positions outside the grid (never produced by the step, whose
-            // advection clamps them) are clamped into it so no key leaves [0, Nc)
-            int cx = min(max((int)((double)px[i] * cdinv), 0), nx - 1);
-            int cy = min(max((int)((double)py[i] * cdinv), 0), ny - 1);
-            int cz = min(max((int)((double)pz[i] * cdzinv), 0), nz - 1);
-            k = (uint32_t)(((int64_t)cx * ny + cy) * nz + cz);
+    const int lane = threadIdx.x & 63;
+    const int64_t first = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64 * kKeyChunks) + lane;
+    float x[kKeyChunks], y[kKeyChunks], z[kKeyChunks];
+    uint8_t rm[kKeyChunks];
+#pragma unroll
+    for (int c = 0; c < kKeyChunks; c++) {
+        const int64_t i = first + 64 * c;
+        x[c] = y[c] = z[c] = 0.f;
+        rm[c] = 0;
+        if (i < n) {
+            rm[c] = removed[i];
+            x[c] = px[i];
+            y[c] = py[i];
+            z[c] = pz[i];
         }
-        key[i] = k;
+    }
+    uint32_t k[kKeyChunks];
+#pragma unroll
+    for (int c = 0; c < kKeyChunks; c++) {
+        const int64_t i = first + 64 * c;
+        k[c] = 0xffffffffu;   // lanes past n: a key of their own, no count
+        if (i < n) {
+            if (rm[c]) {
+                k[c] = nc;
+            } else {
+                // ivec3(pos * cellDInv), simulator.cpp:358-359, evaluated in FP64
+                // positions outside the grid (never produced by the step, whose
+                // advection clamps them) are clamped into it so no key leaves [0, Nc)
+                const int cx = min(max((int)((double)x[c] * cdinv), 0), nx - 1);
+                const int cy = min(max((int)((double)y[c] * cdinv), 0), ny - 1);
+                const int cz = min(max((int)((double)z[c] * cdzinv), 0), nz - 1);
+                k[c] = (uint32_t)(((int64_t)cx * ny + cy) * nz + cz);
+            }
+            key[i] = k[c];
+        }
     }
     // The particles arrive sorted by cell (the previous bin), so neighbouring
-    // lanes share keys: each run of equal keys takes one atomicAdd of its
-    // length and its lanes number themselves from the returned base.  The
-    // rank inside a cell is provisional (the segment sort fixes the order).
-    const int lane = threadIdx.x & 63;
-    const uint32_t kprev = (uint32_t)__shfl_up((int)k, 1, 64);
-    const bool head = lane == 0 || kprev != k;
-    const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+    // lanes share keys: each run of equal keys in a chunk takes one atomicAdd
+    // of its length and its lanes number themselves from the returned base.
+    // The rank inside a cell is provisional (the segment sort fixes the order).
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-    const int s = 63 - __clzll(heads & upto);   // this lane's run head
-    const uint64_t later = heads & ~upto;
-    const int e = later ? __ffsll((long long)later) - 1 : 64;   // next run head
-    int base = 0;
-    if (head && in) base = atomicAdd(&count[k], e - s);
-    base = __shfl(base, s, 64);
-    if (in) rank[i] = base + (lane - s);
+    int s[kKeyChunks], base[kKeyChunks];
+#pragma unroll
+    for (int c = 0; c < kKeyChunks; c++) {
+        const uint32_t kprev = (uint32_t)__shfl_up((int)k[c], 1, 64);
+        const bool head = lane == 0 || kprev != k[c];
+        const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+        s[c] = 63 - __clzll(heads & upto);   // this lane's run head
+        const uint64_t later = heads & ~upto;
+        const int e = later ? __ffsll((long long)later) - 1 : 64;   // next run head
+        base[c] = 0;
+        if (head && first + 64 * c < n) base[c] = atomicAdd(&count[k[c]], e - s[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < kKeyChunks; c++) {
+        const int b = __shfl(base[c], s[c], 64);
+        const int64_t i = first + 64 * c;
+        if (i < n) rank[i] = b + (lane - s[c]);
+    }
 }
 
+// kKeyChunks particles per thread (a wave's chunks of 64 consecutive ones),
+// each step's loads issued for all of them before the next step's
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(const uint32_t* __restrict__ key, const int32_t* __restrict__ rank,
                                                         const int32_t* __restrict__ start, int32_t* __restrict__ perm,
                                                         int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    perm[start[key[i]] + rank[i]] = (int32_t)i;
+    const int64_t first =
+        ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64 * kKeyChunks) + (threadIdx.x & 63);
+    uint32_t k[kKeyChunks];
+    int32_t r[kKeyChunks], st[kKeyChunks];
+#pragma unroll
+    for (int c = 0; c < kKeyChunks; c++) {
+        const int64_t i = first + 64 * c;
+        k[c] = 0;
+        r[c] = 0;
+        if (i < n) {
+            k[c] = key[i];
+            r[c] = rank[i];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kKeyChunks; c++) st[c] = first + 64 * c < n ? start[k[c]] : 0;
+#pragma unroll
+    for (int c = 0; c < kKeyChunks; c++) {
+        const int64_t i = first + 64 * c;
+        if (i < n) perm[st[c] + r[c]] = (int32_t)i;
+    }
 }
 
 // Canonical order inside a cell: ascending particle id.  A cell of up to
@@ -777,7 +830,7 @@ int stage_bin(fsim_ctx* ctx) {
     FSIM_TRY(fill_i32(ctx, ctx->cell_count, 0, nc + 1));
     if (n > 0) {
         FSIM_PROF(ctx, KID_BIN_KEYS);
-        hipLaunchKernelGGL(k_bin_keys, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->P.pos[0], ctx->P.pos[1],
+        hipLaunchKernelGGL(k_bin_keys, dim3(key_blocks(n)), dim3(kBlock), 0, ctx->stream, ctx->P.pos[0], ctx->P.pos[1],
                            ctx->P.pos[2], ctx->removed, ctx->key, ctx->rank, ctx->cell_count, n, ctx->g.cdinv,
                            ctx->g.cdzinv, ctx->g.nx, ctx->g.ny, ctx->g.nz, (uint32_t)nc);
         FSIM_HIP(ctx, hipGetLastError());
@@ -789,7 +842,7 @@ int stage_bin(fsim_ctx* ctx) {
     if (n == 0) return FSIM_OK;
     {
     FSIM_PROF(ctx, KID_BIN_SCATTER);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, ctx->key, ctx->rank,
+    hipLaunchKernelGGL(k_bin_scatter, dim3(key_blocks(n)), dim3(kBlock), 0, ctx->stream, ctx->key, ctx->rank,
                        ctx->cell_start, ctx->perm, n);
     }
     {
