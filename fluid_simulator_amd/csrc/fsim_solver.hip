@@ -665,6 +665,9 @@ __global__ void __launch_bounds__(kBlock) k_apply_pressure(SolveArgs S, const ui
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S.nc) return;
     const int cz = (int)(i % S.nz), cy = (int)((i / S.nz) % S.ny), cx = (int)(i / ((int64_t)S.nz * S.ny));
+    // only interior cells are WATER (their six neighbours exist); the test keeps
+    // the accesses below in range whatever the mask holds
+    if (cx < 1 || cy < 1 || cz < 1 || cx > S.nx - 2 || cy > S.ny - 2 || cz > S.nz - 2) return;
     const int64_t ri = rsidx(S, cx, cy, cz);
     if (!(nbinfo[ri] & NB_FLUID)) return;
     const int64_t st[3] = {(int64_t)S.ny * S.nz, S.nz, 1};
